@@ -1,0 +1,1031 @@
+// Batched SQP solver for one MI355X: kernels, host loop and the C-ABI of libgato_mi355x.so.
+//
+// Two execution paths share every phase routine (sqp_phases.cuh):
+//   GATO_MODE_PLAIN  one kernel per phase (KKT, Schur, PCG, dz, check, merit, line search) and a
+//                    host loop with a device->host read of the converged count every iteration,
+//                    the structure of the reference's BSQP::solve (gato/bsqp/bsqp.cuh:103-197);
+//   GATO_MODE_FUSED  one persistent kernel, one work-group per problem, running the initial merit,
+//                    every SQP iteration and the final merit on the device with no host round trip.
+// A work-group owns one problem; its 16-lane knot groups each own one knot (looping when N > 32).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gato_mi355x.h"
+#include "sqp_phases.cuh"
+
+namespace gato {
+
+struct Args {
+    int N, B, traj, max_sqp, max_pcg, exit_rule, adapt_rho;
+    float dt, kkt_tol, solve_ratio;
+    CostW cw;
+    float* xu;
+    const float *xs, *ref, *fext, *Qd, *Rd, *xref;
+    float *lam, *rho, *drho;
+    const float *drho_init, *mu, *tol;
+    float *A, *Bm, *c, *Qp, *Qv, *Rg, *q, *r, *Qip, *Qiv, *Ri, *Sd, *So, *Pd, *Po, *gam, *dz, *mpart, *phi;
+    int *conv, *active, *solved;
+    int* st_pcg;
+    float *st_kres, *st_lsm, *st_step;
+    int* st_sqp;
+    float *st_init, *st_final;
+};
+
+template <int PL>
+struct View {  // per-problem pointers
+    static constexpr int NQ = Plant<PL>::NQ, NX = Plant<PL>::NX, NU = Plant<PL>::NU;
+    float* xu;
+    const float *xs, *ref, *fext, *Qd, *Rd, *xref;
+    float *lam, *A, *Bm, *c, *Qp, *Qv, *Rg, *q, *r, *Qip, *Qiv, *Ri, *Sd, *So, *Pd, *Po, *gam, *dz, *mpart;
+    __device__ View(const Args& a, int b) {
+        const int N = a.N;
+        xu = a.xu + (size_t)b * a.traj;
+        xs = a.xs + (size_t)b * NX;
+        ref = a.ref + (size_t)b * 6 * N;
+        fext = a.fext + (size_t)b * 6;
+        Qd = a.Qd + (size_t)b * NX;
+        Rd = a.Rd + (size_t)b * NU;
+        xref = a.xref ? a.xref + (size_t)b * N * NX : nullptr;
+        lam = a.lam + (size_t)b * N * NX;
+        A = a.A + (size_t)b * N * NX * NX;
+        Bm = a.Bm + (size_t)b * N * NX * NU;
+        c = a.c + (size_t)b * N * NX;
+        Qp = a.Qp + (size_t)b * N * NQ * NQ;
+        Qv = a.Qv + (size_t)b * N * NQ;
+        Rg = a.Rg + (size_t)b * N * NU;
+        q = a.q + (size_t)b * N * NX;
+        r = a.r + (size_t)b * N * NU;
+        Qip = a.Qip + (size_t)b * N * NQ * NQ;
+        Qiv = a.Qiv + (size_t)b * N * NQ;
+        Ri = a.Ri + (size_t)b * N * NU;
+        Sd = a.Sd + (size_t)b * N * NX * NX;
+        So = a.So + (size_t)b * N * NX * NX;
+        Pd = a.Pd + (size_t)b * N * NX * NX;
+        Po = a.Po + (size_t)b * N * NX * NX;
+        gam = a.gam + (size_t)b * N * NX;
+        dz = a.dz + (size_t)b * (a.traj + NU);
+        mpart = a.mpart + (size_t)b * NUM_ALPHAS * N;
+    }
+};
+
+// ------------------------------------------------------------------ phases on one problem
+template <int PL>
+__device__ void phase_lin(const Args& a, int b, float* sm) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
+    View<PL> v(a, b);
+    const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
+    float* shg = sm + g * 4 * NQ;
+    const float* xt = v.xu + (size_t)(a.cw.terminal_mode ? N - 1 : N - 2) * (NX + NU);
+    for (int k = g; k < N; k += G) {
+        const float* xk = v.xu + (size_t)(k < N - 1 ? k : 0) * (NX + NU);
+        lin_knot<PL>(k, N, lane, a.dt, a.cw, xk, xt, v.ref + 6 * k, v.xref ? v.xref + k * NX : nullptr, v.Qd, v.Rd,
+                     v.fext, v.xu, v.xs, v.A + (size_t)k * NX * NX, v.Bm + (size_t)k * NX * NU, v.c + (k + 1) * NX,
+                     v.c, v.Qp + k * NQ * NQ, v.Qv + k * NQ, v.Rg + k * NU, v.q + k * NX, v.r + k * NU, shg);
+    }
+}
+
+template <int PL>
+__device__ void phase_schur(const Args& a, int b, float rho, float* sm) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
+    View<PL> v(a, b);
+    const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
+    float* piv = sm + g * (2 * NX + NX * NX);
+    float* Tsh = piv + 2 * NX;
+    for (int k = g; k < N; k += G)
+        invert_q_knot<PL>(lane, k < N - 1, rho, v.Qp + k * NQ * NQ, v.Qv + k * NQ, v.Rg + k * NU,
+                          v.Qip + k * NQ * NQ, v.Qiv + k * NQ, v.Ri + k * NU, piv);
+    __syncthreads();
+    for (int k = g; k < N; k += G) {
+        if (k < N - 1) {
+            float phi[NX], prow[2 * NX];
+            schur_row<PL>(lane, rho, v.A + (size_t)k * NX * NX, v.Bm + (size_t)k * NX * NU, v.Qip + k * NQ * NQ,
+                          v.Qiv + k * NQ, v.Ri + k * NU, v.Qip + (k + 1) * NQ * NQ, v.Qiv + (k + 1) * NQ,
+                          v.q + k * NX, v.q + (k + 1) * NX, v.r + k * NU, v.c + (k + 1) * NX,
+                          v.Sd + (size_t)(k + 1) * NX * NX, v.So + (size_t)(k + 1) * NX * NX,
+                          v.Pd + (size_t)(k + 1) * NX * NX, v.gam + (k + 1) * NX, piv, phi, prow);
+        } else {
+            schur_row0<PL>(lane, rho, v.Qp, v.Qv, v.Qip, v.Qiv, v.q, v.c, v.Sd, v.Pd, v.gam);
+        }
+    }
+    __syncthreads();
+    for (int k = g; k < N - 1; k += G) {
+        const int rr = lane < NX ? lane : 0;
+        float phi[NX], prow[NX];
+        const float* So1 = v.So + (size_t)(k + 1) * NX * NX;
+        const float* Pd1 = v.Pd + (size_t)(k + 1) * NX * NX;
+#pragma unroll
+        for (int m = 0; m < NX; m++) {
+            phi[m] = So1[rr * NX + m];
+            prow[m] = Pd1[rr * NX + m];
+        }
+        precond_offdiag<PL>(lane, phi, prow, v.Pd + (size_t)k * NX * NX, Tsh, v.Po + (size_t)(k + 1) * NX * NX);
+    }
+}
+
+// y = (D, L) block-tridiagonal times the padded LDS vector x at row (k, l)
+template <int NX>
+__device__ __forceinline__ float btd_row(const float* D, const float* L, const float* x, int N, int k, int l) {
+    constexpr int XP = 16;
+    const float* Dr = D + (size_t)k * NX * NX + l * NX;
+    const float* xk = x + (k + 1) * XP;
+    float s = 0.f;
+#pragma unroll
+    for (int c = 0; c < NX; c++) s += Dr[c] * xk[c];
+    if (k > 0) {
+        const float* Lr = L + (size_t)k * NX * NX + l * NX;
+        const float* xm = x + k * XP;
+#pragma unroll
+        for (int c = 0; c < NX; c++) s += Lr[c] * xm[c];
+    }
+    if (k < N - 1) {
+        const float* Lc = L + (size_t)(k + 1) * NX * NX + l;
+        const float* xp = x + (k + 2) * XP;
+#pragma unroll
+        for (int c = 0; c < NX; c++) s += Lc[c * NX] * xp[c];
+    }
+    return s;
+}
+
+// Warm-started PCG on S lambda = gamma (pcg.cuh:13-148). Returns the iteration count.
+template <int PL, int NPASS>
+__device__ int phase_pcg(const Args& a, int b, float eps, float* sm) {
+    constexpr int NX = Plant<PL>::NX, XP = 16;
+    View<PL> v(a, b);
+    const int N = a.N, tid = threadIdx.x, g = tid / KG, lane = tid % KG, G = blockDim.x / KG;
+    float* sp = sm;
+    float* sr = sp + (N + 2) * XP;
+    float* red = sr + (N + 2) * XP;
+    for (int i = tid; i < 2 * (N + 2) * XP; i += blockDim.x) sm[i] = 0.f;
+    __syncthreads();
+    float x[NPASS], r[NPASS], z[NPASS], p[NPASS], Ap[NPASS];
+    bool val[NPASS];
+    int kk[NPASS];
+#pragma unroll
+    for (int s = 0; s < NPASS; s++) {
+        kk[s] = g + s * G;
+        val[s] = (lane < NX) && (kk[s] < N);
+        x[s] = val[s] ? v.lam[kk[s] * NX + lane] : 0.f;
+        if (val[s]) sp[(kk[s] + 1) * XP + lane] = x[s];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NPASS; s++) r[s] = val[s] ? v.gam[kk[s] * NX + lane] - btd_row<NX>(v.Sd, v.So, sp, N, kk[s], lane) : 0.f;
+#pragma unroll
+    for (int s = 0; s < NPASS; s++)
+        if (val[s]) sr[(kk[s] + 1) * XP + lane] = r[s];
+    __syncthreads();
+    float loc = 0.f;
+#pragma unroll
+    for (int s = 0; s < NPASS; s++) {
+        z[s] = val[s] ? btd_row<NX>(v.Pd, v.Po, sr, N, kk[s], lane) : 0.f;
+        p[s] = z[s];
+        loc += r[s] * z[s];
+    }
+    float rho = block_sum(loc, red);
+    if (fabsf(rho) < PCG_ABS_TOL) return 0;
+    const float rho0 = fabsf(rho);
+    int it = 0;
+    for (int i = 0; i < a.max_pcg; i++) {
+        it++;
+#pragma unroll
+        for (int s = 0; s < NPASS; s++)
+            if (val[s]) sp[(kk[s] + 1) * XP + lane] = p[s];
+        __syncthreads();
+        loc = 0.f;
+#pragma unroll
+        for (int s = 0; s < NPASS; s++) {
+            Ap[s] = val[s] ? btd_row<NX>(v.Sd, v.So, sp, N, kk[s], lane) : 0.f;
+            loc += p[s] * Ap[s];
+        }
+        const float alpha = rho / block_sum(loc, red);
+#pragma unroll
+        for (int s = 0; s < NPASS; s++) {
+            x[s] += alpha * p[s];
+            r[s] -= alpha * Ap[s];
+            if (val[s]) sr[(kk[s] + 1) * XP + lane] = r[s];
+        }
+        __syncthreads();
+        loc = 0.f;
+#pragma unroll
+        for (int s = 0; s < NPASS; s++) {
+            z[s] = val[s] ? btd_row<NX>(v.Pd, v.Po, sr, N, kk[s], lane) : 0.f;
+            loc += r[s] * z[s];
+        }
+        const float rn = block_sum(loc, red);
+        if (fabsf(rn) < PCG_ABS_TOL + eps * rho0) break;
+        const float beta = rn / rho;
+        rho = rn;
+#pragma unroll
+        for (int s = 0; s < NPASS; s++) p[s] = z[s] + beta * p[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NPASS; s++)
+        if (val[s]) v.lam[kk[s] * NX + lane] = x[s];
+    return it;
+}
+
+template <int PL>
+__device__ float phase_dz(const Args& a, int b, float* sm) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
+    View<PL> v(a, b);
+    const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
+    float* shv = sm + g * KG;
+    float* red = sm + G * KG;
+    float res = 0.f;
+    for (int k = g; k < N; k += G) {
+        const float* lam1 = v.lam + (k < N - 1 ? (k + 1) * NX : 0);
+        res = fmaxf(res, dz_knot<PL>(k, N, lane, v.A + (size_t)k * NX * NX, v.Bm + (size_t)k * NX * NU,
+                                     v.Qip + k * NQ * NQ, v.Qiv + k * NQ, v.Ri + k * NU, v.q + k * NX, v.r + k * NU,
+                                     v.c + k * NX, v.lam + k * NX, lam1, v.dz + (size_t)k * (NX + NU), shv));
+    }
+    return block_max(res, red);
+}
+
+// merit at nal step lengths alpha_j = 2^-j (merit.cuh:16-92): per-(alpha, knot) partial terms,
+// then fixed-order sums over knots. out[j] (LDS) receives phi(alpha_j). The reference's
+// initial / final merit (dz = 0, one alpha) is nal = 1.
+template <int PL>
+__device__ void phase_merit(const Args& a, int b, float mu, int nal, float* out) {
+    using P = Plant<PL>;
+    constexpr int NX = P::NX, NU = P::NU;
+    View<PL> v(a, b);
+    const int N = a.N;
+    for (int it = threadIdx.x; it < N * nal; it += blockDim.x) {
+        const int k = it % N, j = it / N;
+        const float alpha = 1.f / (float)(1 << j);
+        const size_t off = (size_t)k * (NX + NU);
+        v.mpart[j * N + k] = merit_item<PL>(k, N, alpha, a.dt, a.cw, mu, v.xu + off, v.dz + off, v.ref + 6 * k,
+                                            v.xref ? v.xref + k * NX : nullptr, v.Qd, v.Rd, v.fext, v.xu, v.dz,
+                                            v.xs);
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)nal) {
+        float s = 0.f;
+        for (int k = 0; k < N; k++) s += v.mpart[threadIdx.x * N + k];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+// line search + rho adaptation + update (line_search.cuh:12-98). merits in LDS; returns via shared.
+template <int PL>
+__device__ void phase_ls(const Args& a, int b, int it, const float* merits, float* shs) {
+    constexpr int NX = Plant<PL>::NX, NU = Plant<PL>::NU;
+    View<PL> v(a, b);
+    if (threadIdx.x == 0) {
+        float best = 1e38f;
+        int jb = 0;
+        for (int j = 0; j < NUM_ALPHAS; j++)
+            if (merits[j] < best) { best = merits[j]; jb = j; }
+        const bool ok = best < a.phi[b];
+        if (a.adapt_rho) {
+            const float mult = ok ? fminf(a.drho[b] / RHO_FACTOR, 1.f / RHO_FACTOR) : fmaxf(a.drho[b] * RHO_FACTOR, RHO_FACTOR);
+            a.drho[b] = mult;
+            a.rho[b] = fminf(fmaxf(a.rho[b] * mult, RHO_MIN), RHO_MAX);
+        }
+        float step = -1.f;
+        if (!ok) {
+            if (a.rho[b] > RHO_MAX) a.rho[b] = RHO_INIT;
+        } else {
+            step = 1.f / (float)(1 << jb);
+            a.phi[b] = best;
+        }
+        a.st_step[it * a.B + b] = step;
+        a.st_lsm[it * a.B + b] = a.phi[b];
+        shs[0] = step;
+    }
+    __syncthreads();
+    const float step = shs[0];
+    if (step > 0.f)
+        for (int i = threadIdx.x; i < a.traj; i += blockDim.x) v.xu[i] += step * v.dz[i];
+    (void)NX; (void)NU;
+}
+
+// ------------------------------------------------------------------ plain path kernels
+template <int PL>
+__global__ void __launch_bounds__(512) k_merit(Args a, int nal, int which /*0 init, 1 final, 2 line search*/,
+                                               int it) {
+    extern __shared__ float sm[];
+    const int b = blockIdx.x;
+    if (which == 2 && a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
+    phase_merit<PL>(a, b, a.mu[b], nal, sm);
+    if (threadIdx.x == 0 && which < 2) {
+        a.phi[b] = sm[0];
+        (which == 0 ? a.st_init : a.st_final)[b] = sm[0];
+    }
+    if (which == 2) phase_ls<PL>(a, b, it, sm, sm + NUM_ALPHAS);
+}
+
+template <int PL>
+__global__ void __launch_bounds__(512) k_lin(Args a) {
+    extern __shared__ float sm[];
+    const int b = blockIdx.x;
+    if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
+    phase_lin<PL>(a, b, sm);
+}
+
+template <int PL>
+__global__ void __launch_bounds__(512) k_schur(Args a) {
+    extern __shared__ float sm[];
+    const int b = blockIdx.x;
+    if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
+    phase_schur<PL>(a, b, a.rho[b], sm);
+}
+
+template <int PL, int NPASS>
+__global__ void __launch_bounds__(512) k_pcg(Args a, int it) {
+    extern __shared__ float sm[];
+    const int b = blockIdx.x;
+    const bool skip = (a.exit_rule == GATO_EXIT_KKT) ? !a.active[b] : a.conv[b];  // quirk Q9 (pcg.cuh:30-33)
+    int n = 0;
+    if (!skip) n = phase_pcg<PL, NPASS>(a, b, a.tol[b], sm);
+    if (threadIdx.x == 0) a.st_pcg[it * a.B + b] = n;
+}
+
+template <int PL>
+__global__ void __launch_bounds__(512) k_dz(Args a, int it) {
+    extern __shared__ float sm[];
+    const int b = blockIdx.x;
+    if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) {
+        if (threadIdx.x == 0) a.st_kres[it * a.B + b] = 0.f;
+        return;
+    }
+    const float kr = phase_dz<PL>(a, b, sm);
+    if (threadIdx.x == 0) a.st_kres[it * a.B + b] = kr;
+}
+
+// host-side convergence bookkeeping of bsqp.cuh:142-165, per problem, plus the converged count
+__global__ void k_check(Args a, int it) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.B) return;
+    int conv = a.conv[b];
+    if (a.exit_rule == GATO_EXIT_PCG0) {
+        if (a.st_pcg[it * a.B + b] == 0) conv = 1;
+        a.st_sqp[b] += 1;  // quirk Q10: batch-uniform count
+    } else if (a.active[b]) {
+        a.st_sqp[b] += 1;
+        if (a.st_kres[it * a.B + b] <= a.kkt_tol) { conv = 1; a.active[b] = 0; }
+    }
+    if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) {  // frozen: no line search this iteration
+        a.st_step[it * a.B + b] = 0.f;
+        a.st_lsm[it * a.B + b] = a.phi[b];
+    }
+    a.conv[b] = conv;
+    if (conv) atomicAdd(&a.solved[it], 1);
+}
+
+__global__ void k_begin(Args a) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.B) return;
+    a.conv[b] = 0;
+    a.active[b] = 1;
+    a.st_sqp[b] = 0;
+    if (b == 0)
+        for (int i = 0; i < a.max_sqp; i++) a.solved[i] = 0;
+    for (int i = 0; i < a.max_sqp; i++) {
+        a.st_pcg[i * a.B + b] = 0;
+        a.st_kres[i * a.B + b] = 0.f;
+        a.st_lsm[i * a.B + b] = 0.f;
+        a.st_step[i * a.B + b] = 0.f;
+    }
+}
+
+__global__ void k_zero(float* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.f;
+}
+
+__global__ void k_end(Args a) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= a.B) return;
+    a.drho[b] = a.drho_init[b];  // bsqp.cuh:189
+}
+
+// ------------------------------------------------------------------ fused persistent kernel
+// One work-group per problem; problems are independent (a converged problem stops iterating;
+// the reference instead keeps stepping converged problems until the whole batch has converged).
+template <int PL, int NPASS>
+__global__ void __launch_bounds__(512) k_fused(Args a) {
+    extern __shared__ float sm[];
+    __shared__ float shs[16];
+    __shared__ int shi[4];
+    constexpr int NX = Plant<PL>::NX, NU = Plant<PL>::NU;
+    for (int b = blockIdx.x; b < a.B; b += gridDim.x) {
+        View<PL> v(a, b);
+        const float mu = a.mu[b], eps = a.tol[b];
+        for (int i = threadIdx.x; i < a.traj + NU; i += blockDim.x) v.dz[i] = 0.f;
+        __syncthreads();
+        phase_merit<PL>(a, b, mu, 1, shs);
+        if (threadIdx.x == 0) {
+            a.phi[b] = shs[0];
+            a.st_init[b] = shs[0];
+            a.st_sqp[b] = 0;
+        }
+        __syncthreads();
+        int it = 0;
+        for (; it < a.max_sqp; it++) {
+            phase_lin<PL>(a, b, sm);
+            __syncthreads();
+            phase_schur<PL>(a, b, a.rho[b], sm);
+            __syncthreads();
+            const int n = phase_pcg<PL, NPASS>(a, b, eps, sm);
+            __syncthreads();
+            const float kr = phase_dz<PL>(a, b, sm);
+            if (threadIdx.x == 0) {
+                a.st_pcg[it * a.B + b] = n;
+                a.st_kres[it * a.B + b] = kr;
+                a.st_sqp[b] = it + 1;
+                const int conv = (a.exit_rule == GATO_EXIT_PCG0) ? (n == 0) : (kr <= a.kkt_tol);
+                a.conv[b] = conv;
+                shi[0] = conv;
+            }
+            __syncthreads();
+            if (shi[0]) break;
+            phase_merit<PL>(a, b, mu, NUM_ALPHAS, shs);
+            phase_ls<PL>(a, b, it, shs, shs + NUM_ALPHAS);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            for (int j = it + (it < a.max_sqp ? 1 : 0); j < a.max_sqp; j++) {
+                a.st_pcg[j * a.B + b] = 0;
+                a.st_kres[j * a.B + b] = 0.f;
+            }
+            for (int j = it; j < a.max_sqp; j++) {
+                a.st_step[j * a.B + b] = 0.f;
+                a.st_lsm[j * a.B + b] = a.phi[b];
+            }
+        }
+        for (int i = threadIdx.x; i < a.traj + NU; i += blockDim.x) v.dz[i] = 0.f;
+        __syncthreads();
+        phase_merit<PL>(a, b, mu, 1, shs);
+        if (threadIdx.x == 0) {
+            a.st_final[b] = shs[0];
+            a.drho[b] = a.drho_init[b];
+        }
+        __syncthreads();
+    }
+    (void)NX;
+}
+
+// ------------------------------------------------------------------ sim_forward (sim.cuh:14-48)
+template <int PL>
+__global__ void k_sim(float* xn, const float* x, const float* u, const float* fext, float dt, int B) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    float q[NQ], qd[NQ], uu[NQ], qdd[NQ], L[NQ * (NQ + 1) / 2];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) { q[i] = x[i]; qd[i] = x[NQ + i]; uu[i] = u[i]; }
+    Kin<PL> K;
+    K.set(q);
+    forward_dynamics<PL>(K, qd, uu, fext + 6 * b, qdd, L);
+#pragma unroll
+    for (int i = 0; i < NQ; i++) {
+        xn[b * 2 * NQ + NQ + i] = qd[i] + dt * qdd[i];
+        xn[b * 2 * NQ + i] = q[i] + dt * qd[i] + 0.5f * qdd[i] * dt * dt;
+    }
+}
+
+}  // namespace gato
+
+// =================================================================== host side
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return fail((int)e_, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int plant_nq(int plant) { return plant == GATO_PLANT_IIWA14 ? 7 : plant == GATO_PLANT_INDY7 ? 6 : -1; }
+
+}  // namespace
+
+struct gato_solver {
+    int plant, N, B, nq, nx, nu, traj;
+    gato_params prm;
+    int adapt_rho = 1;
+    char* pool = nullptr;
+    size_t pool_bytes = 0;
+    gato::Args a{};
+    float *d_fext, *d_Qd, *d_Rd, *d_xref;
+    float *d_rho, *d_drho, *d_drho_init, *d_mu, *d_tol;
+    std::vector<float> h_rho_init, h_drho_init;
+    bool joint_cost = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_kernel_us = 0.0;
+    int last_launches = 0;
+    // host copies of device stats
+    std::vector<int> h_pcg, h_sqp, h_conv;
+    std::vector<float> h_kres, h_lsm, h_step, h_init, h_final;
+};
+
+namespace {
+
+template <typename T>
+T* carve(gato_solver* s, size_t& off, size_t count) {
+    off = (off + 255) & ~size_t(255);
+    T* p = reinterpret_cast<T*>(s->pool + off);
+    off += count * sizeof(T);
+    return p;
+}
+
+size_t layout(gato_solver* s, bool assign) {
+    const size_t N = s->N, B = s->B, nx = s->nx, nu = s->nu, nq = s->nq;
+    size_t off = 0;
+    auto F = [&](size_t n) -> float* { float* p = carve<float>(s, off, n); return assign ? p : nullptr; };
+    auto I = [&](size_t n) -> int* { int* p = carve<int>(s, off, n); return assign ? p : nullptr; };
+    gato::Args& a = s->a;
+    float* t;
+    t = F(B * 6); if (assign) s->d_fext = t;
+    t = F(B * nx); if (assign) s->d_Qd = t;
+    t = F(B * nu); if (assign) s->d_Rd = t;
+    t = F(B * N * nx); if (assign) s->d_xref = t;
+    t = F(B * N * nx); if (assign) a.lam = t;
+    t = F(B); if (assign) s->d_rho = t;
+    t = F(B); if (assign) s->d_drho = t;
+    t = F(B); if (assign) s->d_drho_init = t;
+    t = F(B); if (assign) s->d_mu = t;
+    t = F(B); if (assign) s->d_tol = t;
+    t = F(B * N * nx * nx); if (assign) a.A = t;
+    t = F(B * N * nx * nu); if (assign) a.Bm = t;
+    t = F(B * N * nx); if (assign) a.c = t;
+    t = F(B * N * nq * nq); if (assign) a.Qp = t;
+    t = F(B * N * nq); if (assign) a.Qv = t;
+    t = F(B * N * nu); if (assign) a.Rg = t;
+    t = F(B * N * nx); if (assign) a.q = t;
+    t = F(B * N * nu); if (assign) a.r = t;
+    t = F(B * N * nq * nq); if (assign) a.Qip = t;
+    t = F(B * N * nq); if (assign) a.Qiv = t;
+    t = F(B * N * nu); if (assign) a.Ri = t;
+    t = F(B * N * nx * nx); if (assign) a.Sd = t;
+    t = F(B * N * nx * nx); if (assign) a.So = t;
+    t = F(B * N * nx * nx); if (assign) a.Pd = t;
+    t = F(B * N * nx * nx); if (assign) a.Po = t;
+    t = F(B * N * nx); if (assign) a.gam = t;
+    t = F(B * (s->traj + nu)); if (assign) a.dz = t;
+    t = F(B * gato::NUM_ALPHAS * N); if (assign) a.mpart = t;
+    t = F(B); if (assign) a.phi = t;
+    int* ip;
+    ip = I(B); if (assign) a.conv = ip;
+    ip = I(B); if (assign) a.active = ip;
+    ip = I(std::max<size_t>(B, s->prm.max_sqp_iters)); if (assign) a.solved = ip;
+    const size_t it = std::max<uint32_t>(1u, s->prm.max_sqp_iters);
+    ip = I(it * B); if (assign) a.st_pcg = ip;
+    t = F(it * B); if (assign) a.st_kres = t;
+    t = F(it * B); if (assign) a.st_lsm = t;
+    t = F(it * B); if (assign) a.st_step = t;
+    ip = I(B); if (assign) a.st_sqp = ip;
+    t = F(B); if (assign) a.st_init = t;
+    t = F(B); if (assign) a.st_final = t;
+    return off + 256;
+}
+
+int grid_threads(int N) { return gato::KG * std::min(N, 32); }
+int npass(int N) { return (N + 31) / 32; }
+
+size_t lds_bytes(const gato_solver* s) {
+    const int G = std::min(s->N, 32), nx = s->nx, nq = s->nq;
+    size_t f = 0;
+    f = std::max<size_t>(f, (size_t)G * 4 * nq);                   // lin
+    f = std::max<size_t>(f, (size_t)G * (2 * nx + nx * nx));        // schur
+    f = std::max<size_t>(f, (size_t)2 * (s->N + 2) * 16 + 16);      // pcg
+    f = std::max<size_t>(f, (size_t)G * gato::KG + 16);             // dz
+    f = std::max<size_t>(f, (size_t)2 * gato::NUM_ALPHAS + 16);     // merit + ls
+    return f * sizeof(float);
+}
+
+void fill_args(gato_solver* s, float* d_xu, const float* d_xs, const float* d_ref, float dt) {
+    gato::Args& a = s->a;
+    a.N = s->N;
+    a.B = s->B;
+    a.traj = s->traj;
+    a.max_sqp = s->prm.max_sqp_iters;
+    a.max_pcg = s->prm.max_pcg_iters;
+    a.exit_rule = s->prm.exit_rule;
+    a.adapt_rho = s->adapt_rho;
+    a.dt = dt;
+    a.kkt_tol = s->prm.kkt_tol;
+    a.solve_ratio = s->prm.solve_ratio;
+    a.cw = gato::CostW{s->prm.q_cost, s->prm.N_cost, s->prm.q_lim_cost, s->prm.vel_lim_cost, s->prm.ctrl_lim_cost,
+                       s->prm.terminal_mode, s->prm.hess_mode};
+    a.xu = d_xu;
+    a.xs = d_xs;
+    a.ref = d_ref;
+    a.fext = s->d_fext;
+    a.Qd = s->d_Qd;
+    a.Rd = s->d_Rd;
+    a.xref = s->joint_cost ? s->d_xref : nullptr;
+    a.rho = s->d_rho;
+    a.drho = s->d_drho;
+    a.drho_init = s->d_drho_init;
+    a.mu = s->d_mu;
+    a.tol = s->d_tol;
+}
+
+template <int PL>
+int launch_solve(gato_solver* s, hipStream_t st) {
+    const int B = s->B, N = s->N;
+    const dim3 blk(grid_threads(N));
+    const size_t lds = lds_bytes(s);
+    gato::Args& a = s->a;
+    const int sb = (B + 255) / 256;
+    hipLaunchKernelGGL(gato::k_begin, dim3(sb), dim3(256), 0, st, a);
+    HIP_TRY(hipGetLastError());
+    const int np = npass(N);
+    if (s->prm.mode == GATO_MODE_FUSED) {
+        HIP_TRY(hipEventRecord(s->ev0, st));
+        if (np == 1) hipLaunchKernelGGL((gato::k_fused<PL, 1>), dim3(B), blk, lds, st, a);
+        else if (np == 2) hipLaunchKernelGGL((gato::k_fused<PL, 2>), dim3(B), blk, lds, st, a);
+        else hipLaunchKernelGGL((gato::k_fused<PL, 4>), dim3(B), blk, lds, st, a);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(s->ev1, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        s->last_kernel_us = ms * 1e3;
+        s->last_launches = 1;
+        return 0;
+    }
+    // plain: host-driven loop (bsqp.cuh:103-197)
+    {
+        const size_t n = (size_t)B * (s->traj + s->nu);
+        hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, n);
+    }
+    hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, 1, 0, 0);
+    HIP_TRY(hipGetLastError());
+    double pcg_ms = 0.0;
+    int pcg_launches = 0;
+    int it = 0;
+    for (; it < (int)s->prm.max_sqp_iters; it++) {
+        hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
+        hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, a);
+        HIP_TRY(hipEventRecord(s->ev0, st));
+        if (np == 1) hipLaunchKernelGGL((gato::k_pcg<PL, 1>), dim3(B), blk, lds, st, a, it);
+        else if (np == 2) hipLaunchKernelGGL((gato::k_pcg<PL, 2>), dim3(B), blk, lds, st, a, it);
+        else hipLaunchKernelGGL((gato::k_pcg<PL, 4>), dim3(B), blk, lds, st, a, it);
+        HIP_TRY(hipEventRecord(s->ev1, st));
+        hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, a, it);
+        hipLaunchKernelGGL(gato::k_check, dim3(sb), dim3(256), 0, st, a, it);
+        HIP_TRY(hipGetLastError());
+        int solved = 0;
+        HIP_TRY(hipMemcpyAsync(&solved, a.solved + it, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        pcg_ms += ms;
+        pcg_launches++;
+        if ((double)solved >= (double)B * s->prm.solve_ratio) { it++; break; }
+        hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, gato::NUM_ALPHAS, 2, it);
+        HIP_TRY(hipGetLastError());
+    }
+    {
+        const size_t n = (size_t)B * (s->traj + s->nu);
+        hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, n);
+    }
+    hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, 1, 1, 0);
+    hipLaunchKernelGGL(gato::k_end, dim3(sb), dim3(256), 0, st, a);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    s->last_kernel_us = pcg_launches ? pcg_ms * 1e3 / pcg_launches : 0.0;
+    s->last_launches = pcg_launches;
+    return 0;
+}
+
+int upload_defaults(gato_solver* s) {
+    const int B = s->B;
+    std::vector<float> Qd((size_t)B * s->nx), Rd((size_t)B * s->nu);
+    for (int b = 0; b < B; b++) {
+        for (int i = 0; i < s->nx; i++) Qd[(size_t)b * s->nx + i] = i < s->nq ? 0.f : s->prm.qd_cost;
+        for (int i = 0; i < s->nu; i++) Rd[(size_t)b * s->nu + i] = s->prm.u_cost;
+    }
+    HIP_TRY(hipMemcpy(s->d_Qd, Qd.data(), Qd.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->d_Rd, Rd.data(), Rd.size() * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gato_last_error(void) { return g_err.c_str(); }
+
+int gato_plant_dims(int32_t plant, int32_t* nq, int32_t* nx) {
+    const int n = plant_nq(plant);
+    if (n < 0) return fail(GATO_E_ARG, "unknown plant");
+    if (nq) *nq = n;
+    if (nx) *nx = 2 * n;
+    return 0;
+}
+
+int gato_create(gato_solver** out, int32_t plant, int32_t knots, int32_t batch, const gato_params* params) {
+    if (!out || !params) return fail(GATO_E_ARG, "null argument");
+    const int nq = plant_nq(plant);
+    if (nq < 0) return fail(GATO_E_ARG, "unknown plant");
+    if (knots < 2 || knots > 128) return fail(GATO_E_ARG, "knots must be in [2, 128]");
+    if (batch < 1) return fail(GATO_E_ARG, "batch must be >= 1");
+    if (params->max_sqp_iters < 1 || params->max_sqp_iters > 1000) return fail(GATO_E_ARG, "max_sqp_iters must be in [1, 1000]");
+    if (params->mode != GATO_MODE_PLAIN && params->mode != GATO_MODE_FUSED) return fail(GATO_E_ARG, "unknown mode");
+    if (params->exit_rule != GATO_EXIT_PCG0 && params->exit_rule != GATO_EXIT_KKT) return fail(GATO_E_ARG, "unknown exit rule");
+    gato_solver* s = new gato_solver();
+    s->plant = plant;
+    s->N = knots;
+    s->B = batch;
+    s->nq = nq;
+    s->nx = 2 * nq;
+    s->nu = nq;
+    s->traj = (s->nx + s->nu) * knots - s->nu;
+    s->prm = *params;
+    s->pool_bytes = layout(s, false);
+    hipError_t e = hipMalloc(&s->pool, s->pool_bytes);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(GATO_E_ALLOC, std::string("hipMalloc workspace: ") + hipGetErrorString(e));
+    }
+    layout(s, true);
+    int rc = 0;
+    auto cleanup = [&](int code) { gato_destroy(s); return code; };
+    if ((e = hipMemset(s->pool, 0, s->pool_bytes)) != hipSuccess) return cleanup(fail((int)e, "hipMemset workspace"));
+    if ((e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess)
+        return cleanup(fail((int)e, "hipEventCreate"));
+    s->h_rho_init.assign(batch, params->rho);
+    s->h_drho_init.assign(batch, 1.f);
+    std::vector<float> mu(batch, params->mu), tol(batch, params->pcg_tol);
+    if ((e = hipMemcpy(s->d_rho, s->h_rho_init.data(), batch * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(s->d_drho, s->h_drho_init.data(), batch * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(s->d_drho_init, s->h_drho_init.data(), batch * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(s->d_mu, mu.data(), batch * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(s->d_tol, tol.data(), batch * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return cleanup(fail((int)e, "hipMemcpy initial state"));
+    if ((rc = upload_defaults(s)) != 0) return cleanup(rc);
+    const size_t it = params->max_sqp_iters;
+    s->h_pcg.resize(it * batch);
+    s->h_kres.resize(it * batch);
+    s->h_lsm.resize(it * batch);
+    s->h_step.resize(it * batch);
+    s->h_sqp.resize(batch);
+    s->h_conv.resize(batch);
+    s->h_init.resize(batch);
+    s->h_final.resize(batch);
+    *out = s;
+    return 0;
+}
+
+int gato_destroy(gato_solver* s) {
+    if (!s) return 0;
+    if (s->ev0) hipEventDestroy(s->ev0);
+    if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->pool) hipFree(s->pool);
+    delete s;
+    return 0;
+}
+
+int gato_solve(gato_solver* s, float* d_xu, const float* d_xs, const float* d_ref, float dt, void* stream,
+               gato_stats* stats) {
+    if (!s || !d_xu || !d_xs || !d_ref) return fail(GATO_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    fill_args(s, d_xu, d_xs, d_ref, dt);
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    int rc = s->plant == GATO_PLANT_IIWA14 ? launch_solve<0>(s, st) : launch_solve<1>(s, st);
+    if (rc) return rc;
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    if (!stats) return 0;
+    const int B = s->B;
+    const size_t it = s->prm.max_sqp_iters;
+    gato::Args& a = s->a;
+    HIP_TRY(hipMemcpyAsync(s->h_pcg.data(), a.st_pcg, it * B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_kres.data(), a.st_kres, it * B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_lsm.data(), a.st_lsm, it * B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_step.data(), a.st_step, it * B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_sqp.data(), a.st_sqp, B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_conv.data(), a.conv, B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_init.data(), a.st_init, B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_final.data(), a.st_final, B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    stats->solve_time_us = std::chrono::duration<double, std::micro>(t1 - t0).count();
+    int n_iters = 0;
+    for (int b = 0; b < B; b++) n_iters = std::max(n_iters, s->h_sqp[b]);
+    int ls_iters = 0;
+    for (size_t i = 0; i < it; i++)
+        for (int b = 0; b < B; b++)
+            if (s->h_step[i * B + b] != 0.f) ls_iters = std::max<int>(ls_iters, (int)i + 1);
+    stats->n_iters = n_iters;
+    stats->ls_num_iters = ls_iters;
+    auto cp = [&](auto* dst, const auto& src, size_t n) { if (dst) std::memcpy(dst, src.data(), n * sizeof(src[0])); };
+    cp(stats->sqp_iters, s->h_sqp, B);
+    cp(stats->kkt_converged, s->h_conv, B);
+    cp(stats->initial_merit, s->h_init, B);
+    cp(stats->final_merit, s->h_final, B);
+    cp(stats->pcg_iters, s->h_pcg, it * B);
+    cp(stats->ls_min_merit, s->h_lsm, it * B);
+    cp(stats->ls_step_size, s->h_step, it * B);
+    cp(stats->kkt_residual, s->h_kres, it * B);
+    return 0;
+}
+
+int gato_set_joint_cost(gato_solver* s, const float* d_Qd, const float* d_Rd, const float* d_xref, void* stream) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    hipStream_t st = (hipStream_t)stream;
+    if (!d_Qd && !d_Rd && !d_xref) {
+        s->joint_cost = false;
+        return upload_defaults(s);
+    }
+    if (d_Qd) HIP_TRY(hipMemcpyAsync(s->d_Qd, d_Qd, (size_t)s->B * s->nx * 4, hipMemcpyDeviceToDevice, st));
+    if (d_Rd) HIP_TRY(hipMemcpyAsync(s->d_Rd, d_Rd, (size_t)s->B * s->nu * 4, hipMemcpyDeviceToDevice, st));
+    if (d_xref) {
+        HIP_TRY(hipMemcpyAsync(s->d_xref, d_xref, (size_t)s->B * s->N * s->nx * 4, hipMemcpyDeviceToDevice, st));
+        s->joint_cost = true;
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int gato_set_f_ext_batch(gato_solver* s, const float* d_f_ext, void* stream) {
+    if (!s || !d_f_ext) return fail(GATO_E_ARG, "null argument");
+    HIP_TRY(hipMemcpyAsync(s->d_fext, d_f_ext, (size_t)s->B * 6 * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}
+
+int gato_set_rho_penalty_batch(gato_solver* s, const float* h_rho, int32_t set_as_reset_default) {
+    if (!s || !h_rho) return fail(GATO_E_ARG, "null argument");
+    if (set_as_reset_default) std::memcpy(s->h_rho_init.data(), h_rho, s->B * 4);
+    HIP_TRY(hipMemcpy(s->d_rho, h_rho, s->B * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int gato_set_drho_batch(gato_solver* s, const float* h_drho, int32_t set_as_reset_default) {
+    if (!s || !h_drho) return fail(GATO_E_ARG, "null argument");
+    if (set_as_reset_default) {
+        std::memcpy(s->h_drho_init.data(), h_drho, s->B * 4);
+        HIP_TRY(hipMemcpy(s->d_drho_init, h_drho, s->B * 4, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemcpy(s->d_drho, h_drho, s->B * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int gato_set_mu_batch(gato_solver* s, const float* h_mu) {
+    if (!s || !h_mu) return fail(GATO_E_ARG, "null argument");
+    HIP_TRY(hipMemcpy(s->d_mu, h_mu, s->B * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int gato_set_pcg_tol_batch(gato_solver* s, const float* h_tol) {
+    if (!s || !h_tol) return fail(GATO_E_ARG, "null argument");
+    HIP_TRY(hipMemcpy(s->d_tol, h_tol, s->B * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int gato_reset_dual(gato_solver* s) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    HIP_TRY(hipMemset(s->a.lam, 0, (size_t)s->B * s->N * s->nx * 4));
+    return 0;
+}
+
+int gato_reset_rho(gato_solver* s) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    HIP_TRY(hipMemcpy(s->d_rho, s->h_rho_init.data(), s->B * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->d_drho, s->h_drho_init.data(), s->B * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int gato_set_rho_adaptation(gato_solver* s, int32_t enabled) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    s->adapt_rho = enabled ? 1 : 0;
+    return 0;
+}
+
+int gato_set_mode(gato_solver* s, int32_t mode, int32_t exit_rule) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    if (mode != GATO_MODE_PLAIN && mode != GATO_MODE_FUSED) return fail(GATO_E_ARG, "unknown mode");
+    if (exit_rule != GATO_EXIT_PCG0 && exit_rule != GATO_EXIT_KKT) return fail(GATO_E_ARG, "unknown exit rule");
+    s->prm.mode = mode;
+    s->prm.exit_rule = exit_rule;
+    return 0;
+}
+
+int gato_get_state(gato_solver* s, float* d_lambda, float* h_rho, float* h_drho) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    if (d_lambda) HIP_TRY(hipMemcpy(d_lambda, s->a.lam, (size_t)s->B * s->N * s->nx * 4, hipMemcpyDeviceToDevice));
+    if (h_rho) HIP_TRY(hipMemcpy(h_rho, s->d_rho, s->B * 4, hipMemcpyDeviceToHost));
+    if (h_drho) HIP_TRY(hipMemcpy(h_drho, s->d_drho, s->B * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int gato_set_state(gato_solver* s, const float* d_lambda, const float* h_rho, const float* h_drho) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    if (d_lambda) HIP_TRY(hipMemcpy(s->a.lam, d_lambda, (size_t)s->B * s->N * s->nx * 4, hipMemcpyDeviceToDevice));
+    if (h_rho) HIP_TRY(hipMemcpy(s->d_rho, h_rho, s->B * 4, hipMemcpyHostToDevice));
+    if (h_drho) HIP_TRY(hipMemcpy(s->d_drho, h_drho, s->B * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int gato_sim_forward(gato_solver* s, float* d_xnext, const float* d_x, const float* d_u, float dt, void* stream) {
+    if (!s || !d_xnext || !d_x || !d_u) return fail(GATO_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = (s->B + 63) / 64;
+    if (s->plant == GATO_PLANT_IIWA14)
+        hipLaunchKernelGGL(gato::k_sim<0>, dim3(nb), dim3(64), 0, st, d_xnext, d_x, d_u, s->d_fext, dt, s->B);
+    else
+        hipLaunchKernelGGL(gato::k_sim<1>, dim3(nb), dim3(64), 0, st, d_xnext, d_x, d_u, s->d_fext, dt, s->B);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_xs, const float* d_ref, float dt,
+                     void* stream) {
+    if (!s || !d_xu || !d_xs || !d_ref) return fail(GATO_E_ARG, "null argument");
+    if (phase < 0 || phase > 4) return fail(GATO_E_ARG, "unknown phase");
+    hipStream_t st = (hipStream_t)stream;
+    fill_args(s, d_xu, d_xs, d_ref, dt);
+    const dim3 blk(grid_threads(s->N));
+    const size_t lds = lds_bytes(s);
+    gato::Args a = s->a;
+    a.exit_rule = GATO_EXIT_PCG0;  // run every problem
+    const int sb = (s->B + 255) / 256;
+    hipLaunchKernelGGL(gato::k_begin, dim3(sb), dim3(256), 0, st, a);
+    const int np = npass(s->N);
+    const bool ii = s->plant == GATO_PLANT_IIWA14;
+    switch (phase) {
+        case 0:
+            if (ii) hipLaunchKernelGGL(gato::k_lin<0>, dim3(s->B), blk, lds, st, a);
+            else hipLaunchKernelGGL(gato::k_lin<1>, dim3(s->B), blk, lds, st, a);
+            break;
+        case 1:
+            if (ii) hipLaunchKernelGGL(gato::k_schur<0>, dim3(s->B), blk, lds, st, a);
+            else hipLaunchKernelGGL(gato::k_schur<1>, dim3(s->B), blk, lds, st, a);
+            break;
+        case 2:
+            if (ii) {
+                if (np == 1) hipLaunchKernelGGL((gato::k_pcg<0, 1>), dim3(s->B), blk, lds, st, a, 0);
+                else if (np == 2) hipLaunchKernelGGL((gato::k_pcg<0, 2>), dim3(s->B), blk, lds, st, a, 0);
+                else hipLaunchKernelGGL((gato::k_pcg<0, 4>), dim3(s->B), blk, lds, st, a, 0);
+            } else {
+                if (np == 1) hipLaunchKernelGGL((gato::k_pcg<1, 1>), dim3(s->B), blk, lds, st, a, 0);
+                else if (np == 2) hipLaunchKernelGGL((gato::k_pcg<1, 2>), dim3(s->B), blk, lds, st, a, 0);
+                else hipLaunchKernelGGL((gato::k_pcg<1, 4>), dim3(s->B), blk, lds, st, a, 0);
+            }
+            break;
+        case 3:
+            if (ii) hipLaunchKernelGGL(gato::k_dz<0>, dim3(s->B), blk, lds, st, a, 0);
+            else hipLaunchKernelGGL(gato::k_dz<1>, dim3(s->B), blk, lds, st, a, 0);
+            break;
+        case 4:
+            // merit terms only: reuse the line-search launch with a large phi so nothing is applied
+            if (ii) hipLaunchKernelGGL(gato::k_merit<0>, dim3(s->B), blk, lds, st, a, gato::NUM_ALPHAS, 3, 0);
+            else hipLaunchKernelGGL(gato::k_merit<1>, dim3(s->B), blk, lds, st, a, gato::NUM_ALPHAS, 3, 0);
+            break;
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int gato_debug_copy(gato_solver* s, const char* name, void* d_dst, int64_t cap, int64_t* count) {
+    if (!s || !name || !d_dst) return fail(GATO_E_ARG, "null argument");
+    const size_t N = s->N, B = s->B, nx = s->nx, nu = s->nu, nq = s->nq, it = s->prm.max_sqp_iters;
+    const gato::Args& a = s->a;
+    struct E { const char* n; const void* p; size_t c; };
+    const E tab[] = {{"A", a.A, B * N * nx * nx},         {"B", a.Bm, B * N * nx * nu},   {"c", a.c, B * N * nx},
+                     {"Qp", a.Qp, B * N * nq * nq},       {"Qv", a.Qv, B * N * nq},       {"Rg", a.Rg, B * N * nu},
+                     {"q", a.q, B * N * nx},              {"r", a.r, B * N * nu},         {"Qip", a.Qip, B * N * nq * nq},
+                     {"Qiv", a.Qiv, B * N * nq},          {"Ri", a.Ri, B * N * nu},       {"Sd", a.Sd, B * N * nx * nx},
+                     {"So", a.So, B * N * nx * nx},       {"Pd", a.Pd, B * N * nx * nx},  {"Po", a.Po, B * N * nx * nx},
+                     {"gamma", a.gam, B * N * nx},        {"lambda", a.lam, B * N * nx},  {"dz", a.dz, B * (s->traj + nu)},
+                     {"merit_parts", a.mpart, B * gato::NUM_ALPHAS * N}, {"pcg_iters", a.st_pcg, it * B},
+                     {"kkt_residual", a.st_kres, it * B}};
+    for (const E& e : tab) {
+        if (std::strcmp(e.n, name) != 0) continue;
+        if (count) *count = (int64_t)e.c;
+        if ((int64_t)e.c > cap) return fail(GATO_E_ARG, std::string("buffer too small for ") + name);
+        HIP_TRY(hipMemcpy(d_dst, e.p, e.c * 4, hipMemcpyDeviceToDevice));
+        return 0;
+    }
+    return fail(GATO_E_ARG, std::string("unknown workspace array ") + name);
+}
+
+int gato_last_kernel_time(gato_solver* s, double* avg_us, int32_t* launches) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    if (avg_us) *avg_us = s->last_kernel_us;
+    if (launches) *launches = s->last_launches;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,526 @@
+// SQP phase routines on CDNA4: one 16-lane "knot group" per knot (four groups per wave64),
+// shared by the one-kernel-per-phase path and the persistent fused kernel.
+//
+// Reference semantics (Mo0dy/GATO):
+//   KKT setup      gato/bsqp/kernels/setup_kkt.cuh:14-108, gato/dynamics/integrator.cuh:21-257,
+//                  gato/dynamics/iiwa14/iiwa14_plant.cuh:104-155, 339-450 (indy7_plant.cuh:141-450)
+//   Schur + P      gato/bsqp/kernels/schur_linsys.cuh:13-260
+//   PCG            gato/bsqp/kernels/pcg.cuh:13-148
+//   dz             gato/bsqp/kernels/schur_linsys.cuh:315-431
+//   merit          gato/bsqp/kernels/merit.cuh:16-92, iiwa14_plant.cuh:276-327
+//   line search    gato/bsqp/kernels/line_search.cuh:12-98
+//
+// Layouts (fp32): A_k (nx x nx), B_k (nx x nu), S/P blocks (nx x nx) are row-major;
+// the position block of Q_k (nq x nq) is row-major, the velocity block and R_k are diagonal.
+// Q~_k = Q_k + rho diag(1_nq, 0) (linalg.cuh:85-96): its inverse is blkdiag(Qip, 1/Qv).
+#pragma once
+#include "dynamics.cuh"
+
+namespace gato {
+
+constexpr int KG = 16;       // lanes per knot group
+constexpr int NUM_ALPHAS = 8;  // settings.h:15
+constexpr float RHO_INIT = 1e-3f, RHO_FACTOR = 1.2f, RHO_MIN = 1e-8f, RHO_MAX = 10.f;  // settings.h:18-22
+constexpr float PCG_ABS_TOL = 1e-6f;  // pcg.cuh:27
+
+struct CostW {
+    float q_cost, N_cost, q_lim, vel_lim, ctrl_lim;
+    int terminal_mode;  // 0: reference (terminal block at x_{N-2} with q_cost); 1: at x_{N-1} with N_cost
+    int hess_mode;      // 0: reference rank-1 EE Hessian; 1: Gauss-Newton J^T J
+};
+
+// Order LDS traffic between lanes of one wavefront (knot groups never straddle a wave).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------- barrier terms
+// -log(max(v - lo, 1e-10)) - log(max(hi - v, 1e-10))  (iiwa14_plant.cuh:104-111)
+__device__ __forceinline__ float barrier(float v, float lo, float hi) {
+    float a = v - lo, b = hi - v;
+    a = a <= 1e-10f ? 1e-10f : a;
+    b = b <= 1e-10f ? 1e-10f : b;
+    return -logf(a) - logf(b);
+}
+// gradient: iiwa14 sign-preserving clamp (iiwa14_plant.cuh:114-136), indy7 plain clamp (indy7_plant.cuh:141-148)
+template <int PL>
+__device__ __forceinline__ float barrier_g(float v, float lo, float hi) {
+    float a = v - lo, b = hi - v;
+    const float eps = 1e-6f;
+    if (PL == 1) {
+        a = a <= eps ? eps : a;
+        b = b <= eps ? eps : b;
+    } else {
+        a = a >= 0.f ? (a < eps ? eps : a) : (a > -eps ? -eps : a);
+        b = b >= 0.f ? (b < eps ? eps : b) : (b > -eps ? -eps : b);
+    }
+    return -1.f / a + 1.f / b;
+}
+// iiwa14 curvature 1/d_lo^2 + 1/d_hi^2 with |d| >= 1e-6 (iiwa14_plant.cuh:141-155)
+__device__ __forceinline__ float barrier_h(float v, float lo, float hi) {
+    float a = fabsf(v - lo), b = fabsf(hi - v);
+    a = a < 1e-6f ? 1e-6f : a;
+    b = b < 1e-6f ? 1e-6f : b;
+    return 1.f / (a * a) + 1.f / (b * b);
+}
+
+template <typename Tarr>
+__device__ __forceinline__ float pick(const Tarr& arr, int n, int idx) {
+    float r = 0.f;
+#pragma unroll
+    for (int i = 0; i < n; i++) r = (i == idx) ? arr[i] : r;
+    return r;
+}
+
+template <int PL>
+__device__ __forceinline__ float plim(int which, int i) {
+    using P = Plant<PL>;
+    float r = 0.f;
+#pragma unroll
+    for (int k = 0; k < P::NQ; k++) {
+        const float v = which == 0 ? P::qlo(k) : which == 1 ? P::qhi(k) : which == 2 ? P::vlo(k)
+                       : which == 3 ? P::vhi(k) : which == 4 ? P::ulo(k) : P::uhi(k);
+        r = (k == i) ? v : r;
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------- KKT setup, one knot
+// k <= N-2: A_k, B_k, c_{k+1} and the cost model at (x_k, u_k, ref_k);
+// k == N-1: terminal cost model (quirk Q4: at x_{N-2} with ref_{N-1} and q_cost) and c_0 = x_0 - x_s.
+// xk points at [x_k u_k x_{k+1}] (k <= N-2) or is unused; xt is the terminal-block state.
+template <int PL>
+__device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const CostW& cw, const float* xk,
+                                         const float* xt, const float* refk, const float* xrefk, const float* Qd,
+                                         const float* Rd, const float* fext, const float* x0, const float* xs,
+                                         float* A, float* Bm, float* cn, float* c0, float* Qp, float* Qv, float* Rg,
+                                         float* qv, float* rv, float* shg) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
+    const bool term = (k == N - 1);
+    const float* xsrc = term ? xt : xk;
+    float q[NQ], qd[NQ], u[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) {
+        q[i] = xsrc[i];
+        qd[i] = xsrc[NQ + i];
+        u[i] = term ? 0.f : xk[NX + i];
+    }
+    Kin<PL> K;
+    K.set(q);
+    if (!term) {
+        float qdd[NQ], L[NQ * (NQ + 1) / 2];
+        forward_dynamics<PL>(K, qd, u, fext, qdd, L);
+        const int col = lane < NX ? lane : 0;
+        const int j = col < NQ ? col : col - NQ;
+        const int dir = col < NQ ? 0 : 1;
+        const V6 fj = joint_force<PL>(K, qd, qdd, fext, j);
+        float dq[NQ];
+        rnea_tangent<PL>(K, qd, qdd, fj, j, dir, dq);
+#pragma unroll
+        for (int i = 0; i < NQ; i++) dq[i] = -dq[i];
+        chol_solve<NQ>(L, dq);  // d qdd / d x_col
+        float mc[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; i++) mc[i] = (i == j) ? 1.f : 0.f;
+        chol_solve<NQ>(L, mc);  // M^-1 e_j
+        const float h2 = 0.5f * dt * dt;
+        if (lane < NX) {  // column `lane` of A (integrator.cuh:143-169)
+#pragma unroll
+            for (int r = 0; r < NQ; r++) {
+                float top = (r == lane ? 1.f : 0.f) + ((lane >= NQ && r == lane - NQ) ? dt : 0.f) + h2 * dq[r];
+                float bot = ((NQ + r) == lane ? 1.f : 0.f) + dt * dq[r];
+                A[r * NX + lane] = top;
+                A[(NQ + r) * NX + lane] = bot;
+            }
+        }
+        if (lane < NU) {  // column `lane` of B (integrator.cuh:170-184)
+#pragma unroll
+            for (int r = 0; r < NQ; r++) {
+                Bm[r * NU + lane] = h2 * mc[r];
+                Bm[(NQ + r) * NU + lane] = dt * mc[r];
+            }
+        }
+        if (lane < NQ) {  // c_{k+1} = x_{k+1} - x+ (integrator.cuh:49-62)
+            const float ql = pick(q, NQ, lane), qdl = pick(qd, NQ, lane), al = pick(qdd, NQ, lane);
+            const float* xn = xk + NX + NU;
+            cn[lane] = xn[lane] - (ql + dt * qdl + 0.5f * al * dt * dt);
+            cn[NQ + lane] = xn[NQ + lane] - (qdl + dt * al);
+        }
+    }
+    // cost model (iiwa14_plant.cuh:339-424); joint-space diagonal weights generalise qd_cost/u_cost
+    const float w = term ? (cw.terminal_mode ? cw.N_cost : cw.q_cost) : cw.q_cost;
+    float p[3], Jc[3];
+    ee_position<PL>(K, lane < NQ ? lane : -1, p, Jc);
+    const float e0 = p[0] - refk[0], e1 = p[1] - refk[1], e2 = p[2] - refk[2];
+    const float gl = Jc[0] * e0 + Jc[1] * e1 + Jc[2] * e2;
+    if (lane < NQ) {
+        shg[lane] = gl;
+        if (cw.hess_mode) { shg[NQ + 3 * lane] = Jc[0]; shg[NQ + 3 * lane + 1] = Jc[1]; shg[NQ + 3 * lane + 2] = Jc[2]; }
+    }
+    wave_sync();
+    if (lane < NQ) {
+        const float ql = pick(q, NQ, lane), qdl = pick(qd, NQ, lane);
+        const float bgl = barrier_g<PL>(ql, plim<PL>(0, lane), plim<PL>(1, lane));
+#pragma unroll
+        for (int m = 0; m < NQ; m++) {
+            float v;
+            if (cw.hess_mode)
+                v = w * (Jc[0] * shg[NQ + 3 * m] + Jc[1] * shg[NQ + 3 * m + 1] + Jc[2] * shg[NQ + 3 * m + 2]);
+            else
+                v = w * gl * shg[m];  // rank-1 EE term (quirk Q5)
+            if (PL == 1) v += cw.q_lim * bgl * barrier_g<PL>(q[m], P::qlo(m), P::qhi(m));
+            if (m == lane) {
+                v += Qd[lane];
+                if (PL != 1) v += cw.q_lim * barrier_h(ql, plim<PL>(0, lane), plim<PL>(1, lane));
+            }
+            Qp[lane * NQ + m] = v;
+        }
+        const float xr_q = xrefk ? xrefk[lane] : 0.f, xr_v = xrefk ? xrefk[NQ + lane] : 0.f;
+        qv[lane] = w * gl + cw.q_lim * bgl + Qd[lane] * (ql - xr_q);
+        const float bv = barrier_g<PL>(qdl, plim<PL>(2, lane), plim<PL>(3, lane));
+        qv[NQ + lane] = Qd[NQ + lane] * (qdl - xr_v) + cw.vel_lim * bv;
+        Qv[lane] = Qd[NQ + lane] + cw.vel_lim * (PL == 1 ? bv * bv : barrier_h(qdl, plim<PL>(2, lane), plim<PL>(3, lane)));
+        if (!term) {
+            const float ul = pick(u, NQ, lane);
+            const float bu = barrier_g<PL>(ul, plim<PL>(4, lane), plim<PL>(5, lane));
+            rv[lane] = Rd[lane] * ul + cw.ctrl_lim * bu;
+            Rg[lane] = Rd[lane] + cw.ctrl_lim * (PL == 1 ? bu * bu : barrier_h(ul, plim<PL>(4, lane), plim<PL>(5, lane)));
+        } else {
+            rv[lane] = 0.f;
+            Rg[lane] = 1.f;
+        }
+    }
+    if (term && lane < NX) c0[lane] = x0[lane] - xs[lane];
+    wave_sync();
+}
+
+// ---------------------------------------------------------------- Schur step 1: Q~_k^-1, R_k^-1
+// Gauss-Jordan without pivoting on the nq x nq position block (linalg.cuh:457-519), lane l owns row l.
+template <int PL>
+__device__ __forceinline__ void invert_q_knot(int lane, bool has_r, float rho, const float* Qp, const float* Qv,
+                                              const float* Rg, float* Qip, float* Qiv, float* Ri, float* piv) {
+    constexpr int NQ = Plant<PL>::NQ;
+    float row[2 * NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; c++) {
+        row[c] = lane < NQ ? Qp[lane * NQ + c] + (c == lane ? rho : 0.f) : 0.f;
+        row[NQ + c] = (c == lane) ? 1.f : 0.f;
+    }
+#pragma unroll
+    for (int p = 0; p < NQ; p++) {
+        if (lane == p) {
+            const float inv = 1.f / row[p];
+#pragma unroll
+            for (int c = 0; c < 2 * NQ; c++) {
+                row[c] *= inv;
+                piv[c] = row[c];
+            }
+        }
+        wave_sync();
+        if (lane < NQ && lane != p) {
+            const float f = row[p];
+#pragma unroll
+            for (int c = 0; c < 2 * NQ; c++) row[c] -= f * piv[c];
+        }
+        wave_sync();
+    }
+    if (lane < NQ) {
+#pragma unroll
+        for (int c = 0; c < NQ; c++) Qip[lane * NQ + c] = row[NQ + c];
+        Qiv[lane] = 1.f / Qv[lane];
+        if (has_r) Ri[lane] = 1.f / Rg[lane];
+    }
+}
+
+// Q~^-1[r][m] for block-diagonal Q~^-1 = blkdiag(Qip, diag(Qiv))
+template <int PL>
+__device__ __forceinline__ float qinv_at(const float* Qip, const float* Qiv, int r, int m) {
+    constexpr int NQ = Plant<PL>::NQ;
+    if (r < NQ && m < NQ) return Qip[r * NQ + m];
+    if (r >= NQ && m == r) return Qiv[r - NQ];
+    return 0.f;
+}
+
+// Gauss-Jordan inverse of an nx x nx matrix distributed one row per lane (rows in registers).
+template <int NX>
+__device__ __forceinline__ void gj_rows(int lane, float* row /*2*NX*/, float* piv) {
+#pragma unroll
+    for (int p = 0; p < NX; p++) {
+        if (lane == p) {
+            const float inv = 1.f / row[p];
+#pragma unroll
+            for (int c = 0; c < 2 * NX; c++) {
+                row[c] *= inv;
+                piv[c] = row[c];
+            }
+        }
+        wave_sync();
+        if (lane < NX && lane != p) {
+            const float f = row[p];
+#pragma unroll
+            for (int c = 0; c < 2 * NX; c++) row[c] -= f * piv[c];
+        }
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------- Schur step 2: block row k+1 (k <= N-2)
+// theta_k = A Q~_k^-1 A^T + B R^-1 B^T + Q~_{k+1}^-1, phi_k = A Q~_k^-1,
+// gamma_{k+1} = c_{k+1} - Q~_{k+1}^-1 q_{k+1} + phi_k q_k + B R^-1 r_k,
+// S_{k+1,k+1} = -theta_k, S_{k+1,k} = phi_k, P_{k+1,k+1} = -(theta_k + rho diag(1_nq, 0))^-1 (quirk Q8).
+// Lane r keeps its phi_k row and P_{k+1,k+1} row for step 3.
+template <int PL>
+__device__ __forceinline__ void schur_row(int lane, float rho, const float* A, const float* Bm, const float* Qip0,
+                                          const float* Qiv0, const float* Ri0, const float* Qip1, const float* Qiv1,
+                                          const float* q0, const float* q1, const float* r0, const float* c1,
+                                          float* Sd1, float* So1, float* Pd1, float* g1, float* piv, float* phi,
+                                          float* prow) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
+    const int r = lane < NX ? lane : 0;
+    float arow[NX];
+#pragma unroll
+    for (int l = 0; l < NX; l++) arow[l] = A[r * NX + l];
+#pragma unroll
+    for (int c = 0; c < NQ; c++) {
+        float s = 0.f;
+#pragma unroll
+        for (int l = 0; l < NQ; l++) s += arow[l] * Qip0[l * NQ + c];
+        phi[c] = s;
+        phi[NQ + c] = arow[NQ + c] * Qiv0[c];
+    }
+    float brr[NU];
+#pragma unroll
+    for (int c = 0; c < NU; c++) brr[c] = Bm[r * NU + c] * Ri0[c];
+    float th[NX];
+    float g = c1[r];
+#pragma unroll
+    for (int m = 0; m < NX; m++) {
+        float s = qinv_at<PL>(Qip1, Qiv1, r, m);
+        g -= s * q1[m];
+#pragma unroll
+        for (int l = 0; l < NX; l++) s += phi[l] * A[m * NX + l];
+#pragma unroll
+        for (int l = 0; l < NU; l++) s += brr[l] * Bm[m * NU + l];
+        th[m] = s;
+        g += phi[m] * q0[m];
+    }
+#pragma unroll
+    for (int l = 0; l < NU; l++) g += brr[l] * r0[l];
+    if (lane < NX) {
+        g1[r] = g;
+#pragma unroll
+        for (int m = 0; m < NX; m++) {
+            Sd1[r * NX + m] = -th[m];
+            So1[r * NX + m] = phi[m];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NX; m++) {
+        prow[m] = th[m] + ((m == r && m < NQ) ? rho : 0.f);
+        prow[NX + m] = (m == r) ? 1.f : 0.f;
+    }
+    gj_rows<NX>(lane, prow, piv);
+#pragma unroll
+    for (int m = 0; m < NX; m++) prow[m] = -prow[NX + m];
+    if (lane < NX) {
+#pragma unroll
+        for (int m = 0; m < NX; m++) Pd1[r * NX + m] = prow[m];
+    }
+}
+
+// Block row 0: S_00 = -Q~_0^-1, P_00 = -Q~_0, gamma_0 = c_0 - Q~_0^-1 q_0 (schur_linsys.cuh:164-210)
+template <int PL>
+__device__ __forceinline__ void schur_row0(int lane, float rho, const float* Qp0, const float* Qv0, const float* Qip0,
+                                           const float* Qiv0, const float* q0, const float* c0, float* Sd0,
+                                           float* Pd0, float* g0) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX;
+    if (lane >= NX) return;
+    const int r = lane;
+    float g = c0[r];
+#pragma unroll
+    for (int m = 0; m < NX; m++) {
+        const float qi = qinv_at<PL>(Qip0, Qiv0, r, m);
+        g -= qi * q0[m];
+        Sd0[r * NX + m] = -qi;
+        float qt = 0.f;
+        if (r < NQ && m < NQ) qt = Qp0[r * NQ + m] + (r == m ? rho : 0.f);
+        else if (r >= NQ && m == r) qt = Qv0[r - NQ];
+        Pd0[r * NX + m] = -qt;
+    }
+    g0[r] = g;
+}
+
+// ---------------------------------------------------------------- Schur step 3: P_{k+1,k} = -P_{k+1,k+1} phi_k P_{k,k}
+template <int PL>
+__device__ __forceinline__ void precond_offdiag(int lane, const float* phi, const float* prow, const float* Pd0,
+                                                float* Tsh, float* Po1) {
+    constexpr int NX = Plant<PL>::NX;
+    float t[NX];
+#pragma unroll
+    for (int c = 0; c < NX; c++) {
+        float s = 0.f;
+#pragma unroll
+        for (int m = 0; m < NX; m++) s += phi[m] * Pd0[m * NX + c];
+        t[c] = s;
+    }
+    if (lane < NX) {
+#pragma unroll
+        for (int c = 0; c < NX; c++) Tsh[lane * NX + c] = t[c];
+    }
+    wave_sync();
+    if (lane < NX) {
+#pragma unroll
+        for (int c = 0; c < NX; c++) {
+            float s = 0.f;
+#pragma unroll
+            for (int m = 0; m < NX; m++) s += prow[m] * Tsh[m * NX + c];
+            Po1[lane * NX + c] = -s;
+        }
+    }
+    wave_sync();
+}
+
+// ---------------------------------------------------------------- dz, one knot (schur_linsys.cuh:315-431)
+// v = q_k - lambda_k + A_k^T lambda_{k+1}; dx_k = -Q~_k^-1 v;  w = r_k + B_k^T lambda_{k+1}; du_k = -R_k^-1 w.
+// Returns the lane's max |stationarity| and |defect| entries (the KKT residual of the problem is
+// the max over knots and lanes).
+template <int PL>
+__device__ __forceinline__ float dz_knot(int k, int N, int lane, const float* A, const float* Bm, const float* Qip,
+                                         const float* Qiv, const float* Ri, const float* q, const float* rr,
+                                         const float* c, const float* lam0, const float* lam1, float* dzk,
+                                         float* shv) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
+    const bool last = (k == N - 1);
+    float res = 0.f;
+    if (lane < NX) {
+        float v = q[lane] - lam0[lane];
+        if (!last) {
+#pragma unroll
+            for (int i = 0; i < NX; i++) v += A[i * NX + lane] * lam1[i];
+        }
+        shv[lane] = v;
+        res = fmaxf(fabsf(v), fabsf(c[lane]));
+    }
+    wave_sync();
+    if (lane < NX) {
+        float s;
+        if (lane < NQ) {
+            s = 0.f;
+#pragma unroll
+            for (int l = 0; l < NQ; l++) s += Qip[lane * NQ + l] * shv[l];
+        } else {
+            s = Qiv[lane - NQ] * shv[lane];
+        }
+        dzk[lane] = -s;
+    }
+    if (!last && lane < NU) {
+        float w = rr[lane];
+#pragma unroll
+        for (int i = 0; i < NX; i++) w += Bm[i * NU + lane] * lam1[i];
+        dzk[NX + lane] = -Ri[lane] * w;
+        res = fmaxf(res, fabsf(w));
+    }
+    wave_sync();
+    return res;
+}
+
+// ---------------------------------------------------------------- merit, one (knot, alpha) item
+// l_k + mu * |c|_1 at z + alpha dz (merit.cuh:16-92, iiwa14_plant.cuh:276-327).
+template <int PL>
+__device__ __forceinline__ float merit_item(int k, int N, float alpha, float dt, const CostW& cw, float mu,
+                                            const float* xuk, const float* dzk, const float* refk,
+                                            const float* xrefk, const float* Qd, const float* Rd,
+                                            const float* fext, const float* x0, const float* dz0, const float* xs) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
+    const bool last = (k == N - 1);
+    float q[NQ], qd[NQ], u[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) {
+        q[i] = xuk[i] + alpha * dzk[i];
+        qd[i] = xuk[NQ + i] + alpha * dzk[NQ + i];
+        u[i] = last ? 0.f : xuk[NX + i] + alpha * dzk[NX + i];
+    }
+    float cost = 0.f;
+#pragma unroll
+    for (int i = 0; i < NQ; i++) {
+        const float e = qd[i] - (xrefk ? xrefk[NQ + i] : 0.f);
+        const float eq = q[i] - (xrefk ? xrefk[i] : 0.f);
+        cost += 0.5f * Qd[NQ + i] * e * e + 0.5f * Qd[i] * eq * eq;
+        cost += cw.q_lim * barrier(q[i], P::qlo(i), P::qhi(i));
+        cost += cw.vel_lim * barrier(qd[i], P::vlo(i), P::vhi(i));
+        if (!last) {
+            cost += 0.5f * Rd[i] * u[i] * u[i];
+            cost += cw.ctrl_lim * barrier(u[i], P::ulo(i), P::uhi(i));
+        }
+    }
+    Kin<PL> K;
+    K.set(q);
+    float p[3];
+    ee_position<PL>(K, -1, p, nullptr);
+    const float w = last ? cw.N_cost : cw.q_cost;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float e = p[i] - refk[i];
+        cost += 0.5f * w * e * e;
+    }
+    float cons = 0.f;
+    if (!last) {
+        float qdd[NQ], L[NQ * (NQ + 1) / 2];
+        forward_dynamics<PL>(K, qd, u, fext, qdd, L);
+        const float* xn = xuk + NX + NU;
+        const float* dn = dzk + NX + NU;
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+            const float qn = xn[i] + alpha * dn[i], qdn = xn[NQ + i] + alpha * dn[NQ + i];
+            cons += fabsf(qn - (q[i] + dt * qd[i] + 0.5f * qdd[i] * dt * dt));
+            cons += fabsf(qdn - (qd[i] + dt * qdd[i]));
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NX; i++) cons += fabsf(x0[i] + alpha * dz0[i] - xs[i]);
+    }
+    return cost + mu * cons;
+}
+
+// ---------------------------------------------------------------- work-group reductions (deterministic)
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Sum over the work-group; red holds one float per wave. Every thread returns the total.
+__device__ __forceinline__ float block_sum(float v, float* red) {
+    const int nw = (blockDim.x + 63) >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float s = 0.f;
+    for (int i = 0; i < nw; i++) s += red[i];
+    return s;
+}
+__device__ __forceinline__ float block_max(float v, float* red) {
+    const int nw = (blockDim.x + 63) >> 6;
+    v = wave_max(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float s = 0.f;
+    for (int i = 0; i < nw; i++) s = fmaxf(s, red[i]);
+    return s;
+}
+
+}  // namespace gato

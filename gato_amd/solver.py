@@ -1,0 +1,246 @@
+"""Batched SQP solver over PyTorch-ROCm device memory, calling libgato_mi355x.so through its C-ABI.
+
+``Solver`` mirrors the reference's pybind11 class ``PyBSQP<T, B>`` (python/bindings.cu:10-209) and
+the ``BSQP<T, B>`` solver it wraps (gato/bsqp/bsqp.cuh:20-353): same constructor scalars and defaults,
+same ``solve`` result dictionary, setters and ``sim_forward``. Differences, all additive:
+  * the knot count N and batch size B are run-time values;
+  * inputs may be numpy arrays or torch tensors (device tensors are used in place);
+  * inputs are checked for dtype and size (the reference reads raw buffers unchecked);
+  * keyword-only extensions: execution ``mode`` ("fused" persistent kernel or "plain" one kernel
+    per phase), ``exit_rule`` ("pcg0" reference rule or "kkt"), joint-space costs, state get/set.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import GatoParams, GatoStats, check
+
+# reference C++ constructor defaults (bsqp.cuh:25-27)
+DEFAULTS = dict(dt=0.01, max_sqp_iters=5, kkt_tol=1e-4, max_pcg_iters=100, pcg_tol=1e-5, solve_ratio=1.0, mu=10.0,
+                q_cost=1.0, qd_cost=1e-3, u_cost=1e-6, N_cost=50.0, q_lim_cost=1e-3, vel_lim_cost=0.0,
+                ctrl_lim_cost=0.0, rho=1e-3)
+CTOR_ORDER = ("dt", "max_sqp_iters", "kkt_tol", "max_pcg_iters", "pcg_tol", "solve_ratio", "mu", "q_cost",
+              "qd_cost", "u_cost", "N_cost", "q_lim_cost", "vel_lim_cost", "ctrl_lim_cost", "rho")
+MODES = {"plain": _lib.MODE_PLAIN, "fused": _lib.MODE_FUSED}
+EXITS = {"pcg0": _lib.EXIT_PCG0, "kkt": _lib.EXIT_KKT}
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Solver:
+    def __init__(self, plant, N, B, *args, mode="fused", exit_rule="pcg0", terminal_mode=0, hess_mode=0,
+                 device=None, **kw):
+        if len(args) not in (0, len(CTOR_ORDER)):
+            raise TypeError(f"expected 0 or {len(CTOR_ORDER)} positional solver scalars, got {len(args)}")
+        cfg = dict(DEFAULTS)
+        cfg.update(zip(CTOR_ORDER, args))
+        unknown = set(kw) - set(DEFAULTS)
+        if unknown:
+            raise TypeError(f"unknown solver settings {sorted(unknown)}")
+        cfg.update(kw)
+        if plant not in _lib.PLANT_IDS:
+            raise ValueError(f"unknown plant {plant!r}")
+        self.plant, self.N, self.B = plant, int(N), int(B)
+        self.nq, self.nx = _lib.plant_dims(plant)
+        self.nu = self.nq
+        self.traj = (self.nx + self.nu) * self.N - self.nu
+        self.cfg = cfg
+        self.device = torch.device(device or "cuda:0")
+        if self.device.type != "cuda":
+            raise ValueError("the solver runs on a HIP device (torch 'cuda' device on ROCm)")
+        self.lib = _lib.load()
+        p = GatoParams(cfg["dt"], int(cfg["max_sqp_iters"]), cfg["kkt_tol"], int(cfg["max_pcg_iters"]), cfg["pcg_tol"],
+                       cfg["solve_ratio"], cfg["mu"], cfg["q_cost"], cfg["qd_cost"], cfg["u_cost"], cfg["N_cost"],
+                       cfg["q_lim_cost"], cfg["vel_lim_cost"], cfg["ctrl_lim_cost"], cfg["rho"], MODES[mode],
+                       EXITS[exit_rule], int(terminal_mode), int(hess_mode))
+        self._h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            check(self.lib.gato_create(C.byref(self._h), _lib.PLANT_IDS[plant], self.N, self.B, C.byref(p)), "gato_create")
+        self.mode, self.exit_rule = mode, exit_rule
+        f = dict(dtype=torch.float32, device=self.device)
+        self._xu = torch.zeros(self.B, self.traj, **f)
+        self._xs = torch.zeros(self.B, self.nx, **f)
+        self._ref = torch.zeros(self.B, 6 * self.N, **f)
+        self._x = torch.zeros(self.nx, **f)
+        self._u = torch.zeros(self.nu, **f)
+        self._xn = torch.zeros(self.B, self.nx, **f)
+        it = int(cfg["max_sqp_iters"])
+        self._st = dict(sqp_iters=np.zeros(self.B, np.int32), kkt_converged=np.zeros(self.B, np.int32),
+                        initial_merit=np.zeros(self.B, np.float32), final_merit=np.zeros(self.B, np.float32),
+                        pcg_iters=np.zeros((it, self.B), np.int32), ls_min_merit=np.zeros((it, self.B), np.float32),
+                        ls_step_size=np.zeros((it, self.B), np.float32),
+                        kkt_residual=np.zeros((it, self.B), np.float32))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.gato_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # ---------------------------------------------------------------- helpers
+    def _stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _to_dev(self, x, dst, name):
+        n = dst.numel()
+        if isinstance(x, torch.Tensor):
+            if x.numel() != n:
+                raise ValueError(f"{name}: expected {n} values, got {x.numel()}")
+            dst.view(-1).copy_(x.reshape(-1).to(dtype=torch.float32), non_blocking=False)
+        else:
+            a = np.asarray(x, dtype=np.float32)
+            if a.size != n:
+                raise ValueError(f"{name}: expected {n} values, got {a.size}")
+            dst.view(-1).copy_(torch.from_numpy(np.ascontiguousarray(a).reshape(-1)))
+        return dst
+
+    def _host_f32(self, x, n, name):
+        a = np.ascontiguousarray(np.asarray(x.detach().cpu() if isinstance(x, torch.Tensor) else x,
+                                            dtype=np.float32).reshape(-1))
+        if a.size != n:
+            raise ValueError(f"{name}: expected {n} values, got {a.size}")
+        return a
+
+    # ---------------------------------------------------------------- reference methods
+    def solve(self, xu_traj_batch, timestep, x_s_batch, reference_traj_batch, return_torch=False):
+        """BSQP::solve on B problems (bindings.cu:68-148). Returns the reference's result dict
+        (XU, sqp_time_us, sqp_iters, kkt_converged, final_merit, initial_merit, ls_num_iters,
+        pcg_times_us, pcg_iters, ls_min_merit, ls_step_size) plus kkt_residual and n_iters."""
+        xu = self._to_dev(xu_traj_batch, self._xu, "xu_traj_batch")
+        xs = self._to_dev(x_s_batch, self._xs, "x_s_batch")
+        ref = self._to_dev(reference_traj_batch, self._ref, "reference_traj_batch")
+        st = self._st
+        gs = GatoStats(0.0, 0, 0, *[st[k].ctypes.data_as(C.POINTER(C.c_int32 if st[k].dtype == np.int32 else C.c_float))
+                                    for k in ("sqp_iters", "kkt_converged", "initial_merit", "final_merit", "pcg_iters",
+                                              "ls_min_merit", "ls_step_size", "kkt_residual")])
+        with torch.cuda.device(self.device):
+            check(self.lib.gato_solve(self._h, _ptr(xu), _ptr(xs), _ptr(ref), float(timestep), self._stream(),
+                                      C.byref(gs)), "gato_solve")
+        n_ls = gs.ls_num_iters
+        out = {
+            "XU": xu.clone() if return_torch else xu.cpu().numpy(),
+            "sqp_time_us": float(gs.solve_time_us),
+            "sqp_iters": st["sqp_iters"].copy(),
+            "kkt_converged": st["kkt_converged"].copy(),
+            "final_merit": st["final_merit"].copy(),
+            "initial_merit": st["initial_merit"].copy(),
+            "ls_num_iters": int(n_ls),
+            "pcg_times_us": np.zeros(n_ls, np.float32),
+            "pcg_iters": st["pcg_iters"][:n_ls].copy(),
+            "ls_min_merit": st["ls_min_merit"][:n_ls].copy(),
+            "ls_step_size": st["ls_step_size"][:n_ls].copy(),
+            "kkt_residual": st["kkt_residual"][: gs.n_iters].copy(),
+            "n_iters": int(gs.n_iters),
+            "all_pcg_iters": st["pcg_iters"][: gs.n_iters].copy(),
+        }
+        return out
+
+    def set_f_ext_batch(self, f_ext_batch):
+        f = torch.zeros(self.B * 6, dtype=torch.float32, device=self.device)
+        self._to_dev(f_ext_batch, f, "f_ext_batch")
+        check(self.lib.gato_set_f_ext_batch(self._h, _ptr(f), self._stream()), "gato_set_f_ext_batch")
+
+    def set_rho_penalty_batch(self, rho_batch, set_as_reset_default=True):
+        a = self._host_f32(rho_batch, self.B, "rho_batch")
+        check(self.lib.gato_set_rho_penalty_batch(self._h, _fptr(a), int(bool(set_as_reset_default))),
+              "gato_set_rho_penalty_batch")
+
+    def set_drho_batch(self, drho_batch, set_as_reset_default=True):
+        a = self._host_f32(drho_batch, self.B, "drho_batch")
+        check(self.lib.gato_set_drho_batch(self._h, _fptr(a), int(bool(set_as_reset_default))), "gato_set_drho_batch")
+
+    def set_mu_batch(self, mu_batch):
+        a = self._host_f32(mu_batch, self.B, "mu_batch")
+        check(self.lib.gato_set_mu_batch(self._h, _fptr(a)), "gato_set_mu_batch")
+
+    def set_pcg_tol_batch(self, eps_batch):
+        a = self._host_f32(eps_batch, self.B, "eps_batch")
+        check(self.lib.gato_set_pcg_tol_batch(self._h, _fptr(a)), "gato_set_pcg_tol_batch")
+
+    def sim_forward(self, xk, uk, dt):
+        """One trapezoidal step of (xk, uk) under each of the B wrenches (bindings.cu:180-194)."""
+        self._to_dev(xk, self._x, "xk")
+        self._to_dev(uk, self._u, "uk")
+        check(self.lib.gato_sim_forward(self._h, _ptr(self._xn), _ptr(self._x), _ptr(self._u), float(dt),
+                                        self._stream()), "gato_sim_forward")
+        return self._xn.cpu().numpy()
+
+    def reset_dual(self):
+        check(self.lib.gato_reset_dual(self._h), "gato_reset_dual")
+
+    def reset_rho(self):
+        check(self.lib.gato_reset_rho(self._h), "gato_reset_rho")
+
+    def set_rho_adaptation(self, enabled):
+        check(self.lib.gato_set_rho_adaptation(self._h, int(bool(enabled))), "gato_set_rho_adaptation")
+
+    # ---------------------------------------------------------------- extensions
+    def set_mode(self, mode=None, exit_rule=None):
+        self.mode = mode or self.mode
+        self.exit_rule = exit_rule or self.exit_rule
+        check(self.lib.gato_set_mode(self._h, MODES[self.mode], EXITS[self.exit_rule]), "gato_set_mode")
+
+    def set_joint_cost(self, Qd=None, Rd=None, xref=None):
+        """Diagonal joint-space weights (B x nx, B x nu) and joint reference (B x N*nx) replacing
+        qd_cost / u_cost; all None restores the reference cost."""
+        bufs = []
+        for x, n, name in ((Qd, self.B * self.nx, "Qd"), (Rd, self.B * self.nu, "Rd"), (xref, self.B * self.N * self.nx, "xref")):
+            if x is None:
+                bufs.append(None)
+            else:
+                t = torch.zeros(n, dtype=torch.float32, device=self.device)
+                bufs.append(self._to_dev(x, t, name))
+        check(self.lib.gato_set_joint_cost(self._h, *[None if b is None else _ptr(b) for b in bufs], self._stream()),
+              "gato_set_joint_cost")
+
+    def get_state(self):
+        lam = torch.zeros(self.B, self.N * self.nx, dtype=torch.float32, device=self.device)
+        rho = np.zeros(self.B, np.float32)
+        drho = np.zeros(self.B, np.float32)
+        check(self.lib.gato_get_state(self._h, _ptr(lam), _fptr(rho), _fptr(drho)), "gato_get_state")
+        return lam.cpu().numpy(), rho, drho
+
+    def set_state(self, lam=None, rho=None, drho=None):
+        lt = None
+        if lam is not None:
+            lt = torch.zeros(self.B * self.N * self.nx, dtype=torch.float32, device=self.device)
+            self._to_dev(lam, lt, "lambda")
+        r = self._host_f32(rho, self.B, "rho") if rho is not None else None
+        d = self._host_f32(drho, self.B, "drho") if drho is not None else None
+        check(self.lib.gato_set_state(self._h, None if lt is None else _ptr(lt), None if r is None else _fptr(r),
+                                      None if d is None else _fptr(d)), "gato_set_state")
+
+    def debug_phase(self, phase, xu, xs, ref, dt=None):
+        """Run one plain-path phase (0 KKT, 1 Schur, 2 PCG, 3 dz, 4 merit terms) on the workspace."""
+        self._to_dev(xu, self._xu, "xu")
+        self._to_dev(xs, self._xs, "xs")
+        self._to_dev(ref, self._ref, "ref")
+        dt = self.cfg["dt"] if dt is None else dt
+        check(self.lib.gato_debug_phase(self._h, int(phase), _ptr(self._xu), _ptr(self._xs), _ptr(self._ref),
+                                        float(dt), self._stream()), "gato_debug_phase")
+        return self._xu
+
+    def debug_array(self, name):
+        cap = self.B * self.N * self.nx * self.nx + self.B * (self.traj + self.nu) + 1024
+        dst = torch.zeros(cap, dtype=torch.float32, device=self.device)
+        n = C.c_int64()
+        check(self.lib.gato_debug_copy(self._h, name.encode(), _ptr(dst), cap, C.byref(n)), "gato_debug_copy")
+        out = dst[: n.value].cpu().numpy()
+        return out.view(np.int32) if name == "pcg_iters" else out
+
+    def last_kernel_time(self):
+        us = C.c_double()
+        n = C.c_int32()
+        check(self.lib.gato_last_kernel_time(self._h, C.byref(us), C.byref(n)), "gato_last_kernel_time")
+        return us.value, n.value

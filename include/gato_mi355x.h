@@ -1,0 +1,147 @@
+/* gato_mi355x — batched SQP trajectory optimisation on one AMD Instinct MI355X (gfx950).
+ *
+ * C-ABI of the HIP library libgato_mi355x.so. Plain pointers and sizes only; device pointers
+ * are HIP device allocations owned by the caller (e.g. torch tensors on cuda:0).
+ * Every entry point returns 0 on success or a non-zero status (GATO_E_* or a hipError_t);
+ * gato_last_error() describes the last failure of the calling thread. Nothing here exits
+ * the process or throws across the ABI.
+ *
+ * Reference interface replaced (Mo0dy/GATO, python/bindings.cu): the pybind11 class
+ * BSQP_{B}_float of module bsqpN{N}_{plant} (bindings.cu:212-266) and the solver it wraps,
+ * BSQP<T, BatchSize> (gato/bsqp/bsqp.cuh:20-353). Knot count N and batch size B are run-time
+ * arguments here instead of template parameters.
+ */
+#ifndef GATO_MI355X_H
+#define GATO_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GATO_OK 0
+#define GATO_E_ARG 1001     /* invalid argument (null handle, bad size, unknown plant) */
+#define GATO_E_ALLOC 1002   /* device allocation failed */
+#define GATO_E_LAUNCH 1003  /* kernel launch failed */
+
+#define GATO_PLANT_IIWA14 0
+#define GATO_PLANT_INDY7 1
+
+/* execution paths */
+#define GATO_MODE_PLAIN 0 /* one kernel per phase, host loop with a device->host sync per iteration */
+#define GATO_MODE_FUSED 1 /* one persistent kernel: every SQP iteration of every problem on device */
+
+/* convergence rules */
+#define GATO_EXIT_PCG0 0 /* reference: converged <=> PCG took 0 iterations (bsqp.cuh:149-163) */
+#define GATO_EXIT_KKT 1  /* converged <=> max(|stationarity|_inf, |defect|_inf) <= kkt_tol */
+
+/* Solver settings: the 15 constructor scalars of BSQP (bsqp.cuh:43-45, bindings.cu:30-44) plus
+ * the MI355X build's extensions (marked). */
+typedef struct gato_params {
+    float dt;
+    uint32_t max_sqp_iters;
+    float kkt_tol;
+    uint32_t max_pcg_iters;
+    float pcg_tol;
+    float solve_ratio;
+    float mu;
+    float q_cost, qd_cost, u_cost, N_cost, q_lim_cost, vel_lim_cost, ctrl_lim_cost;
+    float rho;
+    /* extensions */
+    int32_t mode;          /* GATO_MODE_* (default GATO_MODE_FUSED) */
+    int32_t exit_rule;     /* GATO_EXIT_* (default GATO_EXIT_PCG0) */
+    int32_t terminal_mode; /* 0: reference terminal block (x_{N-2}, q_cost); 1: consistent (x_{N-1}, N_cost) */
+    int32_t hess_mode;     /* 0: reference rank-1 end-effector Hessian; 1: Gauss-Newton J^T J */
+} gato_params;
+
+/* Per-solve statistics, written to HOST memory by gato_solve. Arrays sized by the caller:
+ * per-problem arrays [B]; per-iteration arrays [max_sqp_iters * B] (row = iteration). */
+typedef struct gato_stats {
+    double solve_time_us;   /* host wall clock around the device work (bsqp.cuh:109, 185) */
+    int32_t n_iters;        /* SQP loop iterations run */
+    int32_t ls_num_iters;   /* iterations that reached the line search */
+    int32_t* sqp_iters;     /* [B] */
+    int32_t* kkt_converged; /* [B] */
+    float* initial_merit;   /* [B] */
+    float* final_merit;     /* [B] */
+    int32_t* pcg_iters;     /* [max_sqp_iters * B] */
+    float* ls_min_merit;    /* [max_sqp_iters * B] */
+    float* ls_step_size;    /* [max_sqp_iters * B]; -1 = line-search failure */
+    float* kkt_residual;    /* [max_sqp_iters * B] */
+} gato_stats;
+
+typedef struct gato_solver gato_solver;
+
+/* BSQP(...) constructor (bsqp.cuh:43-59; bindings.cu:27-56): allocates the workspace for
+ * `batch` problems of `knots` knot points on the current device. */
+int gato_create(gato_solver** out, int32_t plant, int32_t knots, int32_t batch, const gato_params* params);
+/* ~BSQP() (bsqp.cuh:61; bindings.cu:58-66) */
+int gato_destroy(gato_solver* s);
+
+/* BSQP::solve (bsqp.cuh:103-197; bindings.cu:68-148). Device pointers:
+ *   d_xu  [B * ((nx+nu)N - nu)] in/out trajectory, d_xs [B * nx] initial states,
+ *   d_ref [B * 6N] end-effector reference (xyz used, rpy ignored).
+ * stream: hipStream_t (NULL = default stream). Returns after the solve completed. */
+int gato_solve(gato_solver* s, float* d_xu, const float* d_xs, const float* d_ref, float dt, void* stream,
+               gato_stats* stats);
+
+/* Extension: joint-space tracking terms 1/2 sum_i Qd_i (x_i - xref_{k,i})^2 + 1/2 sum_i Rd_i u_i^2
+ * replacing qd_cost / u_cost (NULL pointers restore the reference cost). Device pointers
+ * d_Qd [B * nx], d_Rd [B * nu], d_xref [B * N * nx]; copied, so they may be freed afterwards. */
+int gato_set_joint_cost(gato_solver* s, const float* d_Qd, const float* d_Rd, const float* d_xref, void* stream);
+
+/* set_f_ext_batch (bsqp.cuh:63; bindings.cu:150-154): d_f_ext [B * 6] device pointer */
+int gato_set_f_ext_batch(gato_solver* s, const float* d_f_ext, void* stream);
+/* set_rho_penalty_batch (bsqp.cuh:66-70): host pointer [B] */
+int gato_set_rho_penalty_batch(gato_solver* s, const float* h_rho, int32_t set_as_reset_default);
+/* set_drho_batch (bsqp.cuh:72-76): host pointer [B] */
+int gato_set_drho_batch(gato_solver* s, const float* h_drho, int32_t set_as_reset_default);
+/* set_mu_batch (bsqp.cuh:78): host pointer [B] */
+int gato_set_mu_batch(gato_solver* s, const float* h_mu);
+/* set_pcg_tol_batch (bsqp.cuh:79): host pointer [B] */
+int gato_set_pcg_tol_batch(gato_solver* s, const float* h_tol);
+/* reset_dual (bsqp.cuh:81) */
+int gato_reset_dual(gato_solver* s);
+/* reset_rho (bsqp.cuh:83-87) */
+int gato_reset_rho(gato_solver* s);
+/* set_rho_adaptation (bsqp.cuh:89) */
+int gato_set_rho_adaptation(gato_solver* s, int32_t enabled);
+/* Extension: select GATO_MODE_* / GATO_EXIT_* after construction */
+int gato_set_mode(gato_solver* s, int32_t mode, int32_t exit_rule);
+
+/* Extension: persistent state for deterministic tests. d_lambda [B * N * nx], h_rho/h_drho [B];
+ * any pointer may be NULL to skip it. */
+int gato_get_state(gato_solver* s, float* d_lambda, float* h_rho, float* h_drho);
+int gato_set_state(gato_solver* s, const float* d_lambda, const float* h_rho, const float* h_drho);
+
+/* sim_forward (bsqp.cuh:91; sim.cuh:14-88; bindings.cu:180-194): one trapezoidal step of the
+ * device state d_x [nx] under control d_u [nu] for each of the B wrenches set by
+ * gato_set_f_ext_batch; d_xnext [B * nx]. */
+int gato_sim_forward(gato_solver* s, float* d_xnext, const float* d_x, const float* d_u, float dt, void* stream);
+
+/* Plant dimensions: nq (= nu), nx = 2 nq. */
+int gato_plant_dims(int32_t plant, int32_t* nq, int32_t* nx);
+
+/* Extension: average device time per launch (microseconds) of the dominant kernel during the last
+ * solve, measured with hipEvents on the solve's stream, and the launch count. */
+int gato_last_kernel_time(gato_solver* s, double* avg_us, int32_t* launches);
+
+/* Extension (phase-level tests, no reference counterpart): run one phase of the plain path on
+ * the solver's workspace for all B problems, with the solver's current lambda / rho.
+ * phase: 0 KKT setup, 1 Schur complement + preconditioner, 2 PCG, 3 dz + KKT residual,
+ *        4 merit terms at the 8 step lengths (no line search). */
+int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_xs, const float* d_ref, float dt,
+                     void* stream);
+/* Extension (phase-level tests): copy workspace array `name` ("A", "B", "c", "Qp", "Qv", "Rg", "q", "r",
+ * "Qip", "Qiv", "Ri", "Sd", "So", "Pd", "Po", "gamma", "lambda", "dz", "merit_parts", "pcg_iters",
+ * "kkt_residual") into the device buffer d_dst of capacity `cap` elements; *count = its length. */
+int gato_debug_copy(gato_solver* s, const char* name, void* d_dst, int64_t cap, int64_t* count);
+
+/* Description of the last failure on the calling thread ("" if none). */
+const char* gato_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

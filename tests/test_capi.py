@@ -1,0 +1,68 @@
+"""CPU checks of the C-ABI boundary: the HIP library loads and exports every entry point that
+include/gato_mi355x.h declares, with the ctypes signatures used by gato_amd._lib; argument
+validation that needs no device. No compute calls (no GPU here)."""
+import ctypes as C
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def header_symbols():
+    src = (ROOT / "include" / "gato_mi355x.h").read_text()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gato_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    from gato_amd import _lib
+
+    lib = _lib.load()
+    syms = header_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    # the ctypes table covers the header exactly
+    assert sorted(_lib.SIGNATURES) == syms
+
+
+def test_struct_layouts_match_header():
+    from gato_amd._lib import GatoParams, GatoStats
+
+    # 15 reference scalars + 4 int32 extensions, 4 bytes each
+    assert C.sizeof(GatoParams) == 19 * 4
+    assert C.sizeof(GatoStats) == 8 + 4 + 4 + 8 * 8
+
+
+def test_argument_validation_without_device():
+    from gato_amd import _lib
+
+    lib = _lib.load()
+    nq, nx = C.c_int32(), C.c_int32()
+    assert lib.gato_plant_dims(0, C.byref(nq), C.byref(nx)) == 0 and (nq.value, nx.value) == (7, 14)
+    assert lib.gato_plant_dims(1, C.byref(nq), C.byref(nx)) == 0 and (nq.value, nx.value) == (6, 12)
+    assert lib.gato_plant_dims(5, C.byref(nq), C.byref(nx)) == 1001
+    assert b"unknown plant" in lib.gato_last_error()
+    h = C.c_void_p()
+    p = _lib.GatoParams()
+    p.max_sqp_iters = 5
+    # bad knot count / batch / plant are rejected before any device call
+    assert lib.gato_create(C.byref(h), 0, 1, 4, C.byref(p)) == 1001
+    assert lib.gato_create(C.byref(h), 0, 32, 0, C.byref(p)) == 1001
+    assert lib.gato_create(C.byref(h), 9, 32, 4, C.byref(p)) == 1001
+    assert lib.gato_solve(None, None, None, None, 0.01, None, None) == 1001
+    assert lib.gato_destroy(None) == 0
+
+
+def test_drop_in_module_surface():
+    import importlib
+
+    mod = importlib.import_module("gato_amd.bsqp.bsqpN32_iiwa14")
+    assert mod.KNOT_POINTS == 32
+    for B in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024):
+        cls = getattr(mod, f"BSQP_{B}_float")
+        for m in ("solve", "reset_dual", "set_f_ext_batch", "set_rho_penalty_batch", "set_drho_batch",
+                  "set_mu_batch", "set_pcg_tol_batch", "sim_forward", "reset_rho", "set_rho_adaptation"):
+            assert callable(getattr(cls, m))
+    assert importlib.import_module("gato_amd.bsqp.bsqpN128_indy7").KNOT_POINTS == 128
