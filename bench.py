@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Throughput of the batched SQP solve on one MI355X: iiwa14, N=32, B=512 figure-8 tracking.
+
+A step is one batched solve (<= 10 SQP iterations per problem) of B=512 problems from the same
+synthetic initial guess, inputs resident in HBM. value = problems solved per second (whole job).
+Prints ONE JSON line. --gpus above 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "SQP solves/s & fwd+bwd solves/s, iiwa14 N=32 B=512, KKT residual ≤1e-4"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="fused", choices=["fused", "plain"])
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--knots", type=int, default=32)
+    ap.add_argument("--plain-steps", type=int, default=3, help="steps of the plain path timed for the speed-up (0: skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample (0: skip)")
+    return ap.parse_args()
+
+
+def workload(N, B):
+    from gato_amd import workloads as W
+
+    return W.drop_in_figure8("iiwa14", N=N, B=B, seed=1)
+
+
+def make_solver(wl, mode):
+    from gato_amd.solver import Solver
+
+    p = wl.params
+    kw = {k: p[k] for k in ("dt", "max_sqp_iters", "kkt_tol", "max_pcg_iters", "pcg_tol", "solve_ratio", "mu",
+                            "q_cost", "qd_cost", "u_cost", "N_cost", "q_lim_cost", "vel_lim_cost", "ctrl_lim_cost",
+                            "rho")}
+    return Solver(wl.plant, wl.N, wl.B, mode=mode, **kw)
+
+
+def time_gpu(solver, wl, steps, warmup):
+    import torch
+
+    dev = solver.device
+    xu0 = torch.tensor(wl.xu, dtype=torch.float32, device=dev)
+    xs = torch.tensor(wl.xs, dtype=torch.float32, device=dev)
+    ref = torch.tensor(wl.ref, dtype=torch.float32, device=dev)
+    xu = xu0.clone()
+
+    def one():
+        xu.copy_(xu0)
+        solver.reset_dual()
+        solver.reset_rho()
+        solver.solve_device(xu, xs, ref)
+
+    for _ in range(warmup):
+        one()
+    torch.cuda.synchronize()
+    kern = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+        kern.append(solver.last_kernel_time())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    # one more solve with statistics (untimed) for iteration counts and convergence
+    xu.copy_(xu0)
+    solver.reset_dual()
+    solver.reset_rho()
+    st = solver.solve_device(xu, xs, ref, stats=True)
+    return (t1 - t0), kern, st
+
+
+def algorithmic_bytes(wl):
+    """Compulsory HBM bytes of one batched solve: x_s, reference, XU in and out, the warm-start
+    multipliers in and out, and the per-problem scalars (rho, drho, mu, tol, merit, stats)."""
+    nq = 7 if wl.plant == "iiwa14" else 6
+    nx, nu, N = 2 * nq, nq, wl.N
+    traj = (nx + nu) * N - nu
+    per = nx + 6 * N + 2 * traj + 2 * nx * N + 7
+    return 4 * per * wl.B
+
+
+def algorithmic_flops(st, N):
+    """Nominal FLOPs of the reference iteration per problem-iteration (estimate at iiwa14 N=32,
+    scaled linearly in N): 8.5 MFLOP + 0.08 MFLOP per PCG iteration."""
+    it = st["sqp_iters"].sum()
+    pcg = st["pcg_iters"].sum()
+    s = N / 32.0
+    return s * (8.5e6 * it + 0.08e6 * pcg)
+
+
+def cpu_baseline(wl, seconds):
+    """The FP64 oracle of the same iteration on the host cores, on a bounded sample."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    p = wl.params
+    cfg = O.make_cfg(wl.plant, N=wl.N, dt=p["dt"], max_sqp_iters=p["max_sqp_iters"], kkt_tol=p["kkt_tol"],
+                     max_pcg_iters=p["max_pcg_iters"], solve_ratio=p["solve_ratio"], q_cost=p["q_cost"],
+                     qd_cost=p["qd_cost"], u_cost=p["u_cost"], N_cost=p["N_cost"], q_lim_cost=p["q_lim_cost"],
+                     vel_lim_cost=p["vel_lim_cost"], ctrl_lim_cost=p["ctrl_lim_cost"])
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+
+    def run(n, threads):
+        t = time.perf_counter()
+        O.solve(cfg, wl.xu[:n], wl.xs[:n], wl.ref[:n], mu=p["mu"], pcg_tol=p["pcg_tol"], rho0=p["rho"], nthreads=threads)
+        return time.perf_counter() - t
+
+    probe = max(cores, 4)
+    tp = run(probe, cores)
+    n = int(min(wl.B, max(probe, probe * (0.8 * seconds) / max(tp, 1e-3))))
+    n = max(cores, (n // cores) * cores)
+    tn = run(n, cores)
+    t1 = run(2, 1)
+    return {"value": n / tn, "unit": "problem-solves/s", "cores": cores, "kind": "port",
+            "sample": f"{n} of the {wl.B} problems, same seeded inputs, <= {p['max_sqp_iters']} SQP iterations, "
+                      f"FP64 oracle, OpenMP static over problems",
+            "single_core_value": 2 / t1}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (problems are independent; "
+                                   "sharding is described in DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        sys.exit(1)
+    wl = workload(args.knots, args.batch)
+    solver = make_solver(wl, args.mode)
+    elapsed, kern, st = time_gpu(solver, wl, args.steps, args.warmup)
+    B = wl.B
+    value = B * args.steps / elapsed
+    ms_step = 1e3 * elapsed / args.steps
+    kern_us = [k[0] for k in kern]
+    avg_us = sum(kern_us) / len(kern_us)
+    launches = kern[0][1]
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "problem-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{wl.plant} N={wl.N} B={B} figure-8 end-effector tracking, <= "
+                               f"{wl.params['max_sqp_iters']} SQP iterations per solve",
+                   "plant": wl.plant, "knots": wl.N, "batch": B, "max_sqp_iters": wl.params["max_sqp_iters"],
+                   "mode": args.mode, "parallelism": "single GPU, one work-group per problem"},
+        "kkt_converged_fraction": float((st["kkt_residual"][st["sqp_iters"] - 1, range(B)] <= 1e-4).mean()),
+        "median_final_kkt_residual": float(sorted(st["kkt_residual"][st["sqp_iters"] - 1, range(B)])[B // 2]),
+        "mean_sqp_iters": float(st["sqp_iters"].mean()),
+        "mean_pcg_iters": float(st["pcg_iters"].sum() / max(1, st["sqp_iters"].sum())),
+        "fwd_bwd_solves_per_s": None,
+    }
+    kernel_name = "k_fused (whole solve)" if args.mode == "fused" else "k_pcg (per launch)"
+    if args.mode == "fused":
+        by = algorithmic_bytes(wl)
+        fl = algorithmic_flops(st, wl.N)
+        ach = by / (avg_us * 1e-6) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": kernel_name,
+                           "avg_launch_us": avg_us, "launches_per_step": launches,
+                           "valu": {"achieved": fl / (avg_us * 1e-6) / 1e12, "peak": FP32_PEAK_TFLOPS,
+                                    "unit": "TFLOP/s", "frac": fl / (avg_us * 1e-6) / 1e12 / FP32_PEAK_TFLOPS,
+                                    "flops_per_launch": fl, "note": "nominal FLOPs of the reference iteration"}}
+    else:
+        out["roofline"] = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                           "traffic": None, "kernel": kernel_name, "avg_launch_us": avg_us,
+                           "launches_per_step": launches}
+    if args.mode == "fused" and args.plain_steps > 0:
+        plain = make_solver(wl, "plain")
+        pe, _, _ = time_gpu(plain, wl, args.plain_steps, 1)
+        pv = B * args.plain_steps / pe
+        out["plain_value"] = pv
+        out["speedup_vs_plain"] = value / pv
+    if args.cpu_seconds > 0:
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -325,11 +325,99 @@ __device__ __forceinline__ void forward_dynamics(const Kin<PL>& K, const float* 
     constexpr int NQ = Plant<PL>::NQ;
     float cb[NQ];
     rnea<PL, false>(K, qd, nullptr, fext, cb);
+    __builtin_amdgcn_sched_barrier(0);  // keep the RNEA and CRBA live ranges apart
     crba<PL>(K, L);
     chol_packed<NQ>(L);
 #pragma unroll
     for (int i = 0; i < NQ; i++) qdd[i] = u[i] - cb[i];
     chol_solve<NQ>(L, qdd);
+}
+
+// Per-link RNEA table shared by the lanes of one knot (stride 8 floats per spatial vector):
+// velocity v_i at [8 i], acceleration a_i at [8 (NQ + i)], force transmitted across joint i at [8 (2 NQ + i)].
+template <int PL>
+struct RneaTab {
+    static constexpr int NQ = Plant<PL>::NQ;
+    static constexpr int SIZE = 8 * 3 * NQ;
+    __device__ static void put(float* t, int slot, const V6& x) {
+        float* p = t + 8 * slot;
+        p[0] = x.w0; p[1] = x.w1; p[2] = x.w2; p[3] = x.l0; p[4] = x.l1; p[5] = x.l2;
+    }
+    __device__ static V6 get(const float* t, int slot) {
+        const float* p = t + 8 * slot;
+        return V6{p[0], p[1], p[2], p[3], p[4], p[5]};
+    }
+};
+
+// RNEA at (q, qd, qdd) with f_ext; the `writer` lane publishes v_i, a_i and the transmitted forces.
+template <int PL>
+__device__ __forceinline__ void rnea_publish(const Kin<PL>& K, const float* qd, const float* qdd, const float* fext,
+                                             bool writer, float* tab) {
+    constexpr int NQ = Plant<PL>::NQ;
+    using T = RneaTab<PL>;
+    V6 f[NQ];
+    V6 v = v6zero(), a = V6{0.f, 0.f, 0.f, 0.f, 0.f, kGravity};
+    Unroll<0, NQ>::run([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        v = xmot<PL, i>(K.s[i], K.c[i], v);
+        a = xmot<PL, i>(K.s[i], K.c[i], a);
+        v.w2 += qd[i];
+        a = a + crm_s(v, qd[i]);
+        a.w2 += qdd[i];
+        if (writer) { T::put(tab, i, v); T::put(tab, NQ + i, a); }
+        f[i] = imul<PL, i>(a) + crf(v, imul<PL, i>(v));
+    });
+    if (fext) {
+        f[NQ - 1].w0 -= fext[0]; f[NQ - 1].w1 -= fext[1]; f[NQ - 1].w2 -= fext[2];
+        f[NQ - 1].l0 -= fext[3]; f[NQ - 1].l1 -= fext[4]; f[NQ - 1].l2 -= fext[5];
+    }
+    UnrollDown<0, NQ>::run([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        if (writer) T::put(tab, 2 * NQ + i, f[i]);
+        if constexpr (i > 0) f[i - 1] = f[i - 1] + xfrc<PL, i>(K.s[i], K.c[i], f[i]);
+    });
+}
+
+// Tangent of ID along q_j (dir 0) or qd_j (dir 1), qdd fixed, reading v_i, a_i and the joint-j force
+// from the shared RNEA table (same equations as rnea_tangent below).
+template <int PL>
+__device__ __forceinline__ void rnea_tangent_tab(const Kin<PL>& K, const float* qd, const float* qdd,
+                                                 const float* tab, int j, int dir, float* dtau) {
+    constexpr int NQ = Plant<PL>::NQ;
+    using T = RneaTab<PL>;
+    const V6 fj = T::get(tab, 2 * NQ + j);
+    V6 dv = v6zero(), da = v6zero();
+    V6 df[NQ];
+    Unroll<0, NQ>::run([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        __builtin_amdgcn_sched_barrier(0);  // one link at a time: table loads are not hoisted
+        const V6 v = T::get(tab, i);
+        const V6 a = T::get(tab, NQ + i);
+        V6 xv = v;  // X_i v_{i-1}
+        xv.w2 -= qd[i];
+        V6 xa = a - crm_s(v, qd[i]);  // X_i a_{i-1}
+        xa.w2 -= qdd[i];
+        dv = xmot<PL, i>(K.s[i], K.c[i], dv);
+        da = xmot<PL, i>(K.s[i], K.c[i], da);
+        const bool hit = (i == j);
+        if (hit && dir == 0) {
+            dv = dv - s_crm(xv);
+            da = da - s_crm(xa);
+        }
+        if (hit && dir == 1) dv.w2 += 1.f;
+        da = da + crm_s(dv, qd[i]);
+        if (hit && dir == 1) da = da + V6{v.w1, -v.w0, 0.f, v.l1, -v.l0, 0.f};  // v_i x S
+        df[i] = imul<PL, i>(da) + crf(dv, imul<PL, i>(v)) + crf(v, imul<PL, i>(dv));
+    });
+    UnrollDown<0, NQ>::run([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        dtau[i] = df[i].w2;
+        if constexpr (i > 0) {
+            V6 g = df[i];
+            if (i == j && dir == 0) g = g + s_crf(fj);
+            df[i - 1] = df[i - 1] + xfrc<PL, i>(K.s[i], K.c[i], g);
+        }
+    });
 }
 
 // Force transmitted across joint j by the RNEA at (q, qd, qdd) (f_ext included), for run-time j.

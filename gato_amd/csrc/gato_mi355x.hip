@@ -81,7 +81,7 @@ __device__ void phase_lin(const Args& a, int b, float* sm) {
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
     View<PL> v(a, b);
     const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
-    float* shg = sm + g * 4 * NQ;
+    float* shg = sm + g * lin_lds_floats<PL>();
     const float* xt = v.xu + (size_t)(a.cw.terminal_mode ? N - 1 : N - 2) * (NX + NU);
     for (int k = g; k < N; k += G) {
         const float* xk = v.xu + (size_t)(k < N - 1 ? k : 0) * (NX + NU);
@@ -232,6 +232,109 @@ __device__ int phase_pcg(const Args& a, int b, float eps, float* sm) {
     return it;
 }
 
+// Sum over the work-group with one barrier: red is double-buffered by the caller (alternate
+// halves between consecutive calls), so no barrier is needed before the partial sums are written.
+__device__ __forceinline__ float block_sum1(float v, float* red) {
+    const int nw = (blockDim.x + 63) >> 6;
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float s = 0.f;
+    for (int i = 0; i < nw; i++) s += red[i];
+    return s;
+}
+
+// Same iteration as phase_pcg with the lane's rows of S and P held in registers for the whole
+// solve (one knot per knot group: N <= 32) and four work-group barriers per iteration.
+// The sums run in the same order as btd_row, so the results are identical to phase_pcg.
+template <int PL>
+__device__ int phase_pcg_reg(const Args& a, int b, float eps, float* sm) {
+    constexpr int NX = Plant<PL>::NX, XP = 16;
+    View<PL> v(a, b);
+    const int N = a.N, tid = threadIdx.x, k = tid / KG, lane = tid % KG;
+    float* sp = sm;
+    float* sr = sp + (N + 2) * XP;
+    float* red = sr + (N + 2) * XP;  // 4 x 16 floats: two double-buffered reduction slots
+    for (int i = tid; i < 2 * (N + 2) * XP; i += blockDim.x) sm[i] = 0.f;
+    const bool val = (lane < NX) && (k < N);
+    const int l = lane < NX ? lane : 0;
+    const int kk = k < N ? k : 0;
+    // rows: main block row l, left block row l (k > 0), right block column l (k < N-1)
+    float sd[NX], sl[NX], su[NX], pd[NX], pl[NX], pu[NX];
+    {
+        const float* D = v.Sd + (size_t)kk * NX * NX + l * NX;
+        const float* L = v.So + (size_t)kk * NX * NX + l * NX;
+        const float* U = v.So + (size_t)(kk + 1 < N ? kk + 1 : kk) * NX * NX + l;
+        const float* PD = v.Pd + (size_t)kk * NX * NX + l * NX;
+        const float* PL_ = v.Po + (size_t)kk * NX * NX + l * NX;
+        const float* PU = v.Po + (size_t)(kk + 1 < N ? kk + 1 : kk) * NX * NX + l;
+        const bool hasl = kk > 0, hasu = kk < N - 1;
+#pragma unroll
+        for (int c = 0; c < NX; c++) {
+            sd[c] = D[c];
+            pd[c] = PD[c];
+            sl[c] = hasl ? L[c] : 0.f;
+            pl[c] = hasl ? PL_[c] : 0.f;
+            su[c] = hasu ? U[c * NX] : 0.f;
+            pu[c] = hasu ? PU[c * NX] : 0.f;
+        }
+    }
+    auto mv = [&](const float* M0, const float* M1, const float* M2, const float* x) {
+        const float* xk = x + (kk + 1) * XP;
+        float s = 0.f;
+#pragma unroll
+        for (int c = 0; c < NX; c++) s += M0[c] * xk[c];
+#pragma unroll
+        for (int c = 0; c < NX; c++) s += M1[c] * xk[c - XP];
+#pragma unroll
+        for (int c = 0; c < NX; c++) s += M2[c] * xk[c + XP];
+        return s;
+    };
+    float x = val ? v.lam[kk * NX + l] : 0.f;
+    __syncthreads();
+    if (val) sp[(kk + 1) * XP + l] = x;
+    __syncthreads();
+    float r = val ? v.gam[kk * NX + l] - mv(sd, sl, su, sp) : 0.f;
+    if (val) sr[(kk + 1) * XP + l] = r;
+    __syncthreads();
+    float z = val ? mv(pd, pl, pu, sr) : 0.f;
+    float p = z;
+    int slot = 0;
+    float rho = block_sum1(r * z, red + 16 * slot);
+    slot ^= 1;
+    if (fabsf(rho) < PCG_ABS_TOL) return 0;
+    const float rho0 = fabsf(rho);
+    int it = 0;
+    for (int i = 0; i < a.max_pcg; i++) {
+        it++;
+        if (val) sp[(kk + 1) * XP + l] = p;
+        __syncthreads();
+        const float Ap = val ? mv(sd, sl, su, sp) : 0.f;
+        const float alpha = rho / block_sum1(p * Ap, red + 16 * slot);
+        slot ^= 1;
+        x += alpha * p;
+        r -= alpha * Ap;
+        if (val) sr[(kk + 1) * XP + l] = r;
+        __syncthreads();
+        z = val ? mv(pd, pl, pu, sr) : 0.f;
+        const float rn = block_sum1(r * z, red + 16 * slot);
+        slot ^= 1;
+        if (fabsf(rn) < PCG_ABS_TOL + eps * rho0) break;
+        const float beta = rn / rho;
+        rho = rn;
+        p = z + beta * p;
+    }
+    if (val) v.lam[kk * NX + l] = x;
+    return it;
+}
+
+// register-resident PCG when every knot group owns a single knot (N <= 32), else S and P stream
+template <int PL, int NPASS>
+__device__ __forceinline__ int pcg_dispatch(const Args& a, int b, float eps, float* sm) {
+    if constexpr (NPASS == 1) return phase_pcg_reg<PL>(a, b, eps, sm);
+    else return phase_pcg<PL, NPASS>(a, b, eps, sm);
+}
+
 template <int PL>
 __device__ float phase_dz(const Args& a, int b, float* sm) {
     using P = Plant<PL>;
@@ -347,7 +450,7 @@ __global__ void __launch_bounds__(512) k_pcg(Args a, int it) {
     const int b = blockIdx.x;
     const bool skip = (a.exit_rule == GATO_EXIT_KKT) ? !a.active[b] : a.conv[b];  // quirk Q9 (pcg.cuh:30-33)
     int n = 0;
-    if (!skip) n = phase_pcg<PL, NPASS>(a, b, a.tol[b], sm);
+    if (!skip) n = pcg_dispatch<PL, NPASS>(a, b, a.tol[b], sm);
     if (threadIdx.x == 0) a.st_pcg[it * a.B + b] = n;
 }
 
@@ -436,7 +539,7 @@ __global__ void __launch_bounds__(512) k_fused(Args a) {
             __syncthreads();
             phase_schur<PL>(a, b, a.rho[b], sm);
             __syncthreads();
-            const int n = phase_pcg<PL, NPASS>(a, b, eps, sm);
+            const int n = pcg_dispatch<PL, NPASS>(a, b, eps, sm);
             __syncthreads();
             const float kr = phase_dz<PL>(a, b, sm);
             if (threadIdx.x == 0) {
@@ -603,9 +706,9 @@ int npass(int N) { return (N + 31) / 32; }
 size_t lds_bytes(const gato_solver* s) {
     const int G = std::min(s->N, 32), nx = s->nx, nq = s->nq;
     size_t f = 0;
-    f = std::max<size_t>(f, (size_t)G * 4 * nq);                   // lin
+    f = std::max<size_t>(f, (size_t)G * (28 * nq + nq * (nq + 1) / 2));  // lin
     f = std::max<size_t>(f, (size_t)G * (2 * nx + nx * nx));        // schur
-    f = std::max<size_t>(f, (size_t)2 * (s->N + 2) * 16 + 16);      // pcg
+    f = std::max<size_t>(f, (size_t)2 * (s->N + 2) * 16 + 32);      // pcg
     f = std::max<size_t>(f, (size_t)G * gato::KG + 16);             // dz
     f = std::max<size_t>(f, (size_t)2 * gato::NUM_ALPHAS + 16);     // merit + ls
     return f * sizeof(float);

@@ -87,6 +87,13 @@ __device__ __forceinline__ float plim(int which, int i) {
     return r;
 }
 
+// LDS floats per knot group used by lin_knot: J / gradient scratch, RNEA table, Cholesky factor
+template <int PL>
+__host__ __device__ constexpr int lin_lds_floats() {
+    constexpr int NQ = Plant<PL>::NQ;
+    return 4 * NQ + 24 * NQ + NQ * (NQ + 1) / 2;
+}
+
 // ---------------------------------------------------------------- KKT setup, one knot
 // k <= N-2: A_k, B_k, c_{k+1} and the cost model at (x_k, u_k, ref_k);
 // k == N-1: terminal cost model (quirk Q4: at x_{N-2} with ref_{N-1} and q_cost) and c_0 = x_0 - x_s.
@@ -110,22 +117,35 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const
     }
     Kin<PL> K;
     K.set(q);
+    // knot-group LDS: RNEA table, Cholesky factor of M (identical in every lane of the group,
+    // published by lane 0 so that each lane's tangent pass keeps only its own derivative buffer)
+    float* tab = shg + 4 * NQ;
+    float* shL = tab + RneaTab<PL>::SIZE;
     if (!term) {
-        float qdd[NQ], L[NQ * (NQ + 1) / 2];
-        forward_dynamics<PL>(K, qd, u, fext, qdd, L);
+        float qdd[NQ];
+        {
+            float L[NQ * (NQ + 1) / 2];
+            forward_dynamics<PL>(K, qd, u, fext, qdd, L);
+            if (lane == 0) {
+#pragma unroll
+                for (int i = 0; i < NQ * (NQ + 1) / 2; i++) shL[i] = L[i];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        rnea_publish<PL>(K, qd, qdd, fext, lane == 0, tab);
+        wave_sync();
         const int col = lane < NX ? lane : 0;
         const int j = col < NQ ? col : col - NQ;
         const int dir = col < NQ ? 0 : 1;
-        const V6 fj = joint_force<PL>(K, qd, qdd, fext, j);
         float dq[NQ];
-        rnea_tangent<PL>(K, qd, qdd, fj, j, dir, dq);
+        rnea_tangent_tab<PL>(K, qd, qdd, tab, j, dir, dq);
 #pragma unroll
         for (int i = 0; i < NQ; i++) dq[i] = -dq[i];
-        chol_solve<NQ>(L, dq);  // d qdd / d x_col
+        chol_solve<NQ>(shL, dq);  // d qdd / d x_col
         float mc[NQ];
 #pragma unroll
         for (int i = 0; i < NQ; i++) mc[i] = (i == j) ? 1.f : 0.f;
-        chol_solve<NQ>(L, mc);  // M^-1 e_j
+        chol_solve<NQ>(shL, mc);  // M^-1 e_j
         const float h2 = 0.5f * dt * dt;
         if (lane < NX) {  // column `lane` of A (integrator.cuh:143-169)
 #pragma unroll
@@ -197,62 +217,18 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const
     wave_sync();
 }
 
-// ---------------------------------------------------------------- Schur step 1: Q~_k^-1, R_k^-1
-// Gauss-Jordan without pivoting on the nq x nq position block (linalg.cuh:457-519), lane l owns row l.
-template <int PL>
-__device__ __forceinline__ void invert_q_knot(int lane, bool has_r, float rho, const float* Qp, const float* Qv,
-                                              const float* Rg, float* Qip, float* Qiv, float* Ri, float* piv) {
-    constexpr int NQ = Plant<PL>::NQ;
-    float row[2 * NQ];
-#pragma unroll
-    for (int c = 0; c < NQ; c++) {
-        row[c] = lane < NQ ? Qp[lane * NQ + c] + (c == lane ? rho : 0.f) : 0.f;
-        row[NQ + c] = (c == lane) ? 1.f : 0.f;
-    }
-#pragma unroll
-    for (int p = 0; p < NQ; p++) {
-        if (lane == p) {
-            const float inv = 1.f / row[p];
-#pragma unroll
-            for (int c = 0; c < 2 * NQ; c++) {
-                row[c] *= inv;
-                piv[c] = row[c];
-            }
-        }
-        wave_sync();
-        if (lane < NQ && lane != p) {
-            const float f = row[p];
-#pragma unroll
-            for (int c = 0; c < 2 * NQ; c++) row[c] -= f * piv[c];
-        }
-        wave_sync();
-    }
-    if (lane < NQ) {
-#pragma unroll
-        for (int c = 0; c < NQ; c++) Qip[lane * NQ + c] = row[NQ + c];
-        Qiv[lane] = 1.f / Qv[lane];
-        if (has_r) Ri[lane] = 1.f / Rg[lane];
-    }
-}
-
-// Q~^-1[r][m] for block-diagonal Q~^-1 = blkdiag(Qip, diag(Qiv))
-template <int PL>
-__device__ __forceinline__ float qinv_at(const float* Qip, const float* Qiv, int r, int m) {
-    constexpr int NQ = Plant<PL>::NQ;
-    if (r < NQ && m < NQ) return Qip[r * NQ + m];
-    if (r >= NQ && m == r) return Qiv[r - NQ];
-    return 0.f;
-}
-
-// Gauss-Jordan inverse of an nx x nx matrix distributed one row per lane (rows in registers).
+// In-place Gauss-Jordan inverse without pivoting of an n x n matrix held one row per lane
+// (lane r < n owns row r in registers); the inverse overwrites the rows. Equivalent in exact
+// arithmetic to the reference's elimination on [A | I] (linalg.cuh:365-519). piv: n floats of LDS.
 template <int NX>
-__device__ __forceinline__ void gj_rows(int lane, float* row /*2*NX*/, float* piv) {
+__device__ __forceinline__ void gj_inplace(int lane, float* row, float* piv) {
 #pragma unroll
     for (int p = 0; p < NX; p++) {
         if (lane == p) {
             const float inv = 1.f / row[p];
+            row[p] = 1.f;
 #pragma unroll
-            for (int c = 0; c < 2 * NX; c++) {
+            for (int c = 0; c < NX; c++) {
                 row[c] *= inv;
                 piv[c] = row[c];
             }
@@ -260,10 +236,43 @@ __device__ __forceinline__ void gj_rows(int lane, float* row /*2*NX*/, float* pi
         wave_sync();
         if (lane < NX && lane != p) {
             const float f = row[p];
+            row[p] = 0.f;
 #pragma unroll
-            for (int c = 0; c < 2 * NX; c++) row[c] -= f * piv[c];
+            for (int c = 0; c < NX; c++) row[c] -= f * piv[c];
         }
         wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------- Schur step 1: Q~_k^-1, R_k^-1
+// Gauss-Jordan without pivoting on the nq x nq position block (linalg.cuh:457-519), lane l owns row l.
+template <int PL>
+__device__ __forceinline__ void invert_q_knot(int lane, bool has_r, float rho, const float* Qp, const float* Qv,
+                                              const float* Rg, float* Qip, float* Qiv, float* Ri, float* piv) {
+    constexpr int NQ = Plant<PL>::NQ;
+    float row[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; c++) row[c] = lane < NQ ? Qp[lane * NQ + c] + (c == lane ? rho : 0.f) : 0.f;
+    gj_inplace<NQ>(lane, row, piv);
+    if (lane < NQ) {
+#pragma unroll
+        for (int c = 0; c < NQ; c++) Qip[lane * NQ + c] = row[c];
+        Qiv[lane] = 1.f / Qv[lane];
+        if (has_r) Ri[lane] = 1.f / Rg[lane];
+    }
+}
+
+// Row r of the block-diagonal Q~^-1 = blkdiag(Qip, diag(Qiv)), loaded without branches on r.
+template <int PL>
+__device__ __forceinline__ void qinv_row(const float* Qip, const float* Qiv, int r, float* out) {
+    constexpr int NQ = Plant<PL>::NQ;
+    const int rq = r < NQ ? r : 0;
+#pragma unroll
+    for (int m = 0; m < NQ; m++) {
+        const float a = Qip[rq * NQ + m];
+        const float d = Qiv[m];
+        out[m] = r < NQ ? a : 0.f;
+        out[NQ + m] = (NQ + m == r) ? d : 0.f;
     }
 }
 
@@ -296,10 +305,11 @@ __device__ __forceinline__ void schur_row(int lane, float rho, const float* A, c
 #pragma unroll
     for (int c = 0; c < NU; c++) brr[c] = Bm[r * NU + c] * Ri0[c];
     float th[NX];
+    qinv_row<PL>(Qip1, Qiv1, r, th);
     float g = c1[r];
 #pragma unroll
     for (int m = 0; m < NX; m++) {
-        float s = qinv_at<PL>(Qip1, Qiv1, r, m);
+        float s = th[m];
         g -= s * q1[m];
 #pragma unroll
         for (int l = 0; l < NX; l++) s += phi[l] * A[m * NX + l];
@@ -319,13 +329,10 @@ __device__ __forceinline__ void schur_row(int lane, float rho, const float* A, c
         }
     }
 #pragma unroll
-    for (int m = 0; m < NX; m++) {
-        prow[m] = th[m] + ((m == r && m < NQ) ? rho : 0.f);
-        prow[NX + m] = (m == r) ? 1.f : 0.f;
-    }
-    gj_rows<NX>(lane, prow, piv);
+    for (int m = 0; m < NX; m++) prow[m] = th[m] + ((m == r && m < NQ) ? rho : 0.f);
+    gj_inplace<NX>(lane, prow, piv);
 #pragma unroll
-    for (int m = 0; m < NX; m++) prow[m] = -prow[NX + m];
+    for (int m = 0; m < NX; m++) prow[m] = -prow[m];
     if (lane < NX) {
 #pragma unroll
         for (int m = 0; m < NX; m++) Pd1[r * NX + m] = prow[m];
@@ -342,9 +349,11 @@ __device__ __forceinline__ void schur_row0(int lane, float rho, const float* Qp0
     if (lane >= NX) return;
     const int r = lane;
     float g = c0[r];
+    float qir[NX];
+    qinv_row<PL>(Qip0, Qiv0, r, qir);
 #pragma unroll
     for (int m = 0; m < NX; m++) {
-        const float qi = qinv_at<PL>(Qip0, Qiv0, r, m);
+        const float qi = qir[m];
         g -= qi * q0[m];
         Sd0[r * NX + m] = -qi;
         float qt = 0.f;

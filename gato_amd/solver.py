@@ -146,6 +146,26 @@ class Solver:
         }
         return out
 
+    def solve_device(self, xu, xs, ref, dt=None, stats=False):
+        """Solve in place on device tensors (B x traj, B x nx, B x 6N float32, contiguous) on the
+        current stream, without host copies of the trajectory. Returns the stats dict if asked."""
+        for t, n, name in ((xu, self.B * self.traj, "xu"), (xs, self.B * self.nx, "xs"), (ref, self.B * 6 * self.N, "ref")):
+            if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+                    and t.numel() == n):
+                raise ValueError(f"{name}: expected a contiguous float32 device tensor with {n} elements")
+        dt = self.cfg["dt"] if dt is None else dt
+        st = self._st
+        gs = None
+        if stats:
+            gs = GatoStats(0.0, 0, 0, *[st[k].ctypes.data_as(C.POINTER(C.c_int32 if st[k].dtype == np.int32 else C.c_float))
+                                        for k in ("sqp_iters", "kkt_converged", "initial_merit", "final_merit",
+                                                  "pcg_iters", "ls_min_merit", "ls_step_size", "kkt_residual")])
+        check(self.lib.gato_solve(self._h, _ptr(xu), _ptr(xs), _ptr(ref), float(dt), self._stream(),
+                                  C.byref(gs) if gs is not None else None), "gato_solve")
+        if stats:
+            return {k: v.copy() for k, v in st.items()} | {"n_iters": gs.n_iters, "sqp_time_us": gs.solve_time_us}
+        return None
+
     def set_f_ext_batch(self, f_ext_batch):
         f = torch.zeros(self.B * 6, dtype=torch.float32, device=self.device)
         self._to_dev(f_ext_batch, f, "f_ext_batch")
