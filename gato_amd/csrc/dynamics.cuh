@@ -162,13 +162,31 @@ struct Kin {
     }
 };
 
-// RNEA: tau = ID(q, qd, qdd) with gravity as base acceleration, f_ext (link-local) on the last link.
-// Optionally keeps per-link v, a and the accumulated joint forces f (transmitted across joint i).
-template <int PL, bool KEEP>
-__device__ __forceinline__ void rnea(const Kin<PL>& K, const float* qd, const float* qdd, const float* fext,
-                                     float* tau, V6* vk = nullptr, V6* ak = nullptr, V6* fk = nullptr) {
-    constexpr int NQ = Plant<PL>::NQ;
+// Where the RNEA keeps its per-link forces between the forward and backward sweeps.
+template <int NQ>
+struct FRegs {  // registers (6 NQ floats per lane)
     V6 f[NQ];
+    __device__ __forceinline__ V6 get(int i) const { return f[i]; }
+    __device__ __forceinline__ void set(int i, const V6& v) { f[i] = v; }
+};
+struct FLds {  // LDS: component c of link i at p[(6 i + c) * cs]
+    float* p;
+    int cs;
+    __device__ __forceinline__ V6 get(int i) const {
+        const float* q = p + 6 * i * cs;
+        return V6{q[0], q[cs], q[2 * cs], q[3 * cs], q[4 * cs], q[5 * cs]};
+    }
+    __device__ __forceinline__ void set(int i, const V6& v) {
+        float* q = p + 6 * i * cs;
+        q[0] = v.w0; q[cs] = v.w1; q[2 * cs] = v.w2; q[3 * cs] = v.l0; q[4 * cs] = v.l1; q[5 * cs] = v.l2;
+    }
+};
+
+// RNEA: tau = ID(q, qd, qdd) with gravity as base acceleration, f_ext (link-local) on the last link.
+template <int PL, typename FS>
+__device__ __forceinline__ void rnea_fs(const Kin<PL>& K, const float* qd, const float* qdd, const float* fext,
+                                        float* tau, FS& fs) {
+    constexpr int NQ = Plant<PL>::NQ;
     V6 v = v6zero(), a = V6{0.f, 0.f, 0.f, 0.f, 0.f, kGravity};
     Unroll<0, NQ>::run([&](auto ic) {
         constexpr int i = decltype(ic)::value;
@@ -177,19 +195,26 @@ __device__ __forceinline__ void rnea(const Kin<PL>& K, const float* qd, const fl
         v.w2 += qd[i];
         a = a + crm_s(v, qd[i]);
         if (qdd) a.w2 += qdd[i];
-        if (KEEP) { vk[i] = v; ak[i] = a; }
-        f[i] = imul<PL, i>(a) + crf(v, imul<PL, i>(v));
+        V6 f = imul<PL, i>(a) + crf(v, imul<PL, i>(v));
+        if (i == NQ - 1 && fext) {
+            f.w0 -= fext[0]; f.w1 -= fext[1]; f.w2 -= fext[2];
+            f.l0 -= fext[3]; f.l1 -= fext[4]; f.l2 -= fext[5];
+        }
+        fs.set(i, f);
     });
-    if (fext) {
-        f[NQ - 1].w0 -= fext[0]; f[NQ - 1].w1 -= fext[1]; f[NQ - 1].w2 -= fext[2];
-        f[NQ - 1].l0 -= fext[3]; f[NQ - 1].l1 -= fext[4]; f[NQ - 1].l2 -= fext[5];
-    }
+    V6 fa = fs.get(NQ - 1);
     UnrollDown<0, NQ>::run([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        tau[i] = f[i].w2;
-        if (KEEP) fk[i] = f[i];
-        if constexpr (i > 0) f[i - 1] = f[i - 1] + xfrc<PL, i>(K.s[i], K.c[i], f[i]);
+        tau[i] = fa.w2;
+        if constexpr (i > 0) fa = fs.get(i - 1) + xfrc<PL, i>(K.s[i], K.c[i], fa);
     });
+}
+
+template <int PL, bool KEEP>
+__device__ __forceinline__ void rnea(const Kin<PL>& K, const float* qd, const float* qdd, const float* fext,
+                                     float* tau) {
+    FRegs<Plant<PL>::NQ> fs;
+    rnea_fs<PL>(K, qd, qdd, fext, tau, fs);
 }
 
 // Rigid-body inertia (m, h, Ib) transported from joint frame I to its parent frame I-1.
@@ -318,7 +343,21 @@ __device__ __forceinline__ void chol_solve(const float* L, float* b) {
     }
 }
 
-// qdd = M^-1 (u - c); returns the Cholesky factor in L (packed) when asked.
+// qdd = M^-1 (u - c); returns the Cholesky factor in L (packed). FS keeps the RNEA link forces.
+template <int PL, typename FS>
+__device__ __forceinline__ void forward_dynamics_fs(const Kin<PL>& K, const float* qd, const float* u,
+                                                    const float* fext, float* qdd, float* L, FS& fs) {
+    constexpr int NQ = Plant<PL>::NQ;
+    float cb[NQ];
+    rnea_fs<PL>(K, qd, nullptr, fext, cb, fs);
+    __builtin_amdgcn_sched_barrier(0);  // keep the RNEA and CRBA live ranges apart
+    crba<PL>(K, L);
+    chol_packed<NQ>(L);
+#pragma unroll
+    for (int i = 0; i < NQ; i++) qdd[i] = u[i] - cb[i];
+    chol_solve<NQ>(L, qdd);
+}
+
 template <int PL>
 __device__ __forceinline__ void forward_dynamics(const Kin<PL>& K, const float* qd, const float* u,
                                                  const float* fext, float* qdd, float* L) {
@@ -350,12 +389,33 @@ struct RneaTab {
 };
 
 // RNEA at (q, qd, qdd) with f_ext; the `writer` lane publishes v_i, a_i and the transmitted forces.
+// Value barrier: the optimiser may not assume x is unchanged, so sub-expressions computed from it
+// are not merged with earlier ones (which would keep those earlier values alive in registers).
+__device__ __forceinline__ float opaque(float x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+__device__ __forceinline__ int opaque(int x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+// Every lane of the knot group computes identical values and stores them to the same addresses,
+// so the table needs no ordering beyond the lane's own program order.
 template <int PL>
-__device__ __forceinline__ void rnea_publish(const Kin<PL>& K, const float* qd, const float* qdd, const float* fext,
-                                             bool writer, float* tab) {
+__device__ __forceinline__ void rnea_publish(const Kin<PL>& K0, const float* qd0, const float* qdd, const float* fext,
+                                             float* tab) {
     constexpr int NQ = Plant<PL>::NQ;
     using T = RneaTab<PL>;
-    V6 f[NQ];
+    // recompute from opaque copies instead of reusing the forward-dynamics sweep's values
+    Kin<PL> K;
+    float qd[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) {
+        K.s[i] = opaque(K0.s[i]);
+        K.c[i] = opaque(K0.c[i]);
+        qd[i] = opaque(qd0[i]);
+    }
     V6 v = v6zero(), a = V6{0.f, 0.f, 0.f, 0.f, 0.f, kGravity};
     Unroll<0, NQ>::run([&](auto ic) {
         constexpr int i = decltype(ic)::value;
@@ -364,17 +424,22 @@ __device__ __forceinline__ void rnea_publish(const Kin<PL>& K, const float* qd, 
         v.w2 += qd[i];
         a = a + crm_s(v, qd[i]);
         a.w2 += qdd[i];
-        if (writer) { T::put(tab, i, v); T::put(tab, NQ + i, a); }
-        f[i] = imul<PL, i>(a) + crf(v, imul<PL, i>(v));
+        T::put(tab, i, v);
+        T::put(tab, NQ + i, a);
+        V6 f = imul<PL, i>(a) + crf(v, imul<PL, i>(v));
+        if (i == NQ - 1 && fext) {
+            f.w0 -= fext[0]; f.w1 -= fext[1]; f.w2 -= fext[2];
+            f.l0 -= fext[3]; f.l1 -= fext[4]; f.l2 -= fext[5];
+        }
+        T::put(tab, 2 * NQ + i, f);
     });
-    if (fext) {
-        f[NQ - 1].w0 -= fext[0]; f[NQ - 1].w1 -= fext[1]; f[NQ - 1].w2 -= fext[2];
-        f[NQ - 1].l0 -= fext[3]; f[NQ - 1].l1 -= fext[4]; f[NQ - 1].l2 -= fext[5];
-    }
+    V6 fa = T::get(tab, 3 * NQ - 1);
     UnrollDown<0, NQ>::run([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        if (writer) T::put(tab, 2 * NQ + i, f[i]);
-        if constexpr (i > 0) f[i - 1] = f[i - 1] + xfrc<PL, i>(K.s[i], K.c[i], f[i]);
+        if constexpr (i > 0) {
+            fa = T::get(tab, 2 * NQ + i - 1) + xfrc<PL, i>(K.s[i], K.c[i], fa);
+            T::put(tab, 2 * NQ + i - 1, fa);
+        }
     });
 }
 
@@ -417,6 +482,71 @@ __device__ __forceinline__ void rnea_tangent_tab(const Kin<PL>& K, const float* 
             if (i == j && dir == 0) g = g + s_crf(fj);
             df[i - 1] = df[i - 1] + xfrc<PL, i>(K.s[i], K.c[i], g);
         }
+    });
+}
+
+// Joint axes carried into every later link frame: S_i^(k) = X_k ... X_{i+1} S_i for i <= k, stored at
+// slot k(k+1)/2 + i of an 8-float-stride table (identical in every lane of a knot group).
+template <int PL>
+__device__ __forceinline__ void publish_axes(const Kin<PL>& K, float* ax) {
+    constexpr int NQ = Plant<PL>::NQ;
+    using T = RneaTab<PL>;
+    Unroll<0, NQ>::run([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        T::put(ax, tri(k, k), V6{0.f, 0.f, 1.f, 0.f, 0.f, 0.f});
+        Unroll<0, k>::run([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            T::put(ax, tri(k, i), xmot<PL, k>(K.s[k], K.c[k], T::get(ax, tri(k - 1, i))));
+        });
+    });
+}
+
+__device__ __forceinline__ float dot6(const V6& m, const V6& f) {
+    return m.w0 * f.w0 + m.w1 * f.w1 + m.w2 * f.w2 + m.l0 * f.l0 + m.l1 * f.l1 + m.l2 * f.l2;
+}
+
+// Tangent of ID along q_j (dir 0) or qd_j (dir 1), qdd fixed, as a single forward sweep:
+//   d tau_i = sum_{k >= i} (S_i^(k))^T g_k,  g_k = dI_k-force of link k  (+ S x* F_j at k = j for dir 0),
+// which equals the backward accumulation dF_{k-1} += X_k^T (dF_k + [k = j] S x* F_j) of rnea_tangent
+// (the extra term contributes (S_j)^T (S x* F_j) = 0 to d tau_j itself).
+template <int PL>
+__device__ __forceinline__ void rnea_tangent_ax(const Kin<PL>& K, const float* qd, const float* qdd,
+                                                const float* tab0, const float* ax0, int j, int dir, float* dtau) {
+    constexpr int NQ = Plant<PL>::NQ;
+    using T = RneaTab<PL>;
+    // read the tables through an offset the optimiser cannot see through: the values are reloaded
+    // link by link instead of being forwarded from the sweep that stored them
+    int off = 0;
+    asm volatile("" : "+v"(off));
+    const float* tab = tab0 + off;
+    const float* ax = ax0 + off;
+    const V6 fj = T::get(tab, 2 * NQ + j);
+    // the seed enters through 0/1 factors (no per-link branches: a run-time j would otherwise make the
+    // compiler carry both variants of every link's terms)
+    const float wq = dir == 0 ? 1.f : 0.f, wv = 1.f - wq;
+    V6 dv = v6zero(), da = v6zero();
+#pragma unroll
+    for (int i = 0; i < NQ; i++) dtau[i] = 0.f;
+    Unroll<0, NQ>::run([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        asm volatile("" ::: "memory");  // one link at a time: table loads are neither hoisted nor merged
+        const V6 v = T::get(tab, k);
+        const V6 a = T::get(tab, NQ + k);
+        V6 xv = v;  // X_k v_{k-1}
+        xv.w2 -= qd[k];
+        V6 xa = a - crm_s(v, qd[k]);  // X_k a_{k-1}
+        xa.w2 -= qdd[k];
+        const float h = (k == j) ? 1.f : 0.f;
+        const float hq = h * wq, hv = h * wv;
+        dv = xmot<PL, k>(K.s[k], K.c[k], dv) - hq * s_crm(xv);
+        dv.w2 += hv;
+        da = xmot<PL, k>(K.s[k], K.c[k], da) - hq * s_crm(xa) + crm_s(dv, qd[k]) +
+             hv * V6{v.w1, -v.w0, 0.f, v.l1, -v.l0, 0.f};  // v_k x S
+        const V6 g = imul<PL, k>(da) + crf(dv, imul<PL, k>(v)) + crf(v, imul<PL, k>(dv)) + hq * s_crf(fj);
+        Unroll<0, k + 1>::run([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            dtau[i] += dot6(T::get(ax, tri(k, i)), g);
+        });
     });
 }
 

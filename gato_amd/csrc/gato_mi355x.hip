@@ -19,6 +19,11 @@
 #include "../../include/gato_mi355x.h"
 #include "sqp_phases.cuh"
 
+// Work-groups are up to 512 lanes (8 wavefronts, two per SIMD). Asking for at least four waves per SIMD
+// caps every phase kernel at 128 VGPRs, so two problems share a CU; left free, the scheduler spends
+// registers on latency hiding and drops to one work-group per CU.
+#define GATO_TWO_WG_PER_CU __attribute__((amdgpu_waves_per_eu(4)))
+
 namespace gato {
 
 struct Args {
@@ -76,7 +81,7 @@ struct View {  // per-problem pointers
 
 // ------------------------------------------------------------------ phases on one problem
 template <int PL>
-__device__ void phase_lin(const Args& a, int b, float* sm) {
+__device__ __forceinline__ void phase_lin(const Args& a, int b, float* sm) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
     View<PL> v(a, b);
@@ -84,15 +89,18 @@ __device__ void phase_lin(const Args& a, int b, float* sm) {
     float* shg = sm + g * lin_lds_floats<PL>();
     const float* xt = v.xu + (size_t)(a.cw.terminal_mode ? N - 1 : N - 2) * (NX + NU);
     for (int k = g; k < N; k += G) {
+        // the lane index is re-laundered per knot so that lane-only constants of the knot routine are
+        // recomputed inside the loop rather than hoisted out of it and held in registers throughout
+        const int ln = opaque(lane);
         const float* xk = v.xu + (size_t)(k < N - 1 ? k : 0) * (NX + NU);
-        lin_knot<PL>(k, N, lane, a.dt, a.cw, xk, xt, v.ref + 6 * k, v.xref ? v.xref + k * NX : nullptr, v.Qd, v.Rd,
+        lin_knot<PL>(k, N, ln, a.dt, a.cw, xk, xt, v.ref + 6 * k, v.xref ? v.xref + k * NX : nullptr, v.Qd, v.Rd,
                      v.fext, v.xu, v.xs, v.A + (size_t)k * NX * NX, v.Bm + (size_t)k * NX * NU, v.c + (k + 1) * NX,
                      v.c, v.Qp + k * NQ * NQ, v.Qv + k * NQ, v.Rg + k * NU, v.q + k * NX, v.r + k * NU, shg);
     }
 }
 
 template <int PL>
-__device__ void phase_schur(const Args& a, int b, float rho, float* sm) {
+__device__ __forceinline__ void phase_schur(const Args& a, int b, float rho, float* sm) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
     View<PL> v(a, b);
@@ -156,7 +164,7 @@ __device__ __forceinline__ float btd_row(const float* D, const float* L, const f
 
 // Warm-started PCG on S lambda = gamma (pcg.cuh:13-148). Returns the iteration count.
 template <int PL, int NPASS>
-__device__ int phase_pcg(const Args& a, int b, float eps, float* sm) {
+__device__ __forceinline__ int phase_pcg(const Args& a, int b, float eps, float* sm) {
     constexpr int NX = Plant<PL>::NX, XP = 16;
     View<PL> v(a, b);
     const int N = a.N, tid = threadIdx.x, g = tid / KG, lane = tid % KG, G = blockDim.x / KG;
@@ -248,7 +256,7 @@ __device__ __forceinline__ float block_sum1(float v, float* red) {
 // solve (one knot per knot group: N <= 32) and four work-group barriers per iteration.
 // The sums run in the same order as btd_row, so the results are identical to phase_pcg.
 template <int PL>
-__device__ int phase_pcg_reg(const Args& a, int b, float eps, float* sm) {
+__device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, float* sm) {
     constexpr int NX = Plant<PL>::NX, XP = 16;
     View<PL> v(a, b);
     const int N = a.N, tid = threadIdx.x, k = tid / KG, lane = tid % KG;
@@ -279,16 +287,23 @@ __device__ int phase_pcg_reg(const Args& a, int b, float eps, float* sm) {
             pu[c] = hasu ? PU[c * NX] : 0.f;
         }
     }
-    auto mv = [&](const float* M0, const float* M1, const float* M2, const float* x) {
-        const float* xk = x + (kk + 1) * XP;
-        float s = 0.f;
+    // block-tridiagonal row times the padded LDS vector: 128-bit loads of the three 16-float blocks
+    auto blk = [&](const float* M, const float4* xb, float s) {
+        float xs[XP];
 #pragma unroll
-        for (int c = 0; c < NX; c++) s += M0[c] * xk[c];
+        for (int q = 0; q < XP / 4; q++) {
+            const float4 a = xb[q];
+            xs[4 * q] = a.x; xs[4 * q + 1] = a.y; xs[4 * q + 2] = a.z; xs[4 * q + 3] = a.w;
+        }
 #pragma unroll
-        for (int c = 0; c < NX; c++) s += M1[c] * xk[c - XP];
-#pragma unroll
-        for (int c = 0; c < NX; c++) s += M2[c] * xk[c + XP];
+        for (int c = 0; c < NX; c++) s += M[c] * xs[c];
         return s;
+    };
+    auto mv = [&](const float* M0, const float* M1, const float* M2, const float* x) {
+        const float4* xk = reinterpret_cast<const float4*>(x + (kk + 1) * XP);
+        float s = blk(M0, xk, 0.f);
+        s = blk(M1, xk - XP / 4, s);
+        return blk(M2, xk + XP / 4, s);
     };
     float x = val ? v.lam[kk * NX + l] : 0.f;
     __syncthreads();
@@ -336,7 +351,7 @@ __device__ __forceinline__ int pcg_dispatch(const Args& a, int b, float eps, flo
 }
 
 template <int PL>
-__device__ float phase_dz(const Args& a, int b, float* sm) {
+__device__ __forceinline__ float phase_dz(const Args& a, int b, float* sm) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
     View<PL> v(a, b);
@@ -357,7 +372,7 @@ __device__ float phase_dz(const Args& a, int b, float* sm) {
 // then fixed-order sums over knots. out[j] (LDS) receives phi(alpha_j). The reference's
 // initial / final merit (dz = 0, one alpha) is nal = 1.
 template <int PL>
-__device__ void phase_merit(const Args& a, int b, float mu, int nal, float* out) {
+__device__ __forceinline__ void phase_merit(const Args& a, int b, float mu, int nal, float* out) {
     using P = Plant<PL>;
     constexpr int NX = P::NX, NU = P::NU;
     View<PL> v(a, b);
@@ -381,7 +396,7 @@ __device__ void phase_merit(const Args& a, int b, float mu, int nal, float* out)
 
 // line search + rho adaptation + update (line_search.cuh:12-98). merits in LDS; returns via shared.
 template <int PL>
-__device__ void phase_ls(const Args& a, int b, int it, const float* merits, float* shs) {
+__device__ __forceinline__ void phase_ls(const Args& a, int b, int it, const float* merits, float* shs) {
     constexpr int NX = Plant<PL>::NX, NU = Plant<PL>::NU;
     View<PL> v(a, b);
     if (threadIdx.x == 0) {
@@ -415,9 +430,9 @@ __device__ void phase_ls(const Args& a, int b, int it, const float* merits, floa
 
 // ------------------------------------------------------------------ plain path kernels
 template <int PL>
-__global__ void __launch_bounds__(512) k_merit(Args a, int nal, int which /*0 init, 1 final, 2 line search*/,
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_merit(Args a, int nal, int which /*0 init, 1 final, 2 line search*/,
                                                int it) {
-    extern __shared__ float sm[];
+    extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
     if (which == 2 && a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
     phase_merit<PL>(a, b, a.mu[b], nal, sm);
@@ -429,24 +444,24 @@ __global__ void __launch_bounds__(512) k_merit(Args a, int nal, int which /*0 in
 }
 
 template <int PL>
-__global__ void __launch_bounds__(512) k_lin(Args a) {
-    extern __shared__ float sm[];
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_lin(Args a) {
+    extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
     if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
     phase_lin<PL>(a, b, sm);
 }
 
 template <int PL>
-__global__ void __launch_bounds__(512) k_schur(Args a) {
-    extern __shared__ float sm[];
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_schur(Args a) {
+    extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
     if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
     phase_schur<PL>(a, b, a.rho[b], sm);
 }
 
 template <int PL, int NPASS>
-__global__ void __launch_bounds__(512) k_pcg(Args a, int it) {
-    extern __shared__ float sm[];
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg(Args a, int it) {
+    extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
     const bool skip = (a.exit_rule == GATO_EXIT_KKT) ? !a.active[b] : a.conv[b];  // quirk Q9 (pcg.cuh:30-33)
     int n = 0;
@@ -455,8 +470,8 @@ __global__ void __launch_bounds__(512) k_pcg(Args a, int it) {
 }
 
 template <int PL>
-__global__ void __launch_bounds__(512) k_dz(Args a, int it) {
-    extern __shared__ float sm[];
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_dz(Args a, int it) {
+    extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
     if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) {
         if (threadIdx.x == 0) a.st_kres[it * a.B + b] = 0.f;
@@ -517,7 +532,7 @@ __global__ void k_end(Args a) {
 // the reference instead keeps stepping converged problems until the whole batch has converged).
 template <int PL, int NPASS>
 __global__ void __launch_bounds__(512) k_fused(Args a) {
-    extern __shared__ float sm[];
+    extern __shared__ __align__(16) float sm[];
     __shared__ float shs[16];
     __shared__ int shi[4];
     constexpr int NX = Plant<PL>::NX, NU = Plant<PL>::NU;
@@ -700,13 +715,15 @@ size_t layout(gato_solver* s, bool assign) {
     return off + 256;
 }
 
-int grid_threads(int N) { return gato::KG * std::min(N, 32); }
+// one 16-lane group per knot (at most 32 groups), rounded up to whole wavefronts: the DPP reductions
+// read all 64 lanes of every wave, and surplus groups own no knot
+int grid_threads(int N) { return 64 * ((std::min(N, 32) + 3) / 4); }
 int npass(int N) { return (N + 31) / 32; }
 
 size_t lds_bytes(const gato_solver* s) {
-    const int G = std::min(s->N, 32), nx = s->nx, nq = s->nq;
+    const int G = grid_threads(s->N) / gato::KG, nx = s->nx, nq = s->nq;
     size_t f = 0;
-    f = std::max<size_t>(f, (size_t)G * (28 * nq + nq * (nq + 1) / 2));  // lin
+    f = std::max<size_t>(f, (size_t)G * (24 * nq + 9 * (nq * (nq + 1) / 2)));  // lin
     f = std::max<size_t>(f, (size_t)G * (2 * nx + nx * nx));        // schur
     f = std::max<size_t>(f, (size_t)2 * (s->N + 2) * 16 + 32);      // pcg
     f = std::max<size_t>(f, (size_t)G * gato::KG + 16);             // dz

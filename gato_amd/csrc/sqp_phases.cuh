@@ -87,11 +87,13 @@ __device__ __forceinline__ float plim(int which, int i) {
     return r;
 }
 
-// LDS floats per knot group used by lin_knot: J / gradient scratch, RNEA table, Cholesky factor
+// LDS floats per knot group used by lin_knot: RNEA table (24 nq), Cholesky factor (nq(nq+1)/2) and the
+// transported-axes table (8 nq(nq+1)/2), which also hosts the RNEA force buffer before it is built and
+// the gradient / Jacobian scratch after the tangents are done.
 template <int PL>
 __host__ __device__ constexpr int lin_lds_floats() {
     constexpr int NQ = Plant<PL>::NQ;
-    return 4 * NQ + 24 * NQ + NQ * (NQ + 1) / 2;
+    return 24 * NQ + 9 * (NQ * (NQ + 1) / 2);
 }
 
 // ---------------------------------------------------------------- KKT setup, one knot
@@ -119,26 +121,22 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const
     K.set(q);
     // knot-group LDS: RNEA table, Cholesky factor of M (identical in every lane of the group,
     // published by lane 0 so that each lane's tangent pass keeps only its own derivative buffer)
-    float* tab = shg + 4 * NQ;
+    float* tab = shg;
     float* shL = tab + RneaTab<PL>::SIZE;
+    float* ax = shL + NQ * (NQ + 1) / 2;
     if (!term) {
         float qdd[NQ];
-        {
-            float L[NQ * (NQ + 1) / 2];
-            forward_dynamics<PL>(K, qd, u, fext, qdd, L);
-            if (lane == 0) {
-#pragma unroll
-                for (int i = 0; i < NQ * (NQ + 1) / 2; i++) shL[i] = L[i];
-            }
-        }
+        FLds fs{ax, 1};
+        forward_dynamics_fs<PL>(K, qd, u, fext, qdd, shL, fs);
         __builtin_amdgcn_sched_barrier(0);
-        rnea_publish<PL>(K, qd, qdd, fext, lane == 0, tab);
+        rnea_publish<PL>(K, qd, qdd, fext, tab);
+        publish_axes<PL>(K, ax);
         wave_sync();
         const int col = lane < NX ? lane : 0;
         const int j = col < NQ ? col : col - NQ;
         const int dir = col < NQ ? 0 : 1;
         float dq[NQ];
-        rnea_tangent_tab<PL>(K, qd, qdd, tab, j, dir, dq);
+        rnea_tangent_ax<PL>(K, qd, qdd, tab, ax, j, dir, dq);
 #pragma unroll
         for (int i = 0; i < NQ; i++) dq[i] = -dq[i];
         chol_solve<NQ>(shL, dq);  // d qdd / d x_col
@@ -146,6 +144,13 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const
 #pragma unroll
         for (int i = 0; i < NQ; i++) mc[i] = (i == j) ? 1.f : 0.f;
         chol_solve<NQ>(shL, mc);  // M^-1 e_j
+        // materialise both columns here: otherwise the compiler sinks the whole tangent sweep into the
+        // lane-predicated stores below and keeps every intermediate of it live across the branch
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+            dq[i] = opaque(dq[i]);
+            mc[i] = opaque(mc[i]);
+        }
         const float h2 = 0.5f * dt * dt;
         if (lane < NX) {  // column `lane` of A (integrator.cuh:143-169)
 #pragma unroll
@@ -164,16 +169,26 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const
             }
         }
         if (lane < NQ) {  // c_{k+1} = x_{k+1} - x+ (integrator.cuh:49-62)
-            const float ql = pick(q, NQ, lane), qdl = pick(qd, NQ, lane), al = pick(qdd, NQ, lane);
+            const float ql = xsrc[lane], qdl = xsrc[NQ + lane], al = pick(qdd, NQ, lane);
             const float* xn = xk + NX + NU;
             cn[lane] = xn[lane] - (ql + dt * qdl + 0.5f * al * dt * dt);
             cn[NQ + lane] = xn[NQ + lane] - (qdl + dt * al);
         }
     }
-    // cost model (iiwa14_plant.cuh:339-424); joint-space diagonal weights generalise qd_cost/u_cost
+    // cost model (iiwa14_plant.cuh:339-424); joint-space diagonal weights generalise qd_cost/u_cost.
+    // Lane scalars are re-read from memory and the kinematics restart from opaque copies, so nothing of
+    // the dynamics block above stays live into this section.
     const float w = term ? (cw.terminal_mode ? cw.N_cost : cw.q_cost) : cw.q_cost;
     float p[3], Jc[3];
-    ee_position<PL>(K, lane < NQ ? lane : -1, p, Jc);
+    {
+        Kin<PL> K2;
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+            K2.s[i] = opaque(K.s[i]);
+            K2.c[i] = opaque(K.c[i]);
+        }
+        ee_position<PL>(K2, lane < NQ ? lane : -1, p, Jc);
+    }
     const float e0 = p[0] - refk[0], e1 = p[1] - refk[1], e2 = p[2] - refk[2];
     const float gl = Jc[0] * e0 + Jc[1] * e1 + Jc[2] * e2;
     if (lane < NQ) {
@@ -182,7 +197,7 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const
     }
     wave_sync();
     if (lane < NQ) {
-        const float ql = pick(q, NQ, lane), qdl = pick(qd, NQ, lane);
+        const float ql = xsrc[lane], qdl = xsrc[NQ + lane];
         const float bgl = barrier_g<PL>(ql, plim<PL>(0, lane), plim<PL>(1, lane));
 #pragma unroll
         for (int m = 0; m < NQ; m++) {
@@ -191,7 +206,7 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const
                 v = w * (Jc[0] * shg[NQ + 3 * m] + Jc[1] * shg[NQ + 3 * m + 1] + Jc[2] * shg[NQ + 3 * m + 2]);
             else
                 v = w * gl * shg[m];  // rank-1 EE term (quirk Q5)
-            if (PL == 1) v += cw.q_lim * bgl * barrier_g<PL>(q[m], P::qlo(m), P::qhi(m));
+            if (PL == 1) v += cw.q_lim * bgl * barrier_g<PL>(xsrc[m], P::qlo(m), P::qhi(m));
             if (m == lane) {
                 v += Qd[lane];
                 if (PL != 1) v += cw.q_lim * barrier_h(ql, plim<PL>(0, lane), plim<PL>(1, lane));
@@ -204,7 +219,7 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const
         qv[NQ + lane] = Qd[NQ + lane] * (qdl - xr_v) + cw.vel_lim * bv;
         Qv[lane] = Qd[NQ + lane] + cw.vel_lim * (PL == 1 ? bv * bv : barrier_h(qdl, plim<PL>(2, lane), plim<PL>(3, lane)));
         if (!term) {
-            const float ul = pick(u, NQ, lane);
+            const float ul = xk[NX + lane];
             const float bu = barrier_g<PL>(ul, plim<PL>(4, lane), plim<PL>(5, lane));
             rv[lane] = Rd[lane] * ul + cw.ctrl_lim * bu;
             Rg[lane] = Rd[lane] + cw.ctrl_lim * (PL == 1 ? bu * bu : barrier_h(ul, plim<PL>(4, lane), plim<PL>(5, lane)));
@@ -499,15 +514,34 @@ __device__ __forceinline__ float merit_item(int k, int N, float alpha, float dt,
 }
 
 // ---------------------------------------------------------------- work-group reductions (deterministic)
+// Wave64 reductions on the DPP network: butterflies within each row of 16 lanes (quad_perm,
+// row_half_mirror, row_mirror), then the four row results combined through readlane. The result is
+// wave-uniform. All 64 lanes must be active.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float lane_f(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+constexpr int DPP_QUAD_1032 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int DPP_QUAD_2301 = 0x4E;   // quad_perm [2,3,0,1]
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;
+constexpr int DPP_ROW_MIRROR = 0x140;
+
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_f<DPP_QUAD_1032>(v);
+    v += dpp_f<DPP_QUAD_2301>(v);
+    v += dpp_f<DPP_ROW_HALF_MIRROR>(v);
+    v += dpp_f<DPP_ROW_MIRROR>(v);
+    return (lane_f(v, 0) + lane_f(v, 16)) + (lane_f(v, 32) + lane_f(v, 48));
 }
 __device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
+    v = fmaxf(v, dpp_f<DPP_QUAD_1032>(v));
+    v = fmaxf(v, dpp_f<DPP_QUAD_2301>(v));
+    v = fmaxf(v, dpp_f<DPP_ROW_HALF_MIRROR>(v));
+    v = fmaxf(v, dpp_f<DPP_ROW_MIRROR>(v));
+    return fmaxf(fmaxf(lane_f(v, 0), lane_f(v, 16)), fmaxf(lane_f(v, 32), lane_f(v, 48)));
 }
 
 // Sum over the work-group; red holds one float per wave. Every thread returns the total.

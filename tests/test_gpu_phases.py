@@ -1,0 +1,134 @@
+"""GPU phase parity: each phase of the HIP path against the FP64 oracle run on the SAME inputs
+(the GPU's own fp32 inputs to that phase), so each comparison isolates one phase's arithmetic.
+
+Tolerances (relative to the largest entry of the reference array):
+  * KKT blocks, merit: 1e-5 (fp32 rounding of O(10^3) flops per entry);
+  * Schur blocks S: 1e-4; preconditioner P and gamma: 2e-3 - P_kk = -(theta + rho E)^-1 amplifies
+    fp32 rounding by cond(theta) ~ 1e3-1e4 at rho = 1e-3;
+  * PCG lambda: 1e-2 and iteration counts within 2 - 100 iterations of fp32 CG on a system with
+    cond(PS) ~ 1e3 lose orthogonality at that level; dz: 1e-2 for the same reason.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from gato_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver_and_inputs(plant, N, B, seed=5):
+    from gato_amd.solver import Solver
+
+    wl = W.drop_in_figure8(plant, N=N, B=B)
+    p = wl.params
+    rng = np.random.default_rng(seed)
+    nx = 2 * W.NQ[plant]
+    xu = wl.xu + rng.normal(0, 0.02, wl.xu.shape)
+    xu[:, :nx] = wl.xs
+    kw = {k: p[k] for k in ("dt", "max_sqp_iters", "kkt_tol", "max_pcg_iters", "pcg_tol", "solve_ratio", "mu",
+                            "q_cost", "qd_cost", "u_cost", "N_cost", "q_lim_cost", "vel_lim_cost", "ctrl_lim_cost",
+                            "rho")}
+    s = Solver(plant, N, B, mode="plain", **kw)
+    return s, wl, xu.astype(np.float32).astype(np.float64)
+
+
+def _cfg(O, plant, wl):
+    p = wl.params
+    return O.make_cfg(plant, N=wl.N, dt=p["dt"], max_sqp_iters=p["max_sqp_iters"], max_pcg_iters=p["max_pcg_iters"],
+                      q_cost=p["q_cost"], qd_cost=p["qd_cost"], u_cost=p["u_cost"], N_cost=p["N_cost"],
+                      q_lim_cost=p["q_lim_cost"])
+
+
+def _gpu_kkt(s, B, N, nq):
+    nx, nu = 2 * nq, nq
+    A = s.debug_array("A").reshape(B, N, nx, nx)
+    Bm = s.debug_array("B").reshape(B, N, nx, nu)
+    c = s.debug_array("c").reshape(B, N, nx)
+    Qp = s.debug_array("Qp").reshape(B, N, nq, nq)
+    Qv = s.debug_array("Qv").reshape(B, N, nq)
+    Rg = s.debug_array("Rg").reshape(B, N, nu)
+    q = s.debug_array("q").reshape(B, N, nx)
+    r = s.debug_array("r").reshape(B, N, nu)
+    return A, Bm, c, Qp, Qv, Rg, q, r
+
+
+@pytest.mark.parametrize("plant", ["iiwa14", "indy7"])
+def test_kkt_setup_matches_oracle(gpu, O, plant):
+    N, B = 32, 4
+    nq = W.NQ[plant]
+    s, wl, xu = _solver_and_inputs(plant, N, B)
+    cfg = _cfg(O, plant, wl)
+    s.debug_phase(0, xu, wl.xs, wl.ref)
+    A, Bm, c, Qp, Qv, Rg, q, r = _gpu_kkt(s, B, N, nq)
+    xs32 = wl.xs.astype(np.float32).astype(np.float64)
+    ref32 = wl.ref.astype(np.float32).astype(np.float64)
+    for b in range(B):
+        k = O.setup_kkt(cfg, xu[b], xs32[b], ref32[b])
+        assert rel_err(A[b, :N - 1], k["A"][:N - 1]) < 1e-5
+        assert rel_err(Bm[b, :N - 1], k["B"][:N - 1]) < 1e-5
+        assert rel_err(c[b], k["c"]) < 1e-5
+        assert rel_err(Qp[b], k["Q"][:, :nq, :nq]) < 1e-5
+        assert rel_err(Qv[b], np.diagonal(k["Q"][:, nq:, nq:], axis1=1, axis2=2)) < 1e-5
+        assert rel_err(q[b], k["q"]) < 1e-5
+        assert rel_err(r[b, :N - 1], k["r"][:N - 1]) < 1e-5
+        assert rel_err(Rg[b, :N - 1], np.diagonal(k["R"][:N - 1], axis1=1, axis2=2)) < 1e-5
+
+
+def _kkt_dict(A, Bm, c, Qp, Qv, Rg, q, r, b, nq):
+    N = A.shape[1]
+    nx = 2 * nq
+    Q = np.zeros((N, nx, nx))
+    Q[:, :nq, :nq] = Qp[b]
+    Q[:, nq:, nq:] = np.stack([np.diag(v) for v in Qv[b]])
+    R = np.stack([np.diag(v) for v in Rg[b]]).astype(np.float64)
+    return dict(A=A[b].astype(np.float64), B=Bm[b].astype(np.float64), Q=Q, R=R, q=q[b].astype(np.float64),
+                r=r[b].astype(np.float64), c=c[b].astype(np.float64))
+
+
+@pytest.mark.parametrize("plant", ["iiwa14", "indy7"])
+def test_schur_pcg_dz_merit_match_oracle_on_gpu_inputs(gpu, O, plant):
+    N, B = 32, 4
+    nq = W.NQ[plant]
+    nx, nu = 2 * nq, nq
+    s, wl, xu = _solver_and_inputs(plant, N, B)
+    cfg = _cfg(O, plant, wl)
+    s.debug_phase(0, xu, wl.xs, wl.ref)
+    kk = _gpu_kkt(s, B, N, nq)
+    s.debug_phase(1, xu, wl.xs, wl.ref)
+    Sd = s.debug_array("Sd").reshape(B, N, nx, nx)
+    So = s.debug_array("So").reshape(B, N, nx, nx)
+    Pd = s.debug_array("Pd").reshape(B, N, nx, nx)
+    Po = s.debug_array("Po").reshape(B, N, nx, nx)
+    g = s.debug_array("gamma").reshape(B, N, nx)
+    s.reset_dual()
+    s.debug_phase(2, xu, wl.xs, wl.ref)
+    lam = s.debug_array("lambda").reshape(B, N * nx)
+    pit = s.debug_array("pcg_iters")[:B]
+    s.debug_phase(3, xu, wl.xs, wl.ref)
+    dz = s.debug_array("dz").reshape(B, -1)[:, : (nx + nu) * N - nu]
+    kres = s.debug_array("kkt_residual")[:B]
+    s.debug_phase(4, xu, wl.xs, wl.ref)
+    mp = s.debug_array("merit_parts").reshape(B, 8, N)
+    xs32 = wl.xs.astype(np.float32).astype(np.float64)
+    ref32 = wl.ref.astype(np.float32).astype(np.float64)
+    for b in range(B):
+        kd = _kkt_dict(*kk, b, nq)
+        sc = O.schur(cfg, wl.params["rho"], kd)
+        assert rel_err(Sd[b], sc["Sd"]) < 1e-4
+        assert rel_err(So[b, 1:], sc["So"][1:]) < 1e-4
+        assert rel_err(Pd[b], sc["Pd"]) < 2e-3
+        assert rel_err(Po[b, 1:], sc["Po"][1:]) < 2e-3
+        assert rel_err(g[b], sc["gamma"]) < 2e-3
+        gs = dict(Sd=Sd[b].astype(np.float64), So=So[b].astype(np.float64), Pd=Pd[b].astype(np.float64),
+                  Po=Po[b].astype(np.float64), gamma=g[b].astype(np.float64).reshape(-1))
+        l64, it = O.pcg(nx, N, gs, np.zeros(N * nx), wl.params["pcg_tol"], wl.params["max_pcg_iters"])
+        assert abs(int(pit[b]) - it) <= 2
+        assert rel_err(lam[b], l64) < 1e-2
+        d64, qres, rres = O.dz(cfg, gs | {"Qi": sc["Qi"], "Ri": sc["Ri"]}, kd, lam[b].astype(np.float64))
+        assert rel_err(dz[b], d64) < 1e-2
+        kr = max(np.abs(qres).max(), np.abs(rres).max(), np.abs(kd["c"]).max())
+        assert abs(kres[b] - kr) <= 1e-3 * kr
+        ms = np.array([O.merit(cfg, xu[b], dz[b].astype(np.float64), 1.0 / (1 << j), xs32[b], ref32[b],
+                               wl.params["mu"]) for j in range(8)])
+        assert rel_err(mp[b].sum(1), ms) < 1e-5
