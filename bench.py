@@ -3,7 +3,9 @@
 
 A step is one batched solve (<= 10 SQP iterations per problem) of B=512 problems from the same
 synthetic initial guess, inputs resident in HBM. value = problems solved per second (whole job).
-Prints ONE JSON line. --gpus above 1 exits: this build runs on one GPU.
+The roofline entry is for the dominant kernel, k_pcg, timed with HIP events on the solve stream
+during a timed plain-path region. Prints ONE JSON line. --gpus above 1 exits: this build runs on
+one GPU.
 """
 import argparse
 import json
@@ -25,11 +27,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", default="fused", choices=["fused", "plain"])
+    ap.add_argument("--mode", default="graph", choices=["graph", "fused", "plain"])
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--knots", type=int, default=32)
-    ap.add_argument("--plain-steps", type=int, default=3, help="steps of the plain path timed for the speed-up (0: skip)")
+    ap.add_argument("--plain-steps", type=int, default=3,
+                    help="steps of the plain path timed for the speed-up and the k_pcg roofline (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample (0: skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run three device copies of MIB MiB (a known-size reference for HBM counters)")
     return ap.parse_args()
 
 
@@ -101,6 +106,37 @@ def algorithmic_flops(st, N):
     return s * (8.5e6 * it + 0.08e6 * pcg)
 
 
+def pcg_flops_per_launch(st, nx, N):
+    """Algorithmic FLOPs of one k_pcg launch (all B problems), averaged over the solve's launches.
+    Per PCG iteration and problem: two block-tridiagonal products (S p and P r) of 2 nx^2 (3N - 2)
+    FLOPs each, two dot products and three axpys (10 N nx); the set-up costs two products and a dot.
+    A problem whose PCG took 0 iterations still pays the set-up (its residual is tested)."""
+    mv = 2 * nx * nx * (3 * N - 2)
+    per_it = 2 * mv + 10 * N * nx
+    setup = 2 * mv + 2 * N * nx + 2 * N * nx
+    launches = int(st["n_iters"])
+    it = st["pcg_iters"][:launches].astype(float)
+    total = per_it * it.sum() + setup * it.size
+    return total / max(1, launches)
+
+
+def pcg_bytes_per_launch(nx, N, B):
+    """Compulsory HBM bytes of one k_pcg launch: the S and P block rows (read once into registers),
+    gamma and lambda in, lambda out."""
+    return 4 * B * (4 * N * nx * nx + 3 * N * nx)
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter summary, if present."""
+    f = ROOT / "profiles" / f"{name}_traffic.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
 def cpu_baseline(wl, seconds):
     """The FP64 oracle of the same iteration on the host cores, on a bounded sample."""
     import numpy as np
@@ -147,6 +183,14 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     wl = workload(args.knots, args.batch)
     solver = make_solver(wl, args.mode)
     elapsed, kern, st = time_gpu(solver, wl, args.steps, args.warmup)
@@ -179,27 +223,39 @@ def main():
         "mean_pcg_iters": float(st["pcg_iters"].sum() / max(1, st["sqp_iters"].sum())),
         "fwd_bwd_solves_per_s": None,
     }
-    kernel_name = "k_fused (whole solve)" if args.mode == "fused" else "k_pcg (per launch)"
-    if args.mode == "fused":
-        by = algorithmic_bytes(wl)
-        fl = algorithmic_flops(st, wl.N)
-        ach = by / (avg_us * 1e-6) / 1e9
-        out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": kernel_name,
-                           "avg_launch_us": avg_us, "launches_per_step": launches,
-                           "valu": {"achieved": fl / (avg_us * 1e-6) / 1e12, "peak": FP32_PEAK_TFLOPS,
-                                    "unit": "TFLOP/s", "frac": fl / (avg_us * 1e-6) / 1e12 / FP32_PEAK_TFLOPS,
-                                    "flops_per_launch": fl, "note": "nominal FLOPs of the reference iteration"}}
-    else:
-        out["roofline"] = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
-                           "traffic": None, "kernel": kernel_name, "avg_launch_us": avg_us,
-                           "launches_per_step": launches}
-    if args.mode == "fused" and args.plain_steps > 0:
+    nq = 7 if wl.plant == "iiwa14" else 6
+    nx = 2 * nq
+    pcg_src = None  # (avg launch us, stats) of the k_pcg launches, from a timed plain-path region
+    if args.mode == "plain":
+        pcg_src = (avg_us, st)
+    if args.plain_steps > 0 and args.mode != "plain":
         plain = make_solver(wl, "plain")
-        pe, _, _ = time_gpu(plain, wl, args.plain_steps, 1)
+        pe, pk, pst = time_gpu(plain, wl, args.plain_steps, 1)
         pv = B * args.plain_steps / pe
         out["plain_value"] = pv
         out["speedup_vs_plain"] = value / pv
+        pcg_src = (sum(k[0] for k in pk) / len(pk), pst)
+    if pcg_src is not None:
+        us, pst = pcg_src
+        fl = pcg_flops_per_launch(pst, nx, wl.N)
+        by = pcg_bytes_per_launch(nx, wl.N, B)
+        ach = fl / (us * 1e-6) / 1e12
+        out["roofline"] = {"bound": "mfma", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": ach / FP32_PEAK_TFLOPS, "traffic": measured_traffic("k_pcg"),
+                           "kernel": "k_pcg (PCG on the Schur system, all B problems per launch)",
+                           "avg_launch_us": us, "flops_per_launch": fl, "algorithmic_bytes_per_launch": by,
+                           "hbm_achieved_GBs": by / (us * 1e-6) / 1e9,
+                           "hbm_frac": by / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                           "note": "fp32 vector-ALU work (no MFMA); peak = MI355X fp32 157.3 TFLOP/s"}
+    if args.mode in ("fused", "graph"):
+        by = algorithmic_bytes(wl)
+        fl = algorithmic_flops(st, wl.N)
+        out["whole_solve"] = {"kernel": "k_fused (whole solve)" if args.mode == "fused" else "hipGraph (whole solve)",
+                              "avg_launch_us": avg_us, "launches_per_step": launches,
+                              "hbm_achieved_GBs": by / (avg_us * 1e-6) / 1e9,
+                              "valu_achieved_TFLOPs": fl / (avg_us * 1e-6) / 1e12,
+                              "valu_frac": fl / (avg_us * 1e-6) / 1e12 / FP32_PEAK_TFLOPS,
+                              "note": "nominal FLOPs of the reference iteration, compulsory bytes per solve"}
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -9,7 +9,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libgato_mi355x.so"
 
 PLANT_IDS = {"iiwa14": 0, "indy7": 1}
-MODE_PLAIN, MODE_FUSED = 0, 1
+MODE_PLAIN, MODE_FUSED, MODE_GRAPH = 0, 1, 2
 EXIT_PCG0, EXIT_KKT = 0, 1
 
 

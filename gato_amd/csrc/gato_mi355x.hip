@@ -36,6 +36,8 @@ struct Args {
     const float *drho_init, *mu, *tol;
     float *A, *Bm, *c, *Qp, *Qv, *Rg, *q, *r, *Qip, *Qiv, *Ri, *Sd, *So, *Pd, *Po, *gam, *dz, *mpart, *phi;
     int *conv, *active, *solved;
+    const int* stop;  // graph mode: set once the batch has converged; every later phase launch returns at once
+    int* stop_w;
     int* st_pcg;
     float *st_kres, *st_lsm, *st_step;
     int* st_sqp;
@@ -78,6 +80,9 @@ struct View {  // per-problem pointers
         mpart = a.mpart + (size_t)b * NUM_ALPHAS * N;
     }
 };
+
+// graph mode: the iteration's kernels are all enqueued up front; once the batch-level exit fires they return
+__device__ __forceinline__ bool stopped(const Args& a) { return a.stop && *a.stop; }
 
 // ------------------------------------------------------------------ phases on one problem
 template <int PL>
@@ -434,7 +439,7 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_merit(Args a, int na
                                                int it) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
-    if (which == 2 && a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
+    if (which == 2 && (stopped(a) || (a.exit_rule == GATO_EXIT_KKT && !a.active[b]))) return;
     phase_merit<PL>(a, b, a.mu[b], nal, sm);
     if (threadIdx.x == 0 && which < 2) {
         a.phi[b] = sm[0];
@@ -447,7 +452,7 @@ template <int PL>
 __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_lin(Args a) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
-    if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
+    if (stopped(a) || (a.exit_rule == GATO_EXIT_KKT && !a.active[b])) return;
     phase_lin<PL>(a, b, sm);
 }
 
@@ -455,7 +460,7 @@ template <int PL>
 __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_schur(Args a) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
-    if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
+    if (stopped(a) || (a.exit_rule == GATO_EXIT_KKT && !a.active[b])) return;
     phase_schur<PL>(a, b, a.rho[b], sm);
 }
 
@@ -463,6 +468,7 @@ template <int PL, int NPASS>
 __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg(Args a, int it) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
+    if (stopped(a)) return;
     const bool skip = (a.exit_rule == GATO_EXIT_KKT) ? !a.active[b] : a.conv[b];  // quirk Q9 (pcg.cuh:30-33)
     int n = 0;
     if (!skip) n = pcg_dispatch<PL, NPASS>(a, b, a.tol[b], sm);
@@ -473,6 +479,7 @@ template <int PL>
 __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_dz(Args a, int it) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
+    if (stopped(a)) return;
     if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) {
         if (threadIdx.x == 0) a.st_kres[it * a.B + b] = 0.f;
         return;
@@ -484,7 +491,7 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_dz(Args a, int it) {
 // host-side convergence bookkeeping of bsqp.cuh:142-165, per problem, plus the converged count
 __global__ void k_check(Args a, int it) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= a.B) return;
+    if (b >= a.B || stopped(a)) return;
     int conv = a.conv[b];
     if (a.exit_rule == GATO_EXIT_PCG0) {
         if (a.st_pcg[it * a.B + b] == 0) conv = 1;
@@ -501,8 +508,14 @@ __global__ void k_check(Args a, int it) {
     if (conv) atomicAdd(&a.solved[it], 1);
 }
 
+// graph mode: the host loop's exit test (bsqp.cuh:165-167) on the device, after the iteration's check
+__global__ void k_decide(Args a, int it) {
+    if (threadIdx.x == 0 && !*a.stop && (double)a.solved[it] >= (double)a.B * (double)a.solve_ratio) *a.stop_w = 1;
+}
+
 __global__ void k_begin(Args a) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0 && a.stop_w) *a.stop_w = 0;
     if (b >= a.B) return;
     a.conv[b] = 0;
     a.active[b] = 1;
@@ -647,6 +660,12 @@ struct gato_solver {
     std::vector<float> h_rho_init, h_drho_init;
     bool joint_cost = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int* d_stop = nullptr;
+    // graph mode: the captured solve, valid for the launch arguments it was captured with
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    gato::Args gkey{};
     double last_kernel_us = 0.0;
     int last_launches = 0;
     // host copies of device stats
@@ -710,6 +729,7 @@ size_t layout(gato_solver* s, bool assign) {
     t = F(it * B); if (assign) a.st_lsm = t;
     t = F(it * B); if (assign) a.st_step = t;
     ip = I(B); if (assign) a.st_sqp = ip;
+    ip = I(1); if (assign) s->d_stop = ip;
     t = F(B); if (assign) a.st_init = t;
     t = F(B); if (assign) a.st_final = t;
     return off + 256;
@@ -757,10 +777,79 @@ void fill_args(gato_solver* s, float* d_xu, const float* d_xs, const float* d_re
     a.drho_init = s->d_drho_init;
     a.mu = s->d_mu;
     a.tol = s->d_tol;
+    a.stop = nullptr;
+    a.stop_w = nullptr;
+}
+
+// Every SQP iteration of the plain path enqueued without a host round trip: k_decide raises the device
+// stop flag when the batch-level exit fires and the remaining launches return at once.
+template <int PL>
+void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
+    const int B = s->B, N = s->N;
+    const dim3 blk(grid_threads(N));
+    const size_t lds = lds_bytes(s);
+    const gato::Args& a = s->a;
+    const int sb = (B + 255) / 256;
+    const int np = npass(N);
+    const size_t nz = (size_t)B * (s->traj + s->nu);
+    hipLaunchKernelGGL(gato::k_begin, dim3(sb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
+    hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, 1, 0, 0);
+    for (int it = 0; it < (int)s->prm.max_sqp_iters; it++) {
+        hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
+        hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, a);
+        if (np == 1) hipLaunchKernelGGL((gato::k_pcg<PL, 1>), dim3(B), blk, lds, st, a, it);
+        else if (np == 2) hipLaunchKernelGGL((gato::k_pcg<PL, 2>), dim3(B), blk, lds, st, a, it);
+        else hipLaunchKernelGGL((gato::k_pcg<PL, 4>), dim3(B), blk, lds, st, a, it);
+        hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, a, it);
+        hipLaunchKernelGGL(gato::k_check, dim3(sb), dim3(256), 0, st, a, it);
+        hipLaunchKernelGGL(gato::k_decide, dim3(1), dim3(64), 0, st, a, it);
+        hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, gato::NUM_ALPHAS, 2, it);
+    }
+    hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
+    hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, 1, 1, 0);
+    hipLaunchKernelGGL(gato::k_end, dim3(sb), dim3(256), 0, st, a);
+}
+
+void drop_graph(gato_solver* s) {
+    if (s->gexec) hipGraphExecDestroy(s->gexec);
+    if (s->graph) hipGraphDestroy(s->graph);
+    s->gexec = nullptr;
+    s->graph = nullptr;
+}
+
+template <int PL>
+int launch_graph(gato_solver* s, hipStream_t st) {
+    gato::Args& a = s->a;
+    a.stop = s->d_stop;
+    a.stop_w = s->d_stop;
+    if (!s->gexec || std::memcmp(&s->gkey, &a, sizeof(gato::Args)) != 0) {
+        drop_graph(s);
+        if (!s->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(s->cap_stream, hipStreamCaptureModeThreadLocal));
+        enqueue_graph_solve<PL>(s, s->cap_stream);
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(s->cap_stream, &s->graph);
+        if (le != hipSuccess) { drop_graph(s); return fail((int)le, std::string("graph capture launch: ") + hipGetErrorString(le)); }
+        if (ce != hipSuccess) { drop_graph(s); return fail((int)ce, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce)); }
+        const hipError_t ie = hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) { drop_graph(s); return fail((int)ie, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie)); }
+        std::memcpy(&s->gkey, &a, sizeof(gato::Args));
+    }
+    HIP_TRY(hipEventRecord(s->ev0, st));
+    HIP_TRY(hipGraphLaunch(s->gexec, st));
+    HIP_TRY(hipEventRecord(s->ev1, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->last_kernel_us = ms * 1e3;
+    s->last_launches = 1;
+    return 0;
 }
 
 template <int PL>
 int launch_solve(gato_solver* s, hipStream_t st) {
+    if (s->prm.mode == GATO_MODE_GRAPH) return launch_graph<PL>(s, st);
     const int B = s->B, N = s->N;
     const dim3 blk(grid_threads(N));
     const size_t lds = lds_bytes(s);
@@ -861,7 +950,7 @@ int gato_create(gato_solver** out, int32_t plant, int32_t knots, int32_t batch, 
     if (knots < 2 || knots > 128) return fail(GATO_E_ARG, "knots must be in [2, 128]");
     if (batch < 1) return fail(GATO_E_ARG, "batch must be >= 1");
     if (params->max_sqp_iters < 1 || params->max_sqp_iters > 1000) return fail(GATO_E_ARG, "max_sqp_iters must be in [1, 1000]");
-    if (params->mode != GATO_MODE_PLAIN && params->mode != GATO_MODE_FUSED) return fail(GATO_E_ARG, "unknown mode");
+    if (params->mode < GATO_MODE_PLAIN || params->mode > GATO_MODE_GRAPH) return fail(GATO_E_ARG, "unknown mode");
     if (params->exit_rule != GATO_EXIT_PCG0 && params->exit_rule != GATO_EXIT_KKT) return fail(GATO_E_ARG, "unknown exit rule");
     gato_solver* s = new gato_solver();
     s->plant = plant;
@@ -911,6 +1000,8 @@ int gato_destroy(gato_solver* s) {
     if (!s) return 0;
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
+    drop_graph(s);
+    if (s->cap_stream) hipStreamDestroy(s->cap_stream);
     if (s->pool) hipFree(s->pool);
     delete s;
     return 0;
@@ -1033,7 +1124,7 @@ int gato_set_rho_adaptation(gato_solver* s, int32_t enabled) {
 
 int gato_set_mode(gato_solver* s, int32_t mode, int32_t exit_rule) {
     if (!s) return fail(GATO_E_ARG, "null solver");
-    if (mode != GATO_MODE_PLAIN && mode != GATO_MODE_FUSED) return fail(GATO_E_ARG, "unknown mode");
+    if (mode < GATO_MODE_PLAIN || mode > GATO_MODE_GRAPH) return fail(GATO_E_ARG, "unknown mode");
     if (exit_rule != GATO_EXIT_PCG0 && exit_rule != GATO_EXIT_KKT) return fail(GATO_E_ARG, "unknown exit rule");
     s->prm.mode = mode;
     s->prm.exit_rule = exit_rule;

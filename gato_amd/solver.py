@@ -6,8 +6,9 @@ same ``solve`` result dictionary, setters and ``sim_forward``. Differences, all 
   * the knot count N and batch size B are run-time values;
   * inputs may be numpy arrays or torch tensors (device tensors are used in place);
   * inputs are checked for dtype and size (the reference reads raw buffers unchecked);
-  * keyword-only extensions: execution ``mode`` ("fused" persistent kernel or "plain" one kernel
-    per phase), ``exit_rule`` ("pcg0" reference rule or "kkt"), joint-space costs, state get/set.
+  * keyword-only extensions: execution ``mode`` ("fused" persistent kernel, "plain" one kernel
+    per phase with a host loop, or "graph" - the plain kernels for every iteration gated on a device
+    flag and replayed as one hipGraph), ``exit_rule`` ("pcg0" reference rule or "kkt"), joint-space costs, state get/set.
 """
 import ctypes as C
 
@@ -23,7 +24,7 @@ DEFAULTS = dict(dt=0.01, max_sqp_iters=5, kkt_tol=1e-4, max_pcg_iters=100, pcg_t
                 ctrl_lim_cost=0.0, rho=1e-3)
 CTOR_ORDER = ("dt", "max_sqp_iters", "kkt_tol", "max_pcg_iters", "pcg_tol", "solve_ratio", "mu", "q_cost",
               "qd_cost", "u_cost", "N_cost", "q_lim_cost", "vel_lim_cost", "ctrl_lim_cost", "rho")
-MODES = {"plain": _lib.MODE_PLAIN, "fused": _lib.MODE_FUSED}
+MODES = {"plain": _lib.MODE_PLAIN, "fused": _lib.MODE_FUSED, "graph": _lib.MODE_GRAPH}
 EXITS = {"pcg0": _lib.EXIT_PCG0, "kkt": _lib.EXIT_KKT}
 
 

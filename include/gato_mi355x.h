@@ -31,6 +31,8 @@ extern "C" {
 /* execution paths */
 #define GATO_MODE_PLAIN 0 /* one kernel per phase, host loop with a device->host sync per iteration */
 #define GATO_MODE_FUSED 1 /* one persistent kernel: every SQP iteration of every problem on device */
+#define GATO_MODE_GRAPH 2 /* the plain kernels for every iteration, gated by a device-side stop flag instead of
+                             the host read, captured once into a hipGraph and replayed */
 
 /* convergence rules */
 #define GATO_EXIT_PCG0 0 /* reference: converged <=> PCG took 0 iterations (bsqp.cuh:149-163) */
