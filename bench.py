@@ -33,6 +33,7 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="steps of the plain path timed for the speed-up and the k_pcg roofline (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample (0: skip)")
+    ap.add_argument("--fwd-bwd-steps", type=int, default=3, help="steps of forward solve + backward pass timed (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run three device copies of MIB MiB (a known-size reference for HBM counters)")
     return ap.parse_args()
@@ -85,6 +86,35 @@ def time_gpu(solver, wl, steps, warmup):
     solver.reset_rho()
     st = solver.solve_device(xu, xs, ref, stats=True)
     return (t1 - t0), kern, st
+
+
+def time_fwd_bwd(solver, wl, steps, warmup):
+    """Forward solve plus the adjoint backward pass for a fixed loss gradient dL/dXU (seeded)."""
+    import numpy as np
+    import torch
+
+    dev = solver.device
+    xu0 = torch.tensor(wl.xu, dtype=torch.float32, device=dev)
+    xs = torch.tensor(wl.xs, dtype=torch.float32, device=dev)
+    ref = torch.tensor(wl.ref, dtype=torch.float32, device=dev)
+    gxu = torch.tensor(np.random.default_rng(5).normal(0, 1, wl.xu.shape), dtype=torch.float32, device=dev)
+    xu = xu0.clone()
+
+    def one():
+        xu.copy_(xu0)
+        solver.reset_dual()
+        solver.reset_rho()
+        solver.solve_device(xu, xs, ref)
+        return solver.backward_device(xu, xs, ref, gxu)
+
+    for _ in range(warmup):
+        one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g = one()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, g
 
 
 def algorithmic_bytes(wl):
@@ -223,6 +253,11 @@ def main():
         "mean_pcg_iters": float(st["pcg_iters"].sum() / max(1, st["sqp_iters"].sum())),
         "fwd_bwd_solves_per_s": None,
     }
+    if args.fwd_bwd_steps > 0:
+        fe, g = time_fwd_bwd(solver, wl, args.fwd_bwd_steps, 1)
+        out["fwd_bwd_solves_per_s"] = B * args.fwd_bwd_steps / fe
+        out["fwd_bwd_ms_per_step"] = 1e3 * fe / args.fwd_bwd_steps
+        out["adjoint_pcg_iters_mean"] = float(g["adj_pcg_iters"].mean())
     nq = 7 if wl.plant == "iiwa14" else 6
     nx = 2 * nq
     pcg_src = None  # (avg launch us, stats) of the k_pcg launches, from a timed plain-path region

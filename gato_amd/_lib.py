@@ -53,6 +53,8 @@ SIGNATURES = {
     "gato_sim_forward": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP]),
     "gato_plant_dims": (C.c_int, [C.c_int32, _ip, _ip]),
     "gato_last_kernel_time": (C.c_int, [_VP, C.POINTER(C.c_double), _ip]),
+    "gato_backward": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _VP, _VP, _VP, _VP, _VP, _ip]),
+    "gato_set_backward_params": (C.c_int, [_VP, C.c_int32, C.c_float, C.c_int32]),
     "gato_debug_phase": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP, C.c_float, _VP]),
     "gato_debug_copy": (C.c_int, [_VP, C.c_char_p, _VP, C.c_int64, C.POINTER(C.c_int64)]),
     "gato_last_error": (C.c_char_p, []),

@@ -29,6 +29,7 @@ namespace gato {
 struct Args {
     int N, B, traj, max_sqp, max_pcg, exit_rule, adapt_rho;
     float dt, kkt_tol, solve_ratio;
+    float pcg_abs_tol;  // absolute part of the PCG exit test (1e-6 in the solve, pcg.cuh:27; tiny for the adjoint)
     CostW cw;
     float* xu;
     const float *xs, *ref, *fext, *Qd, *Rd, *xref;
@@ -203,7 +204,7 @@ __device__ __forceinline__ int phase_pcg(const Args& a, int b, float eps, float*
         loc += r[s] * z[s];
     }
     float rho = block_sum(loc, red);
-    if (fabsf(rho) < PCG_ABS_TOL) return 0;
+    if (fabsf(rho) < a.pcg_abs_tol) return 0;
     const float rho0 = fabsf(rho);
     int it = 0;
     for (int i = 0; i < a.max_pcg; i++) {
@@ -233,7 +234,7 @@ __device__ __forceinline__ int phase_pcg(const Args& a, int b, float eps, float*
             loc += r[s] * z[s];
         }
         const float rn = block_sum(loc, red);
-        if (fabsf(rn) < PCG_ABS_TOL + eps * rho0) break;
+        if (fabsf(rn) < a.pcg_abs_tol + eps * rho0) break;
         const float beta = rn / rho;
         rho = rn;
 #pragma unroll
@@ -322,7 +323,7 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
     int slot = 0;
     float rho = block_sum1(r * z, red + 16 * slot);
     slot ^= 1;
-    if (fabsf(rho) < PCG_ABS_TOL) return 0;
+    if (fabsf(rho) < a.pcg_abs_tol) return 0;
     const float rho0 = fabsf(rho);
     int it = 0;
     for (int i = 0; i < a.max_pcg; i++) {
@@ -339,7 +340,7 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
         z = val ? mv(pd, pl, pu, sr) : 0.f;
         const float rn = block_sum1(r * z, red + 16 * slot);
         slot ^= 1;
-        if (fabsf(rn) < PCG_ABS_TOL + eps * rho0) break;
+        if (fabsf(rn) < a.pcg_abs_tol + eps * rho0) break;
         const float beta = rn / rho;
         rho = rn;
         p = z + beta * p;
@@ -606,6 +607,216 @@ __global__ void __launch_bounds__(512) k_fused(Args a) {
     (void)NX;
 }
 
+
+// ------------------------------------------------------------------ adjoint of the SQP step (backward)
+// No reference counterpart. The backward differentiates the step map z+ = z + dz(z; theta) at the solution z,
+// theta = (x_s, reference xyz, diag Q, diag R), with dz from the step's KKT system K [dz; l] = [-g; c]. The
+// adjoint system K [v; v_l] = [dL/dz; 0] has the same Schur complement S and preconditioner P: it is the step
+// system with g' = -dL/dz and c' = 0, so it reuses the Schur, PCG and dz phases on separate buffers. Then
+//   dL/dx_s = -v_l,0,   dL/dtheta = -v^T dg/dtheta - v^T (dG/dtheta) dz   (G depends on diag Q, diag R, ref).
+
+// Right-hand side of a Schur solve from a KKT right-hand side [-g; c] in block form (g = (q', r') per knot):
+//   gamma_0 = c_0 - Q~_0^-1 q'_0,  gamma_{k+1} = c_{k+1} - Q~_{k+1}^-1 q'_{k+1} + phi_k q'_k + B_k R_k^-1 r'_k,
+// the same expression as the Schur phase's gamma (schur_linsys.cuh:60-98), for arbitrary (q', r', c').
+template <int PL>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_gamma_rhs(Args a, const float* qv, const float* rv,
+                                                                      const float* cv, float* gam) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
+    const int b = blockIdx.x;
+    View<PL> v(a, b);
+    const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
+    const float* qb = qv + (size_t)b * N * NX;
+    const float* rb = rv + (size_t)b * N * NU;
+    const float* cb = cv + (size_t)b * N * NX;
+    for (int k = g; k < N; k += G) {
+        if (lane >= NX) continue;
+        const int r = lane;
+        float qi[NX];
+        qinv_row<PL>(v.Qip + k * NQ * NQ, v.Qiv + k * NQ, r, qi);
+        float gs = cb[k * NX + r];
+#pragma unroll
+        for (int m = 0; m < NX; m++) gs -= qi[m] * qb[k * NX + m];
+        if (k > 0) {
+            const float* phi = v.So + (size_t)k * NX * NX + r * NX;
+#pragma unroll
+            for (int m = 0; m < NX; m++) gs += phi[m] * qb[(k - 1) * NX + m];
+            const float* Bk = v.Bm + (size_t)(k - 1) * NX * NU + r * NU;
+            const float* Rk = v.Ri + (k - 1) * NU;
+#pragma unroll
+            for (int l = 0; l < NU; l++) gs += Bk[l] * Rk[l] * rb[(k - 1) * NU + l];
+        }
+        gam[(size_t)b * N * NX + k * NX + r] = gs;
+    }
+}
+
+// Residual of the step's KKT system K [z; l] = rhs at a current solution (z, l), from the blocks themselves
+// (no inverses), written as the right-hand side of the correction solve: q' = -r_x, r' = -r_u, c' = r_c.
+//   r_x = rhs_x - Q~_k z_x + l_k - A_k^T l_{k+1},  r_u = rhs_u - R_k z_u - B_k^T l_{k+1},
+//   r_c0 = rhs_c0 + z_x0,  r_c{k+1} = rhs_c{k+1} - A_k z_xk - B_k z_uk + z_x{k+1}.
+// rhs_z is rz (trajectory layout) when given, else -(gq, gr); rhs_c is cc when given, else 0.
+template <int PL>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_kkt_resid(Args a, const float* rz,
+                                                                      const float* gq, const float* gr,
+                                                                      const float* cc, const float* zall,
+                                                                      const float* lall, float* qo, float* ro,
+                                                                      float* co) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU, ST = NX + NU;
+    const int b = blockIdx.x;
+    View<PL> v(a, b);
+    const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
+    const float rho = a.rho[b];
+    const float* z = zall + (size_t)b * (a.traj + NU);
+    const float* l = lall + (size_t)b * N * NX;
+    for (int k = g; k < N; k += G) {
+        const bool last = (k == N - 1);
+        const float* zk = z + (size_t)k * ST;
+        const float* l1 = l + (k + 1) * NX;
+        if (lane < NX) {
+            const int i = lane;
+            float hx;
+            if (i < NQ) {
+                hx = rho * zk[i];
+#pragma unroll
+                for (int m = 0; m < NQ; m++) hx += v.Qp[k * NQ * NQ + i * NQ + m] * zk[m];
+            } else {
+                hx = v.Qv[k * NQ + i - NQ] * zk[i];
+            }
+            hx -= l[k * NX + i];
+            if (!last) {
+                const float* A = v.A + (size_t)k * NX * NX;
+#pragma unroll
+                for (int m = 0; m < NX; m++) hx += A[m * NX + i] * l1[m];
+            }
+            const float rhs = rz ? rz[(size_t)b * a.traj + (size_t)k * ST + i] : -gq[(size_t)b * N * NX + k * NX + i];
+            qo[(size_t)b * N * NX + k * NX + i] = -(rhs - hx);
+            // constraint row k: c_0 = -z_x0, c_k = A_{k-1} z_x{k-1} + B_{k-1} z_u{k-1} - z_xk
+            float cz;
+            if (k == 0) {
+                cz = -zk[i];
+            } else {
+                const float* A0 = v.A + (size_t)(k - 1) * NX * NX + i * NX;
+                const float* B0 = v.Bm + (size_t)(k - 1) * NX * NU + i * NU;
+                const float* zp = z + (size_t)(k - 1) * ST;
+                cz = -zk[i];
+#pragma unroll
+                for (int m = 0; m < NX; m++) cz += A0[m] * zp[m];
+#pragma unroll
+                for (int m = 0; m < NU; m++) cz += B0[m] * zp[NX + m];
+            }
+            const float rc = cc ? cc[(size_t)b * N * NX + k * NX + i] : 0.f;
+            co[(size_t)b * N * NX + k * NX + i] = rc - cz;
+        }
+        if (lane < NU) {
+            const int i = lane;
+            float ru = 0.f;
+            if (!last) {
+                float hu = v.Rg[k * NU + i] * zk[NX + i];
+                const float* Bm = v.Bm + (size_t)k * NX * NU;
+#pragma unroll
+                for (int m = 0; m < NX; m++) hu += Bm[m * NU + i] * l1[m];
+                const float rhs = rz ? rz[(size_t)b * a.traj + (size_t)k * ST + NX + i]
+                                     : -gr[(size_t)b * N * NU + k * NU + i];
+                ru = -(rhs - hu);
+            }
+            ro[(size_t)b * N * NU + k * NU + i] = ru;
+        }
+    }
+}
+
+// z += dz, l += dl; dl is cleared (the next correction's PCG starts from zero)
+__global__ void k_accumulate(float* z, float* dz, float* l, float* dl, int nz, int nl) {
+    const int b = blockIdx.x;
+    for (int i = threadIdx.x; i < nz; i += blockDim.x) z[(size_t)b * nz + i] += dz[(size_t)b * nz + i];
+    for (int i = threadIdx.x; i < nl; i += blockDim.x) {
+        l[(size_t)b * nl + i] += dl[(size_t)b * nl + i];
+        dl[(size_t)b * nl + i] = 0.f;
+    }
+}
+
+// Parameter gradients from the step (a.dz at the solution), the adjoint solution vz and its multipliers lam_a.
+// Reference xyz of knot k: w (v_q . J_j) + [rank-1 Hessian] w ((v_q . J_j)(g . dq) + (v_q . g)(J_j . dq)),
+// with J, g = J^T (p - ref_k) at the knot's cost state (x_{N-2} for the reference's terminal block).
+template <int PL>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_grad(Args a, const float* vzall, const float* lam_a,
+                                                                 float* gxs, float* gref, float* gQ, float* gR) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU, ST = NX + NU;
+    extern __shared__ __align__(16) float sm[];
+    const int b = blockIdx.x;
+    View<PL> v(a, b);
+    const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
+    float* sq = sm;
+    float* sr = sm + N * KG;
+    const float* vz = vzall + (size_t)b * (a.traj + NU);
+    const float h0 = a.cw.hess_mode ? 0.f : 1.f;
+    for (int k = g; k < N; k += G) {
+        const bool term = (k == N - 1);
+        const int src = term ? (a.cw.terminal_mode ? N - 1 : N - 2) : k;
+        const float* xsrc = v.xu + (size_t)src * ST;
+        const float* vx = vz + (size_t)k * ST;
+        const float* dx = v.dz + (size_t)k * ST;
+        float cq = 0.f, cr = 0.f;
+        if (lane < NX) {
+            const float xr = v.xref ? v.xref[k * NX + lane] : 0.f;
+            cq = -vx[lane] * (xsrc[lane] - xr) - vx[lane] * dx[lane];
+        }
+        if (!term && lane < NU) cr = -vx[NX + lane] * (v.xu[(size_t)k * ST + NX + lane] + dx[NX + lane]);
+        sq[k * KG + lane] = cq;
+        sr[k * KG + lane] = cr;
+        float q[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; i++) q[i] = xsrc[i];
+        Kin<PL> K;
+        K.set(q);
+        float p[3], Jc[3];
+        ee_position<PL>(K, lane < NQ ? lane : -1, p, Jc);
+        const float* rk = v.ref + 6 * k;
+        const float gl = Jc[0] * (p[0] - rk[0]) + Jc[1] * (p[1] - rk[1]) + Jc[2] * (p[2] - rk[2]);
+        const float vq = lane < NQ ? vx[lane] : 0.f, dq = lane < NQ ? dx[lane] : 0.f;
+        const float s0 = row16_sum(vq * Jc[0]), s1 = row16_sum(vq * Jc[1]), s2 = row16_sum(vq * Jc[2]);
+        const float t0 = row16_sum(Jc[0] * dq), t1 = row16_sum(Jc[1] * dq), t2 = row16_sum(Jc[2] * dq);
+        const float gdq = row16_sum(gl * dq), vqg = row16_sum(vq * gl);
+        const float w = term ? (a.cw.terminal_mode ? a.cw.N_cost : a.cw.q_cost) : a.cw.q_cost;
+        if (lane < 6) {
+            const float s = lane == 0 ? s0 : lane == 1 ? s1 : s2;
+            const float t = lane == 0 ? t0 : lane == 1 ? t1 : t2;
+            gref[(size_t)b * 6 * N + 6 * k + lane] = lane < 3 ? w * (s + h0 * (s * gdq + vqg * t)) : 0.f;
+        }
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < NX) {
+        float s = 0.f;
+        for (int k = 0; k < N; k++) s += sq[k * KG + t];
+        gQ[(size_t)b * NX + t] = s;
+        gxs[(size_t)b * NX + t] = -lam_a[(size_t)b * N * NX + t];
+    } else if (t >= KG && t < KG + NU) {
+        float s = 0.f;
+        for (int k = 0; k < N; k++) s += sr[k * KG + t - KG];
+        gR[(size_t)b * NU + t - KG] = s;
+    }
+}
+
+// backward set-up: flags cleared, the step's multipliers warm-started from the solve's, adjoint buffers zeroed
+__global__ void k_bwd_begin(Args a, const float* lam_keep, float* lam_b, float* lam_a, float* dz_b, float* dz_a,
+                            int nlam, int ndz) {
+    const int b = blockIdx.x;
+    if (threadIdx.x == 0) {
+        a.conv[b] = 0;
+        a.active[b] = 1;
+    }
+    for (int i = threadIdx.x; i < nlam; i += blockDim.x) {
+        lam_b[(size_t)b * nlam + i] = lam_keep[(size_t)b * nlam + i];
+        lam_a[(size_t)b * nlam + i] = 0.f;
+    }
+    for (int i = threadIdx.x; i < ndz; i += blockDim.x) {
+        dz_b[(size_t)b * ndz + i] = 0.f;
+        dz_a[(size_t)b * ndz + i] = 0.f;
+    }
+}
+
 // ------------------------------------------------------------------ sim_forward (sim.cuh:14-48)
 template <int PL>
 __global__ void k_sim(float* xn, const float* x, const float* u, const float* fext, float dt, int B) {
@@ -661,6 +872,15 @@ struct gato_solver {
     bool joint_cost = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int* d_stop = nullptr;
+    // backward workspace: the step at the solution (lam_b, dz_b) and the adjoint system (q_a, r_a, gam_a,
+    // lam_a, dz_a), the adjoint PCG settings and its iteration counts
+    float *lam_b = nullptr, *dz_b = nullptr, *q_a = nullptr, *r_a = nullptr, *c_a = nullptr, *gam_a = nullptr,
+          *lam_a = nullptr, *dz_a = nullptr, *dz_v = nullptr, *lam_v = nullptr, *d_bwd_tol = nullptr,
+          *bst_kres = nullptr;
+    int* bst_pcg = nullptr;
+    int bwd_max_pcg = 1000;
+    int bwd_refine = 1;
+    float bwd_tol = 1e-12f;
     // graph mode: the captured solve, valid for the launch arguments it was captured with
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -730,6 +950,19 @@ size_t layout(gato_solver* s, bool assign) {
     t = F(it * B); if (assign) a.st_step = t;
     ip = I(B); if (assign) a.st_sqp = ip;
     ip = I(1); if (assign) s->d_stop = ip;
+    t = F(B * N * nx); if (assign) s->lam_b = t;
+    t = F(B * (s->traj + nu)); if (assign) s->dz_b = t;
+    t = F(B * N * nx); if (assign) s->q_a = t;
+    t = F(B * N * nu); if (assign) s->r_a = t;
+    t = F(B * N * nx); if (assign) s->c_a = t;
+    t = F(B * (s->traj + nu)); if (assign) s->dz_v = t;
+    t = F(B * N * nx); if (assign) s->lam_v = t;
+    t = F(B * N * nx); if (assign) s->gam_a = t;
+    t = F(B * N * nx); if (assign) s->lam_a = t;
+    t = F(B * (s->traj + nu)); if (assign) s->dz_a = t;
+    t = F(B); if (assign) s->d_bwd_tol = t;
+    t = F(B); if (assign) s->bst_kres = t;
+    ip = I(B); if (assign) s->bst_pcg = ip;
     t = F(B); if (assign) a.st_init = t;
     t = F(B); if (assign) a.st_final = t;
     return off + 256;
@@ -763,6 +996,7 @@ void fill_args(gato_solver* s, float* d_xu, const float* d_xs, const float* d_re
     a.dt = dt;
     a.kkt_tol = s->prm.kkt_tol;
     a.solve_ratio = s->prm.solve_ratio;
+    a.pcg_abs_tol = gato::PCG_ABS_TOL;
     a.cw = gato::CostW{s->prm.q_cost, s->prm.N_cost, s->prm.q_lim_cost, s->prm.vel_lim_cost, s->prm.ctrl_lim_cost,
                        s->prm.terminal_mode, s->prm.hess_mode};
     a.xu = d_xu;
@@ -914,6 +1148,65 @@ int launch_solve(gato_solver* s, hipStream_t st) {
     HIP_TRY(hipStreamSynchronize(st));
     s->last_kernel_us = pcg_launches ? pcg_ms * 1e3 / pcg_launches : 0.0;
     s->last_launches = pcg_launches;
+    return 0;
+}
+
+
+template <int PL>
+void launch_pcg(int np, int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a) {
+    if (np == 1) hipLaunchKernelGGL((gato::k_pcg<PL, 1>), dim3(B), blk, lds, st, a, 0);
+    else if (np == 2) hipLaunchKernelGGL((gato::k_pcg<PL, 2>), dim3(B), blk, lds, st, a, 0);
+    else hipLaunchKernelGGL((gato::k_pcg<PL, 4>), dim3(B), blk, lds, st, a, 0);
+}
+
+// The backward pass: the step at the solution and the adjoint system, each solved with the step's Schur
+// complement and preconditioner and improved by `bwd_refine` rounds of iterative refinement (the residual
+// is evaluated with the KKT blocks themselves, so fp32 error in S and Q~^-1 does not limit the result).
+template <int PL>
+int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, float* gQ, float* gR, hipStream_t st) {
+    const int B = s->B, N = s->N, np = npass(N);
+    const dim3 blk(grid_threads(N));
+    const size_t lds = std::max(lds_bytes(s), (size_t)2 * N * gato::KG * sizeof(float));
+    const int nlam = N * s->nx, ndz = s->traj + s->nu;
+    gato::Args ab = s->a;  // the step at the solution, on its own multipliers and dz
+    ab.exit_rule = GATO_EXIT_PCG0;
+    ab.stop = nullptr;
+    ab.stop_w = nullptr;
+    ab.lam = s->lam_b;
+    ab.dz = s->dz_b;
+    ab.st_pcg = s->bst_pcg;
+    ab.st_kres = s->bst_kres;
+    hipLaunchKernelGGL(gato::k_bwd_begin, dim3(B), dim3(256), 0, st, ab, s->a.lam, s->lam_b, s->lam_a, s->dz_b,
+                       s->dz_a, nlam, ndz);
+    hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->dz_v, (size_t)B * ndz);
+    hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->lam_v, (size_t)B * nlam);
+    hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, ab);
+    hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, ab);
+    launch_pcg<PL>(np, B, blk, lds, st, ab);
+    hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, ab, 0);
+    // correction solves: same blocks, right-hand side (q_a, r_a, c_a) -> gam_a, solution (dz_a, lam_a)
+    gato::Args ac = ab;
+    ac.q = s->q_a;
+    ac.r = s->r_a;
+    ac.gam = s->gam_a;
+    ac.lam = s->lam_a;
+    ac.dz = s->dz_a;
+    ac.max_pcg = s->bwd_max_pcg;
+    ac.tol = s->d_bwd_tol;
+    ac.pcg_abs_tol = 1e-30f;
+    auto correct = [&](const float* rz, const float* gq, const float* gr, const float* cc, float* z, float* l) {
+        hipLaunchKernelGGL((gato::k_kkt_resid<PL>), dim3(B), blk, lds, st, ab, rz, gq, gr, cc, z, l, s->q_a, s->r_a,
+                           s->c_a);
+        hipLaunchKernelGGL((gato::k_gamma_rhs<PL>), dim3(B), blk, lds, st, ab, s->q_a, s->r_a, s->c_a, s->gam_a);
+        launch_pcg<PL>(np, B, blk, lds, st, ac);
+        hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, ac, 0);
+        hipLaunchKernelGGL(gato::k_accumulate, dim3(B), dim3(256), 0, st, z, s->dz_a, l, s->lam_a, ndz, nlam);
+    };
+    for (int i = 0; i < s->bwd_refine; i++) correct(nullptr, s->a.q, s->a.r, s->a.c, s->dz_b, s->lam_b);
+    // adjoint: K [v; v_l] = [dL/dz; 0], from zero
+    for (int i = 0; i <= s->bwd_refine; i++) correct(dL, nullptr, nullptr, nullptr, s->dz_v, s->lam_v);
+    hipLaunchKernelGGL((gato::k_grad<PL>), dim3(B), blk, lds, st, ab, s->dz_v, s->lam_v, gxs, gref, gQ, gR);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 
@@ -1230,6 +1523,34 @@ int gato_debug_copy(gato_solver* s, const char* name, void* d_dst, int64_t cap, 
         return 0;
     }
     return fail(GATO_E_ARG, std::string("unknown workspace array ") + name);
+}
+
+int gato_set_backward_params(gato_solver* s, int32_t max_pcg_iters, float pcg_tol, int32_t refine) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    if (max_pcg_iters < 1 || !(pcg_tol >= 0.f) || refine < 0 || refine > 8)
+        return fail(GATO_E_ARG, "need max_pcg_iters >= 1, pcg_tol >= 0 and refine in [0, 8]");
+    s->bwd_max_pcg = max_pcg_iters;
+    s->bwd_tol = pcg_tol;
+    s->bwd_refine = refine;
+    return 0;
+}
+
+int gato_backward(gato_solver* s, const float* d_xu, const float* d_xs, const float* d_ref, float dt,
+                  const float* d_dL_dxu, float* d_grad_xs, float* d_grad_ref, float* d_grad_Qd, float* d_grad_Rd,
+                  void* stream, int32_t* h_adj_pcg_iters) {
+    if (!s || !d_xu || !d_xs || !d_ref || !d_dL_dxu || !d_grad_xs || !d_grad_ref || !d_grad_Qd || !d_grad_Rd)
+        return fail(GATO_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    fill_args(s, const_cast<float*>(d_xu), d_xs, d_ref, dt);
+    std::vector<float> tol(s->B, s->bwd_tol);
+    HIP_TRY(hipMemcpyAsync(s->d_bwd_tol, tol.data(), s->B * 4, hipMemcpyHostToDevice, st));
+    const int rc = s->plant == GATO_PLANT_IIWA14
+                       ? launch_backward<0>(s, d_dL_dxu, d_grad_xs, d_grad_ref, d_grad_Qd, d_grad_Rd, st)
+                       : launch_backward<1>(s, d_dL_dxu, d_grad_xs, d_grad_ref, d_grad_Qd, d_grad_Rd, st);
+    if (rc) return rc;
+    if (h_adj_pcg_iters) HIP_TRY(hipMemcpyAsync(h_adj_pcg_iters, s->bst_pcg, s->B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
 }
 
 int gato_last_kernel_time(gato_solver* s, double* avg_us, int32_t* launches) {

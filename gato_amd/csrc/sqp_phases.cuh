@@ -536,6 +536,14 @@ __device__ __forceinline__ float wave_sum(float v) {
     v += dpp_f<DPP_ROW_MIRROR>(v);
     return (lane_f(v, 0) + lane_f(v, 16)) + (lane_f(v, 32) + lane_f(v, 48));
 }
+// Sum over one 16-lane row (a knot group); every lane of the row receives the sum.
+__device__ __forceinline__ float row16_sum(float v) {
+    v += dpp_f<DPP_QUAD_1032>(v);
+    v += dpp_f<DPP_QUAD_2301>(v);
+    v += dpp_f<DPP_ROW_HALF_MIRROR>(v);
+    v += dpp_f<DPP_ROW_MIRROR>(v);
+    return v;
+}
 __device__ __forceinline__ float wave_max(float v) {
     v = fmaxf(v, dpp_f<DPP_QUAD_1032>(v));
     v = fmaxf(v, dpp_f<DPP_QUAD_2301>(v));

@@ -225,6 +225,30 @@ class Solver:
         check(self.lib.gato_set_joint_cost(self._h, *[None if b is None else _ptr(b) for b in bufs], self._stream()),
               "gato_set_joint_cost")
 
+    def backward_device(self, xu, xs, ref, grad_xu, dt=None):
+        """Gradients of a loss through the solve (the adjoint of the SQP step at the solution xu):
+        returns device tensors xs [B, nx], ref [B, 6N], Qd [B, nx], Rd [B, nu] and the adjoint PCG
+        iteration counts. xu is the trajectory a solve returned; grad_xu = dL/dxu [B, traj]."""
+        for t, n, name in ((xu, self.B * self.traj, "xu"), (xs, self.B * self.nx, "xs"), (ref, self.B * 6 * self.N, "ref"),
+                           (grad_xu, self.B * self.traj, "grad_xu")):
+            if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+                    and t.numel() == n):
+                raise ValueError(f"{name}: expected a contiguous float32 device tensor with {n} elements")
+        dt = self.cfg["dt"] if dt is None else dt
+        f = dict(dtype=torch.float32, device=self.device)
+        g = dict(xs=torch.empty(self.B, self.nx, **f), ref=torch.empty(self.B, 6 * self.N, **f),
+                 Qd=torch.empty(self.B, self.nx, **f), Rd=torch.empty(self.B, self.nu, **f))
+        it = np.zeros(self.B, np.int32)
+        check(self.lib.gato_backward(self._h, _ptr(xu), _ptr(xs), _ptr(ref), float(dt), _ptr(grad_xu), _ptr(g["xs"]),
+                                     _ptr(g["ref"]), _ptr(g["Qd"]), _ptr(g["Rd"]), self._stream(),
+                                     it.ctypes.data_as(C.POINTER(C.c_int32))), "gato_backward")
+        g["adj_pcg_iters"] = it
+        return g
+
+    def set_backward_params(self, max_pcg_iters=1000, pcg_tol=1e-12, refine=1):
+        check(self.lib.gato_set_backward_params(self._h, int(max_pcg_iters), float(pcg_tol), int(refine)),
+              "gato_set_backward_params")
+
     def get_state(self):
         lam = torch.zeros(self.B, self.N * self.nx, dtype=torch.float32, device=self.device)
         rho = np.zeros(self.B, np.float32)

@@ -122,6 +122,23 @@ int gato_set_state(gato_solver* s, const float* d_lambda, const float* h_rho, co
  * gato_set_f_ext_batch; d_xnext [B * nx]. */
 int gato_sim_forward(gato_solver* s, float* d_xnext, const float* d_x, const float* d_u, float dt, void* stream);
 
+/* Backward pass (no reference counterpart; the adjoint of the solve for differentiable MPC).
+ * Differentiates the SQP step map z+ = z + dz(z; theta) at the solution d_xu (the trajectory a solve
+ * returned, with the solver's current rho and lambda), theta = (x_s, reference xyz, diag Q, diag R), for the
+ * loss gradient d_dL_dxu [B * traj]. The step is re-linearised at d_xu; the adjoint KKT system is solved with
+ * that Schur complement and preconditioner by PCG (gato_set_backward_params). At a converged solution
+ * (dz = 0) this is the implicit derivative of the solution under the solver's Hessian model.
+ * Outputs (device): d_grad_xs [B * nx], d_grad_ref [B * 6N] (rpy entries 0), d_grad_Qd [B * nx],
+ * d_grad_Rd [B * nu] - the gradients w.r.t. the diagonal weights of gato_set_joint_cost (or, without it,
+ * of the weights the reference cost implies: Qd = [0, qd_cost], Rd = u_cost). h_adj_pcg_iters [B] (host,
+ * may be NULL) receives the adjoint PCG iteration counts. Does not change lambda, rho or the inputs. */
+int gato_backward(gato_solver* s, const float* d_xu, const float* d_xs, const float* d_ref, float dt,
+                  const float* d_dL_dxu, float* d_grad_xs, float* d_grad_ref, float* d_grad_Qd, float* d_grad_Rd,
+                  void* stream, int32_t* h_adj_pcg_iters);
+/* Backward solves: PCG iteration cap (default 1000), relative tolerance on r^T P r (default 1e-12) and
+ * rounds of iterative refinement of the step and adjoint solutions (default 1, at most 8). */
+int gato_set_backward_params(gato_solver* s, int32_t max_pcg_iters, float pcg_tol, int32_t refine);
+
 /* Plant dimensions: nq (= nu), nx = 2 nq. */
 int gato_plant_dims(int32_t plant, int32_t* nq, int32_t* nx);
 

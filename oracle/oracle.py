@@ -233,3 +233,100 @@ def sim_forward(plant, x, u, fext, dt):
     out = np.zeros((B, 2 * n))
     lib().orc_sim_forward(PLANTS[plant], B, _p(_f64(x)), _p(_f64(u)), _p(fext), C.c_double(dt), _p(out))
     return out
+
+
+# ----------------------------------------------------------------------------- adjoint of the SQP step
+# The backward pass differentiates the SQP step map z+ = z + dz(z; theta) at a fixed linearisation point z,
+# where [dz; lambda] solves the step's KKT system K [dz; lambda] = [-g; c] (G~ = blkdiag(Q~_k, R_k) with the
+# half-diagonal rho of linalg.cuh:85-96, C from the integrator Jacobians), for theta = (x_s, reference, diag Q,
+# diag R). At a converged solution dz = 0 and this is the implicit derivative of the solution under the
+# solver's own Hessian model. (No counterpart in the reference: the north-star adjoint.)
+
+
+def dense_kkt(kkt, rho, nq):
+    """G~ (with rho on the position diagonal), g, C and c of one problem's SQP step, dense."""
+    N, nx, nu = kkt["q"].shape[0], kkt["q"].shape[1], kkt["r"].shape[1]
+    nz = (nx + nu) * N - nu
+    off = lambda k: k * (nx + nu)
+    G = np.zeros((nz, nz))
+    g = np.zeros(nz)
+    for k in range(N):
+        Qt = kkt["Q"][k].copy()
+        Qt[np.arange(nq), np.arange(nq)] += rho
+        G[off(k):off(k) + nx, off(k):off(k) + nx] = Qt
+        g[off(k):off(k) + nx] = kkt["q"][k]
+        if k < N - 1:
+            G[off(k) + nx:off(k) + nx + nu, off(k) + nx:off(k) + nx + nu] = kkt["R"][k]
+            g[off(k) + nx:off(k) + nx + nu] = kkt["r"][k]
+    Cm = np.zeros((N * nx, nz))
+    Cm[:nx, :nx] = -np.eye(nx)
+    for k in range(N - 1):
+        r0 = (k + 1) * nx
+        Cm[r0:r0 + nx, off(k):off(k) + nx] = kkt["A"][k]
+        Cm[r0:r0 + nx, off(k) + nx:off(k) + nx + nu] = kkt["B"][k]
+        Cm[r0:r0 + nx, off(k + 1):off(k + 1) + nx] = -np.eye(nx)
+    return G, g, Cm, kkt["c"].reshape(-1)
+
+
+def _plant_name(code):
+    return {v: k for k, v in PLANTS.items()}[code]
+
+
+def _joint_weights(cfg, Qd, Rd):
+    nq_, nx, nu = dims(cfg)
+    Qw = np.r_[np.zeros(nq_), np.full(nq_, cfg.qd_cost)] if Qd is None else _f64(Qd)
+    Rw = np.full(nu, cfg.u_cost) if Rd is None else _f64(Rd)
+    return Qw, Rw
+
+
+def step_map(cfg, xu, xs, ref, rho, fext=None, Qd=None, Rd=None, xref=None):
+    """dz and lambda of one SQP step at xu by a dense FP64 solve of the step's KKT system."""
+    nq_, nx, nu = dims(cfg)
+    Qw, Rw = _joint_weights(cfg, Qd, Rd)
+    kkt = setup_kkt(cfg, xu, xs, ref, fext, Qw, Rw, xref)
+    G, g, Cm, c = dense_kkt(kkt, rho, nq_)
+    nz, nc = G.shape[0], Cm.shape[0]
+    K = np.block([[G, Cm.T], [Cm, np.zeros((nc, nc))]])
+    w = np.linalg.solve(K, np.concatenate([-g, c]))
+    return w[:nz], w[nz:], K
+
+
+def step_grad(cfg, xu, xs, ref, rho, dL_dz, fext=None, Qd=None, Rd=None, xref=None):
+    """Gradient of L(xu + dz(theta)) w.r.t. x_s [nx], reference [N, 6], diag Q [nx] and diag R [nu], given
+    dL/dz [traj], for one problem. Adjoint: K v = [dL/dz; 0]; dL/dtheta = v^T (d[-g; c]/dtheta - dK/dtheta w)."""
+    nq_, nx, nu = dims(cfg)
+    N = cfg.N
+    Qw, Rw = _joint_weights(cfg, Qd, Rd)
+    xu = _f64(xu)
+    dz_, lam, K = step_map(cfg, xu, xs, ref, rho, fext, Qw, Rw, xref)
+    nz = dz_.shape[0]
+    v = np.linalg.solve(K, np.concatenate([_f64(dL_dz), np.zeros(N * nx)]))
+    vz, vl = v[:nz], v[nz:]
+    xr = np.zeros((N, nx)) if xref is None else _f64(xref).reshape(N, nx)
+    ref = _f64(ref).reshape(N, 6)
+    st = nx + nu
+    gxs = -vl[:nx]
+    gQ = np.zeros(nx)
+    gR = np.zeros(nu)
+    gref = np.zeros((N, 6))
+    for k in range(N):
+        term = k == N - 1
+        src = (N - 1 if cfg.terminal_mode else N - 2) if term else k
+        xsrc = xu[src * st:src * st + nx]
+        vx, dx = vz[k * st:k * st + nx], dz_[k * st:k * st + nx]
+        gQ += -vx * (xsrc - xr[k]) - vx * dx
+        if not term:
+            u = xu[k * st + nx:k * st + nx + nu]
+            vu, du = vz[k * st + nx:k * st + nx + nu], dz_[k * st + nx:k * st + nx + nu]
+            gR += -vu * u - vu * du
+        w = (cfg.N_cost if cfg.terminal_mode else cfg.q_cost) if term else cfg.q_cost
+        p, J = fk(_plant_name(cfg.plant), xsrc[:nq_])
+        gvec = J.T @ (p - ref[k, :3])
+        vq, dq = vx[:nq_], dx[:nq_]
+        for j in range(3):
+            a = J[j]
+            val = w * (vq @ a)
+            if cfg.hess_mode == 0:
+                val += w * ((vq @ a) * (gvec @ dq) + (vq @ gvec) * (a @ dq))
+            gref[k, j] = val
+    return dict(xs=gxs, ref=gref, Qd=gQ, Rd=gR, dz=dz_, lam=lam, v=vz, v_lam=vl)

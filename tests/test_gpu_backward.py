@@ -1,0 +1,99 @@
+"""GPU backward pass (adjoint of the SQP step at the solution) against the FP64 oracle.
+
+The oracle's step_grad is pinned by central differences of the oracle's step map and by the implicit
+derivative at a fixed point (tests/test_oracle_adjoint.py). Here the HIP backward, called through the
+C-ABI, must reproduce it at the GPU's own solution, rho and weights: 1e-3 relative to each gradient's
+largest entry (the north-star bar for adjoint gradients).
+"""
+import numpy as np
+import pytest
+import torch
+
+from gato_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(plant, N, B, seed, hess_mode=0, terminal_mode=0):
+    from gato_amd.solver import Solver
+
+    wl = W.drop_in_figure8(plant, N=N, B=B, seed=seed)
+    rng = np.random.default_rng(seed)
+    nq = W.NQ[plant]
+    nx, nu = 2 * nq, nq
+    Qd = np.exp(rng.uniform(np.log(1e-2), np.log(1e1), (B, nx))).astype(np.float32)
+    Rd = np.exp(rng.uniform(np.log(1e-3), np.log(1e0), (B, nu))).astype(np.float32)
+    kw = dict(dt=0.01, max_sqp_iters=3, max_pcg_iters=200, pcg_tol=1e-6, mu=10.0, q_cost=10.0, N_cost=20.0,
+              qd_cost=1e-3, u_cost=1e-3, q_lim_cost=0.0, vel_lim_cost=0.0, ctrl_lim_cost=0.0, rho=1e-2)
+    s = Solver(plant, N, B, mode="graph", hess_mode=hess_mode, terminal_mode=terminal_mode, **kw)
+    s.set_joint_cost(Qd, Rd)
+    return s, wl, Qd, Rd, kw
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+@pytest.mark.parametrize("plant,hess,term", [("iiwa14", 0, 0), ("iiwa14", 1, 1), ("indy7", 0, 0)])
+def test_backward_matches_oracle_step_gradient(gpu, O, plant, hess, term):
+    N, B = 16, 4
+    s, wl, Qd, Rd, kw = _problem(plant, N, B, seed=21, hess_mode=hess, terminal_mode=term)
+    dev = s.device
+    xu = torch.tensor(wl.xu, dtype=torch.float32, device=dev)
+    xs = torch.tensor(wl.xs, dtype=torch.float32, device=dev)
+    ref = torch.tensor(wl.ref, dtype=torch.float32, device=dev)
+    s.solve_device(xu, xs, ref)
+    rng = np.random.default_rng(22)
+    gxu = rng.normal(0, 1, wl.xu.shape).astype(np.float32)
+    g = s.backward_device(xu, xs, ref, torch.tensor(gxu, device=dev))
+    assert np.all(g["adj_pcg_iters"] > 0)
+    _, rho, _ = s.get_state()
+    cfg = O.make_cfg(plant, N=N, dt=kw["dt"], q_cost=kw["q_cost"], N_cost=kw["N_cost"], qd_cost=kw["qd_cost"],
+                     u_cost=kw["u_cost"], q_lim_cost=0.0, hess_mode=hess, terminal_mode=term)
+    X = xu.cpu().numpy().astype(np.float64)
+    f = lambda a: a.astype(np.float32).astype(np.float64)
+    for b in range(B):
+        o = O.step_grad(cfg, X[b], f(wl.xs[b]), f(wl.ref[b]), float(rho[b]), gxu[b].astype(np.float64),
+                        Qd=Qd[b].astype(np.float64), Rd=Rd[b].astype(np.float64))
+        assert _rel(g["xs"][b].cpu().numpy(), o["xs"]) < 1e-3, b
+        assert _rel(g["ref"][b].cpu().numpy().reshape(N, 6), o["ref"]) < 1e-3, b
+        assert _rel(g["Qd"][b].cpu().numpy(), o["Qd"]) < 1e-3, b
+        assert _rel(g["Rd"][b].cpu().numpy(), o["Rd"]) < 1e-3, b
+
+
+def test_backward_leaves_the_solver_state_alone(gpu):
+    s, wl, Qd, Rd, _ = _problem("iiwa14", 8, 2, seed=23)
+    dev = s.device
+    xu = torch.tensor(wl.xu, dtype=torch.float32, device=dev)
+    xs = torch.tensor(wl.xs, dtype=torch.float32, device=dev)
+    ref = torch.tensor(wl.ref, dtype=torch.float32, device=dev)
+    s.solve_device(xu, xs, ref)
+    lam, rho, drho = s.get_state()
+    x_before = xu.clone()
+    s.backward_device(xu, xs, ref, torch.ones_like(xu))
+    lam2, rho2, drho2 = s.get_state()
+    np.testing.assert_array_equal(lam, lam2)
+    np.testing.assert_array_equal(rho, rho2)
+    assert torch.equal(xu, x_before)
+
+
+def test_autograd_function_routes_through_the_backward(gpu):
+    from gato_amd.autograd import solve
+
+    s, wl, Qd, Rd, _ = _problem("iiwa14", 8, 2, seed=24)
+    dev = s.device
+    xs = torch.tensor(wl.xs, dtype=torch.float32, device=dev, requires_grad=True)
+    ref = torch.tensor(wl.ref, dtype=torch.float32, device=dev, requires_grad=True)
+    Qt = torch.tensor(Qd, device=dev, requires_grad=True)
+    Rt = torch.tensor(Rd, device=dev, requires_grad=True)
+    xu0 = torch.tensor(wl.xu, dtype=torch.float32, device=dev)
+    XU = solve(s, xs, ref, Qt, Rt, xu0)
+    nx = W.NQ["iiwa14"] * 2
+    loss = (XU[:, nx:nx + 7] ** 2).sum() + (XU ** 2).sum() * 1e-3  # u_0 energy + a small trajectory term
+    loss.backward()
+    gx = 2e-3 * XU.detach().clone()
+    gx[:, nx:nx + 7] += 2 * XU.detach()[:, nx:nx + 7]
+    g = s.backward_device(XU.detach().contiguous(), xs.detach(), ref.detach(), gx.contiguous())
+    for t, k in ((xs, "xs"), (ref, "ref"), (Qt, "Qd"), (Rt, "Rd")):
+        assert t.grad is not None and torch.isfinite(t.grad).all()
+        torch.testing.assert_close(t.grad, g[k], rtol=1e-5, atol=1e-6)

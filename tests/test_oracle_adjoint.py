@@ -1,0 +1,96 @@
+"""Pins of the oracle's adjoint of the SQP step (the reference for the GPU backward pass).
+
+step_grad differentiates L(z + dz(theta)) at a fixed linearisation point z, where dz solves the step's KKT
+system, w.r.t. theta = (x_s, reference xyz, diag Q, diag R). It is checked against central differences of
+the same step map (dense FP64 solves of the oracle's KKT system at perturbed theta) and, at a point where the
+step vanishes, against the closed form dz/dtheta = -[I 0] K^-1 d[g; -c]/dtheta.
+"""
+import numpy as np
+import pytest
+
+from gato_amd import workloads as W
+
+
+def _setup(O, plant, N, seed, **cfg_kw):
+    wl = W.drop_in_figure8(plant, N=N, B=1, seed=seed)
+    rng = np.random.default_rng(seed)
+    nq = W.NQ[plant]
+    nx, nu = 2 * nq, nq
+    xu = wl.xu[0] + rng.normal(0, 0.05, wl.xu.shape[1])
+    Qd = np.exp(rng.uniform(np.log(1e-2), np.log(1e1), nx))
+    Rd = np.exp(rng.uniform(np.log(1e-3), np.log(1e0), nu))
+    base = dict(plant=plant, N=N, dt=0.01, q_cost=10.0, N_cost=20.0, qd_cost=1e-3, u_cost=1e-3, q_lim_cost=0.0,
+                vel_lim_cost=0.0, ctrl_lim_cost=0.0)
+    base.update(cfg_kw)
+    cfg = O.make_cfg(**base)
+    return cfg, xu, wl.xs[0], wl.ref[0].reshape(N, 6).copy(), Qd, Rd
+
+
+def _loss(O, cfg, xu, xs, ref, rho, Qd, Rd, wvec):
+    dz, _, _ = O.step_map(cfg, xu, xs, ref.reshape(-1), rho, Qd=Qd, Rd=Rd)
+    return wvec @ (xu + dz)
+
+
+@pytest.mark.parametrize("plant,hess,term", [("iiwa14", 0, 0), ("iiwa14", 1, 1), ("indy7", 0, 0)])
+def test_step_gradient_matches_central_differences(O, plant, hess, term):
+    N, rho, h = 5, 1e-2, 1e-6
+    cfg, xu, xs, ref, Qd, Rd = _setup(O, plant, N, seed=3, hess_mode=hess, terminal_mode=term)
+    rng = np.random.default_rng(4)
+    wvec = rng.normal(0, 1, xu.shape[0])
+    gr = O.step_grad(cfg, xu, xs, ref.reshape(-1), rho, wvec, Qd=Qd, Rd=Rd)
+
+    def fd(get, put, n):
+        out = np.zeros(n)
+        for i in range(n):
+            lp, lm = put(i, h), put(i, -h)
+            out[i] = (lp - lm) / (2 * h)
+        return out
+
+    def with_xs(i, d):
+        x = xs.copy(); x[i] += d
+        return _loss(O, cfg, xu, x, ref, rho, Qd, Rd, wvec)
+
+    def with_ref(i, d):
+        r = ref.copy(); r[i // 3, i % 3] += d
+        return _loss(O, cfg, xu, xs, r, rho, Qd, Rd, wvec)
+
+    def with_Q(i, d):
+        q = Qd.copy(); q[i] += d
+        return _loss(O, cfg, xu, xs, ref, rho, q, Rd, wvec)
+
+    def with_R(i, d):
+        r = Rd.copy(); r[i] += d
+        return _loss(O, cfg, xu, xs, ref, rho, Qd, r, wvec)
+
+    nx, nu = xs.shape[0], Rd.shape[0]
+    checks = [(gr["xs"], fd(None, with_xs, nx)), (gr["ref"][:, :3].reshape(-1), fd(None, with_ref, 3 * N)),
+              (gr["Qd"], fd(None, with_Q, nx)), (gr["Rd"], fd(None, with_R, nu))]
+    for an, num in checks:
+        scale = max(np.abs(num).max(), 1e-12)
+        assert np.abs(an - num).max() <= 1e-5 * scale + 1e-8, (an, num)
+    assert np.all(gr["ref"][:, 3:] == 0)  # the orientation part of the reference is unused
+
+
+def test_gradient_at_a_fixed_point_is_the_implicit_derivative(O):
+    # Drive z to the step map's fixed point (dz = 0) and compare with -[I 0] K^-1 d[g; -c]/d x_s: with
+    # dz = 0 the dK/dtheta terms vanish, and x_s enters only through c_0 = x_0 - x_s. The consistent terminal
+    # block is used: with the reference's (gradient at x_{N-2}, curvature on x_{N-1}) the step never vanishes.
+    N, rho = 4, 1e-2
+    cfg, xu, xs, ref, Qd, Rd = _setup(O, "iiwa14", N, seed=5, q_cost=0.0, N_cost=0.0, terminal_mode=1)
+    for _ in range(60):
+        dz, _, _ = O.step_map(cfg, xu, xs, ref.reshape(-1), rho, Qd=Qd, Rd=Rd)
+        xu = xu + dz
+    dz, _, K = O.step_map(cfg, xu, xs, ref.reshape(-1), rho, Qd=Qd, Rd=Rd)
+    assert np.abs(dz).max() < 1e-9
+    nx = xs.shape[0]
+    nz = xu.shape[0]
+    rhs = np.zeros(K.shape[0])
+    wvec = np.random.default_rng(6).normal(0, 1, nz)
+    gr = O.step_grad(cfg, xu, xs, ref.reshape(-1), rho, wvec, Qd=Qd, Rd=Rd)
+    # d[dz; lambda]/dx_s,i = K^-1 [0; dc/dx_s,i] with dc_0/dx_s = -I
+    expect = np.zeros(nx)
+    for i in range(nx):
+        rhs[:] = 0
+        rhs[nz + i] = -1.0
+        expect[i] = wvec @ np.linalg.solve(K, rhs)[:nz]
+    np.testing.assert_allclose(gr["xs"], expect, rtol=1e-9, atol=1e-12)
