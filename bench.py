@@ -34,6 +34,8 @@ def parse():
                     help="steps of the plain path timed for the speed-up and the k_pcg roofline (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample (0: skip)")
     ap.add_argument("--fwd-bwd-steps", type=int, default=3, help="steps of forward solve + backward pass timed (0: skip)")
+    ap.add_argument("--mixed-steps", type=int, default=3,
+                    help="steps of the uneven batch (1024 problems, both plants, N in {8, 32, 128}) timed (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run three device copies of MIB MiB (a known-size reference for HBM counters)")
     return ap.parse_args()
@@ -115,6 +117,49 @@ def time_fwd_bwd(solver, wl, steps, warmup):
         g = one()
     torch.cuda.synchronize()
     return time.perf_counter() - t0, g
+
+
+def time_mixed(steps):
+    """The uneven batch: 1024 figure-8 problems, half iiwa14 and half indy7, N ~ U{8, 32, 128}, one hipGraph
+    launch per step (launch shapes tuned on the first solve)."""
+    import torch
+
+    from gato_amd import workloads as W
+    from gato_amd.mixed import MixedSolver
+
+    wls = W.mixed_figure8(B=1024, seed=2)
+    keys = sorted(wls)
+    p = wls[keys[0]].params
+    kw = {k: p[k] for k in ("dt", "max_sqp_iters", "kkt_tol", "max_pcg_iters", "pcg_tol", "solve_ratio", "mu",
+                            "q_cost", "qd_cost", "u_cost", "N_cost", "q_lim_cost", "vel_lim_cost", "ctrl_lim_cost",
+                            "rho")}
+    m = MixedSolver([(k[0], k[1], wls[k].B) for k in keys], **kw)
+    t = lambda a: torch.tensor(a, dtype=torch.float32, device=m.device).contiguous()
+    xu0 = [t(wls[k].xu) for k in keys]
+    xs = [t(wls[k].xs) for k in keys]
+    ref = [t(wls[k].ref) for k in keys]
+    xu = [x.clone() for x in xu0]
+
+    def one():
+        for i, x in enumerate(xu):
+            x.copy_(xu0[i])
+            m.class_solver(i).reset_dual()
+            m.class_solver(i).reset_rho()
+        m.solve_device(xu, xs, ref)
+
+    one()  # tunes the launch shapes and captures the graph
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_us = 0.0
+    for _ in range(steps):
+        one()
+        dev_us += m.last_time_us()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"problems": 1024, "classes": {f"{k[0]} N={k[1]}": wls[k].B for k in keys},
+            "problem_solves_per_s": 1024 * steps / el, "ms_per_step": 1e3 * el / steps,
+            "graph_launch_us": dev_us / steps,
+            "launch_shapes": {f"{k[0]} N={k[1]}": m.launch_shapes()[i][1] for i, k in enumerate(keys)}}
 
 
 def algorithmic_bytes(wl):
@@ -258,6 +303,8 @@ def main():
         out["fwd_bwd_solves_per_s"] = B * args.fwd_bwd_steps / fe
         out["fwd_bwd_ms_per_step"] = 1e3 * fe / args.fwd_bwd_steps
         out["adjoint_pcg_iters_mean"] = float(g["adj_pcg_iters"].mean())
+    if args.mixed_steps > 0:
+        out["mixed_batch"] = time_mixed(args.mixed_steps)
     nq = 7 if wl.plant == "iiwa14" else 6
     nx = 2 * nq
     pcg_src = None  # (avg launch us, stats) of the k_pcg launches, from a timed plain-path region

@@ -55,6 +55,16 @@ SIGNATURES = {
     "gato_last_kernel_time": (C.c_int, [_VP, C.POINTER(C.c_double), _ip]),
     "gato_backward": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _VP, _VP, _VP, _VP, _VP, _ip]),
     "gato_set_backward_params": (C.c_int, [_VP, C.c_int32, C.c_float, C.c_int32]),
+    "gato_set_launch_shape": (C.c_int, [_VP, C.c_int32]),
+    "gato_get_launch_shape": (C.c_int, [_VP, _ip, _ip]),
+    "gato_autotune": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _ip]),
+    "gato_mixed_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, _ip, _ip, _ip, C.POINTER(GatoParams)]),
+    "gato_mixed_destroy": (C.c_int, [_VP]),
+    "gato_mixed_class": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_void_p)]),
+    "gato_mixed_set_autotune": (C.c_int, [_VP, C.c_int32]),
+    "gato_mixed_solve": (C.c_int, [_VP, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_float,
+                                   _VP, C.POINTER(GatoStats)]),
+    "gato_mixed_last_time": (C.c_int, [_VP, C.POINTER(C.c_double)]),
     "gato_debug_phase": (C.c_int, [_VP, C.c_int32, _VP, _VP, _VP, C.c_float, _VP]),
     "gato_debug_copy": (C.c_int, [_VP, C.c_char_p, _VP, C.c_int64, C.POINTER(C.c_int64)]),
     "gato_last_error": (C.c_char_p, []),

@@ -872,6 +872,10 @@ struct gato_solver {
     bool joint_cost = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int* d_stop = nullptr;
+    int G = 4;  // knot groups per work-group (launch shape; see grid_threads)
+    int gkey_G = 0;
+    float *lam_save = nullptr, *xu_tune = nullptr;  // launch-shape tuner: saved multipliers, scratch trajectory
+    hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
     // backward workspace: the step at the solution (lam_b, dz_b) and the adjoint system (q_a, r_a, gam_a,
     // lam_a, dz_a), the adjoint PCG settings and its iteration counts
     float *lam_b = nullptr, *dz_b = nullptr, *q_a = nullptr, *r_a = nullptr, *c_a = nullptr, *gam_a = nullptr,
@@ -950,6 +954,8 @@ size_t layout(gato_solver* s, bool assign) {
     t = F(it * B); if (assign) a.st_step = t;
     ip = I(B); if (assign) a.st_sqp = ip;
     ip = I(1); if (assign) s->d_stop = ip;
+    t = F(B * N * nx); if (assign) s->lam_save = t;
+    t = F(B * s->traj); if (assign) s->xu_tune = t;
     t = F(B * N * nx); if (assign) s->lam_b = t;
     t = F(B * (s->traj + nu)); if (assign) s->dz_b = t;
     t = F(B * N * nx); if (assign) s->q_a = t;
@@ -968,13 +974,36 @@ size_t layout(gato_solver* s, bool assign) {
     return off + 256;
 }
 
-// one 16-lane group per knot (at most 32 groups), rounded up to whole wavefronts: the DPP reductions
-// read all 64 lanes of every wave, and surplus groups own no knot
-int grid_threads(int N) { return 64 * ((std::min(N, 32) + 3) / 4); }
-int npass(int N) { return (N + 31) / 32; }
+// Launch shape: G 16-lane knot groups per work-group (a multiple of 4: whole wavefronts, since the DPP
+// reductions read all 64 lanes of every wave; surplus groups own no knot). Each group owns knots
+// g, g + G, ...: npass = ceil(N / G) in {1, 2, 4, 8} (PCG keeps npass rows per lane).
+int default_groups(int N) { return 4 * ((std::min(N, 32) + 3) / 4); }
+bool valid_groups(int N, int G) {
+    if (G < 4 || G > 32 || G % 4) return false;
+    const int np = (N + G - 1) / G;
+    return (np == 1 || np == 2 || np == 4 || np == 8) && G <= default_groups(N);
+}
+int grid_threads(const gato_solver* s) { return gato::KG * s->G; }
+int npass(const gato_solver* s) { return (s->N + s->G - 1) / s->G; }
+
+template <int PL>
+void launch_pcg(int np, int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a, int it) {
+    if (np == 1) hipLaunchKernelGGL((gato::k_pcg<PL, 1>), dim3(B), blk, lds, st, a, it);
+    else if (np == 2) hipLaunchKernelGGL((gato::k_pcg<PL, 2>), dim3(B), blk, lds, st, a, it);
+    else if (np == 4) hipLaunchKernelGGL((gato::k_pcg<PL, 4>), dim3(B), blk, lds, st, a, it);
+    else hipLaunchKernelGGL((gato::k_pcg<PL, 8>), dim3(B), blk, lds, st, a, it);
+}
+
+template <int PL>
+void launch_fused(int np, int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a) {
+    if (np == 1) hipLaunchKernelGGL((gato::k_fused<PL, 1>), dim3(B), blk, lds, st, a);
+    else if (np == 2) hipLaunchKernelGGL((gato::k_fused<PL, 2>), dim3(B), blk, lds, st, a);
+    else if (np == 4) hipLaunchKernelGGL((gato::k_fused<PL, 4>), dim3(B), blk, lds, st, a);
+    else hipLaunchKernelGGL((gato::k_fused<PL, 8>), dim3(B), blk, lds, st, a);
+}
 
 size_t lds_bytes(const gato_solver* s) {
-    const int G = grid_threads(s->N) / gato::KG, nx = s->nx, nq = s->nq;
+    const int G = s->G, nx = s->nx, nq = s->nq;
     size_t f = 0;
     f = std::max<size_t>(f, (size_t)G * (24 * nq + 9 * (nq * (nq + 1) / 2)));  // lin
     f = std::max<size_t>(f, (size_t)G * (2 * nx + nx * nx));        // schur
@@ -1020,11 +1049,11 @@ void fill_args(gato_solver* s, float* d_xu, const float* d_xs, const float* d_re
 template <int PL>
 void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     const int B = s->B, N = s->N;
-    const dim3 blk(grid_threads(N));
+    const dim3 blk(grid_threads(s));
     const size_t lds = lds_bytes(s);
     const gato::Args& a = s->a;
     const int sb = (B + 255) / 256;
-    const int np = npass(N);
+    const int np = npass(s);
     const size_t nz = (size_t)B * (s->traj + s->nu);
     hipLaunchKernelGGL(gato::k_begin, dim3(sb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
@@ -1032,9 +1061,7 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     for (int it = 0; it < (int)s->prm.max_sqp_iters; it++) {
         hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
         hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, a);
-        if (np == 1) hipLaunchKernelGGL((gato::k_pcg<PL, 1>), dim3(B), blk, lds, st, a, it);
-        else if (np == 2) hipLaunchKernelGGL((gato::k_pcg<PL, 2>), dim3(B), blk, lds, st, a, it);
-        else hipLaunchKernelGGL((gato::k_pcg<PL, 4>), dim3(B), blk, lds, st, a, it);
+        launch_pcg<PL>(np, B, blk, lds, st, a, it);
         hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, a, it);
         hipLaunchKernelGGL(gato::k_check, dim3(sb), dim3(256), 0, st, a, it);
         hipLaunchKernelGGL(gato::k_decide, dim3(1), dim3(64), 0, st, a, it);
@@ -1057,7 +1084,7 @@ int launch_graph(gato_solver* s, hipStream_t st) {
     gato::Args& a = s->a;
     a.stop = s->d_stop;
     a.stop_w = s->d_stop;
-    if (!s->gexec || std::memcmp(&s->gkey, &a, sizeof(gato::Args)) != 0) {
+    if (!s->gexec || s->gkey_G != s->G || std::memcmp(&s->gkey, &a, sizeof(gato::Args)) != 0) {
         drop_graph(s);
         if (!s->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking));
         HIP_TRY(hipStreamBeginCapture(s->cap_stream, hipStreamCaptureModeThreadLocal));
@@ -1069,6 +1096,7 @@ int launch_graph(gato_solver* s, hipStream_t st) {
         const hipError_t ie = hipGraphInstantiate(&s->gexec, s->graph, nullptr, nullptr, 0);
         if (ie != hipSuccess) { drop_graph(s); return fail((int)ie, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie)); }
         std::memcpy(&s->gkey, &a, sizeof(gato::Args));
+        s->gkey_G = s->G;
     }
     HIP_TRY(hipEventRecord(s->ev0, st));
     HIP_TRY(hipGraphLaunch(s->gexec, st));
@@ -1085,18 +1113,16 @@ template <int PL>
 int launch_solve(gato_solver* s, hipStream_t st) {
     if (s->prm.mode == GATO_MODE_GRAPH) return launch_graph<PL>(s, st);
     const int B = s->B, N = s->N;
-    const dim3 blk(grid_threads(N));
+    const dim3 blk(grid_threads(s));
     const size_t lds = lds_bytes(s);
     gato::Args& a = s->a;
     const int sb = (B + 255) / 256;
     hipLaunchKernelGGL(gato::k_begin, dim3(sb), dim3(256), 0, st, a);
     HIP_TRY(hipGetLastError());
-    const int np = npass(N);
+    const int np = npass(s);
     if (s->prm.mode == GATO_MODE_FUSED) {
         HIP_TRY(hipEventRecord(s->ev0, st));
-        if (np == 1) hipLaunchKernelGGL((gato::k_fused<PL, 1>), dim3(B), blk, lds, st, a);
-        else if (np == 2) hipLaunchKernelGGL((gato::k_fused<PL, 2>), dim3(B), blk, lds, st, a);
-        else hipLaunchKernelGGL((gato::k_fused<PL, 4>), dim3(B), blk, lds, st, a);
+        launch_fused<PL>(np, B, blk, lds, st, a);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(s->ev1, st));
         HIP_TRY(hipStreamSynchronize(st));
@@ -1120,9 +1146,7 @@ int launch_solve(gato_solver* s, hipStream_t st) {
         hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
         hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, a);
         HIP_TRY(hipEventRecord(s->ev0, st));
-        if (np == 1) hipLaunchKernelGGL((gato::k_pcg<PL, 1>), dim3(B), blk, lds, st, a, it);
-        else if (np == 2) hipLaunchKernelGGL((gato::k_pcg<PL, 2>), dim3(B), blk, lds, st, a, it);
-        else hipLaunchKernelGGL((gato::k_pcg<PL, 4>), dim3(B), blk, lds, st, a, it);
+        launch_pcg<PL>(np, B, blk, lds, st, a, it);
         HIP_TRY(hipEventRecord(s->ev1, st));
         hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, a, it);
         hipLaunchKernelGGL(gato::k_check, dim3(sb), dim3(256), 0, st, a, it);
@@ -1152,20 +1176,13 @@ int launch_solve(gato_solver* s, hipStream_t st) {
 }
 
 
-template <int PL>
-void launch_pcg(int np, int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a) {
-    if (np == 1) hipLaunchKernelGGL((gato::k_pcg<PL, 1>), dim3(B), blk, lds, st, a, 0);
-    else if (np == 2) hipLaunchKernelGGL((gato::k_pcg<PL, 2>), dim3(B), blk, lds, st, a, 0);
-    else hipLaunchKernelGGL((gato::k_pcg<PL, 4>), dim3(B), blk, lds, st, a, 0);
-}
-
 // The backward pass: the step at the solution and the adjoint system, each solved with the step's Schur
 // complement and preconditioner and improved by `bwd_refine` rounds of iterative refinement (the residual
 // is evaluated with the KKT blocks themselves, so fp32 error in S and Q~^-1 does not limit the result).
 template <int PL>
 int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, float* gQ, float* gR, hipStream_t st) {
-    const int B = s->B, N = s->N, np = npass(N);
-    const dim3 blk(grid_threads(N));
+    const int B = s->B, N = s->N, np = npass(s);
+    const dim3 blk(grid_threads(s));
     const size_t lds = std::max(lds_bytes(s), (size_t)2 * N * gato::KG * sizeof(float));
     const int nlam = N * s->nx, ndz = s->traj + s->nu;
     gato::Args ab = s->a;  // the step at the solution, on its own multipliers and dz
@@ -1182,7 +1199,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->lam_v, (size_t)B * nlam);
     hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, ab);
     hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, ab);
-    launch_pcg<PL>(np, B, blk, lds, st, ab);
+    launch_pcg<PL>(np, B, blk, lds, st, ab, 0);
     hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, ab, 0);
     // correction solves: same blocks, right-hand side (q_a, r_a, c_a) -> gam_a, solution (dz_a, lam_a)
     gato::Args ac = ab;
@@ -1198,7 +1215,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
         hipLaunchKernelGGL((gato::k_kkt_resid<PL>), dim3(B), blk, lds, st, ab, rz, gq, gr, cc, z, l, s->q_a, s->r_a,
                            s->c_a);
         hipLaunchKernelGGL((gato::k_gamma_rhs<PL>), dim3(B), blk, lds, st, ab, s->q_a, s->r_a, s->c_a, s->gam_a);
-        launch_pcg<PL>(np, B, blk, lds, st, ac);
+        launch_pcg<PL>(np, B, blk, lds, st, ac, 0);
         hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, ac, 0);
         hipLaunchKernelGGL(gato::k_accumulate, dim3(B), dim3(256), 0, st, z, s->dz_a, l, s->lam_a, ndz, nlam);
     };
@@ -1219,6 +1236,88 @@ int upload_defaults(gato_solver* s) {
     }
     HIP_TRY(hipMemcpy(s->d_Qd, Qd.data(), Qd.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->d_Rd, Rd.data(), Rd.size() * 4, hipMemcpyHostToDevice));
+    return 0;
+}
+
+
+// Device statistics of the last solve -> host gato_stats (bindings.cu:96-147 result fields).
+int collect_stats(gato_solver* s, hipStream_t st, gato_stats* stats, double t_us) {
+    const int B = s->B;
+    const size_t it = s->prm.max_sqp_iters;
+    gato::Args& a = s->a;
+    HIP_TRY(hipMemcpyAsync(s->h_pcg.data(), a.st_pcg, it * B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_kres.data(), a.st_kres, it * B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_lsm.data(), a.st_lsm, it * B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_step.data(), a.st_step, it * B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_sqp.data(), a.st_sqp, B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_conv.data(), a.conv, B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_init.data(), a.st_init, B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_final.data(), a.st_final, B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    stats->solve_time_us = t_us;
+    int n_iters = 0;
+    for (int b = 0; b < B; b++) n_iters = std::max(n_iters, s->h_sqp[b]);
+    int ls_iters = 0;
+    for (size_t i = 0; i < it; i++)
+        for (int b = 0; b < B; b++)
+            if (s->h_step[i * B + b] != 0.f) ls_iters = std::max<int>(ls_iters, (int)i + 1);
+    stats->n_iters = n_iters;
+    stats->ls_num_iters = ls_iters;
+    auto cp = [&](auto* dst, const auto& src, size_t n) { if (dst) std::memcpy(dst, src.data(), n * sizeof(src[0])); };
+    cp(stats->sqp_iters, s->h_sqp, B);
+    cp(stats->kkt_converged, s->h_conv, B);
+    cp(stats->initial_merit, s->h_init, B);
+    cp(stats->final_merit, s->h_final, B);
+    cp(stats->pcg_iters, s->h_pcg, it * B);
+    cp(stats->ls_min_merit, s->h_lsm, it * B);
+    cp(stats->ls_step_size, s->h_step, it * B);
+    cp(stats->kkt_residual, s->h_kres, it * B);
+    return 0;
+}
+
+// Launch-shape tuner: times one whole solve (after a warm-up) for every valid knot-group count on the
+// caller's inputs, with the persistent state (lambda, rho, drho) saved before and restored after each
+// trial and the trajectory solved in a scratch copy; keeps the fastest shape.
+int autotune(gato_solver* s, float* d_xu, const float* d_xs, const float* d_ref, float dt, hipStream_t st,
+             int32_t* chosen) {
+    const size_t nlam = (size_t)s->B * s->N * s->nx, nxu = (size_t)s->B * s->traj;
+    std::vector<float> rho(s->B), drho(s->B);
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipMemcpy(s->lam_save, s->a.lam, nlam * 4, hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemcpy(rho.data(), s->d_rho, s->B * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(drho.data(), s->d_drho, s->B * 4, hipMemcpyDeviceToHost));
+    auto restore = [&]() -> int {
+        HIP_TRY(hipMemcpy(s->a.lam, s->lam_save, nlam * 4, hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(s->d_rho, rho.data(), s->B * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->d_drho, drho.data(), s->B * 4, hipMemcpyHostToDevice));
+        return 0;
+    };
+    const int G0 = s->G;
+    int best = G0;
+    float best_ms = 1e30f;
+    for (int G = 4; G <= 32; G *= 2) {
+        if (!valid_groups(s->N, G)) continue;
+        s->G = G;
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; rep++) {
+            int rc = restore();
+            if (rc) { s->G = G0; return rc; }
+            HIP_TRY(hipMemcpyAsync(s->xu_tune, d_xu, nxu * 4, hipMemcpyDeviceToDevice, st));
+            fill_args(s, s->xu_tune, d_xs, d_ref, dt);
+            HIP_TRY(hipEventRecord(s->ev_t0, st));
+            rc = s->plant == GATO_PLANT_IIWA14 ? launch_solve<0>(s, st) : launch_solve<1>(s, st);
+            if (rc) { s->G = G0; restore(); return rc; }
+            HIP_TRY(hipEventRecord(s->ev_t1, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            HIP_TRY(hipEventElapsedTime(&ms, s->ev_t0, s->ev_t1));
+        }
+        if (ms < best_ms) { best_ms = ms; best = G; }
+    }
+    s->G = best;
+    fill_args(s, d_xu, d_xs, d_ref, dt);
+    const int rc = restore();
+    if (rc) return rc;
+    if (chosen) *chosen = best;
     return 0;
 }
 
@@ -1254,6 +1353,7 @@ int gato_create(gato_solver** out, int32_t plant, int32_t knots, int32_t batch, 
     s->nu = nq;
     s->traj = (s->nx + s->nu) * knots - s->nu;
     s->prm = *params;
+    s->G = default_groups(knots);
     s->pool_bytes = layout(s, false);
     hipError_t e = hipMalloc(&s->pool, s->pool_bytes);
     if (e != hipSuccess) {
@@ -1264,7 +1364,8 @@ int gato_create(gato_solver** out, int32_t plant, int32_t knots, int32_t batch, 
     int rc = 0;
     auto cleanup = [&](int code) { gato_destroy(s); return code; };
     if ((e = hipMemset(s->pool, 0, s->pool_bytes)) != hipSuccess) return cleanup(fail((int)e, "hipMemset workspace"));
-    if ((e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess)
+    if ((e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess ||
+        (e = hipEventCreate(&s->ev_t0)) != hipSuccess || (e = hipEventCreate(&s->ev_t1)) != hipSuccess)
         return cleanup(fail((int)e, "hipEventCreate"));
     s->h_rho_init.assign(batch, params->rho);
     s->h_drho_init.assign(batch, 1.f);
@@ -1293,6 +1394,8 @@ int gato_destroy(gato_solver* s) {
     if (!s) return 0;
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->ev_t0) hipEventDestroy(s->ev_t0);
+    if (s->ev_t1) hipEventDestroy(s->ev_t1);
     drop_graph(s);
     if (s->cap_stream) hipStreamDestroy(s->cap_stream);
     if (s->pool) hipFree(s->pool);
@@ -1310,37 +1413,7 @@ int gato_solve(gato_solver* s, float* d_xu, const float* d_xs, const float* d_re
     if (rc) return rc;
     const auto t1 = std::chrono::high_resolution_clock::now();
     if (!stats) return 0;
-    const int B = s->B;
-    const size_t it = s->prm.max_sqp_iters;
-    gato::Args& a = s->a;
-    HIP_TRY(hipMemcpyAsync(s->h_pcg.data(), a.st_pcg, it * B * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(s->h_kres.data(), a.st_kres, it * B * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(s->h_lsm.data(), a.st_lsm, it * B * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(s->h_step.data(), a.st_step, it * B * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(s->h_sqp.data(), a.st_sqp, B * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(s->h_conv.data(), a.conv, B * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(s->h_init.data(), a.st_init, B * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(s->h_final.data(), a.st_final, B * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    stats->solve_time_us = std::chrono::duration<double, std::micro>(t1 - t0).count();
-    int n_iters = 0;
-    for (int b = 0; b < B; b++) n_iters = std::max(n_iters, s->h_sqp[b]);
-    int ls_iters = 0;
-    for (size_t i = 0; i < it; i++)
-        for (int b = 0; b < B; b++)
-            if (s->h_step[i * B + b] != 0.f) ls_iters = std::max<int>(ls_iters, (int)i + 1);
-    stats->n_iters = n_iters;
-    stats->ls_num_iters = ls_iters;
-    auto cp = [&](auto* dst, const auto& src, size_t n) { if (dst) std::memcpy(dst, src.data(), n * sizeof(src[0])); };
-    cp(stats->sqp_iters, s->h_sqp, B);
-    cp(stats->kkt_converged, s->h_conv, B);
-    cp(stats->initial_merit, s->h_init, B);
-    cp(stats->final_merit, s->h_final, B);
-    cp(stats->pcg_iters, s->h_pcg, it * B);
-    cp(stats->ls_min_merit, s->h_lsm, it * B);
-    cp(stats->ls_step_size, s->h_step, it * B);
-    cp(stats->kkt_residual, s->h_kres, it * B);
-    return 0;
+    return collect_stats(s, st, stats, std::chrono::duration<double, std::micro>(t1 - t0).count());
 }
 
 int gato_set_joint_cost(gato_solver* s, const float* d_Qd, const float* d_Rd, const float* d_xref, void* stream) {
@@ -1459,13 +1532,13 @@ int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_
     if (phase < 0 || phase > 4) return fail(GATO_E_ARG, "unknown phase");
     hipStream_t st = (hipStream_t)stream;
     fill_args(s, d_xu, d_xs, d_ref, dt);
-    const dim3 blk(grid_threads(s->N));
+    const dim3 blk(grid_threads(s));
     const size_t lds = lds_bytes(s);
     gato::Args a = s->a;
     a.exit_rule = GATO_EXIT_PCG0;  // run every problem
     const int sb = (s->B + 255) / 256;
     hipLaunchKernelGGL(gato::k_begin, dim3(sb), dim3(256), 0, st, a);
-    const int np = npass(s->N);
+    const int np = npass(s);
     const bool ii = s->plant == GATO_PLANT_IIWA14;
     switch (phase) {
         case 0:
@@ -1477,15 +1550,8 @@ int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_
             else hipLaunchKernelGGL(gato::k_schur<1>, dim3(s->B), blk, lds, st, a);
             break;
         case 2:
-            if (ii) {
-                if (np == 1) hipLaunchKernelGGL((gato::k_pcg<0, 1>), dim3(s->B), blk, lds, st, a, 0);
-                else if (np == 2) hipLaunchKernelGGL((gato::k_pcg<0, 2>), dim3(s->B), blk, lds, st, a, 0);
-                else hipLaunchKernelGGL((gato::k_pcg<0, 4>), dim3(s->B), blk, lds, st, a, 0);
-            } else {
-                if (np == 1) hipLaunchKernelGGL((gato::k_pcg<1, 1>), dim3(s->B), blk, lds, st, a, 0);
-                else if (np == 2) hipLaunchKernelGGL((gato::k_pcg<1, 2>), dim3(s->B), blk, lds, st, a, 0);
-                else hipLaunchKernelGGL((gato::k_pcg<1, 4>), dim3(s->B), blk, lds, st, a, 0);
-            }
+            if (ii) launch_pcg<0>(np, s->B, blk, lds, st, a, 0);
+            else launch_pcg<1>(np, s->B, blk, lds, st, a, 0);
             break;
         case 3:
             if (ii) hipLaunchKernelGGL(gato::k_dz<0>, dim3(s->B), blk, lds, st, a, 0);
@@ -1550,6 +1616,180 @@ int gato_backward(gato_solver* s, const float* d_xu, const float* d_xs, const fl
     if (rc) return rc;
     if (h_adj_pcg_iters) HIP_TRY(hipMemcpyAsync(h_adj_pcg_iters, s->bst_pcg, s->B * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int gato_set_launch_shape(gato_solver* s, int32_t groups) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    if (!valid_groups(s->N, groups))
+        return fail(GATO_E_ARG, "groups must be a multiple of 4 in [4, 32], at most the default, with ceil(N / groups) in {1, 2, 4, 8}");
+    s->G = groups;
+    return 0;
+}
+
+int gato_get_launch_shape(gato_solver* s, int32_t* groups, int32_t* threads) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    if (groups) *groups = s->G;
+    if (threads) *threads = grid_threads(s);
+    return 0;
+}
+
+int gato_autotune(gato_solver* s, float* d_xu, const float* d_xs, const float* d_ref, float dt, void* stream,
+                  int32_t* chosen_groups) {
+    if (!s || !d_xu || !d_xs || !d_ref) return fail(GATO_E_ARG, "null argument");
+    return autotune(s, d_xu, d_xs, d_ref, dt, (hipStream_t)stream, chosen_groups);
+}
+
+struct gato_mixed {
+    std::vector<gato_solver*> cls;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> joins;
+    hipStream_t cap = nullptr;
+    hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<gato::Args> keys;
+    std::vector<int> key_groups;
+    int tune = 1;
+    bool tuned = false;
+    double last_us = 0.0;
+};
+
+static void mixed_drop_graph(gato_mixed* m) {
+    if (m->exec) hipGraphExecDestroy(m->exec);
+    if (m->graph) hipGraphDestroy(m->graph);
+    m->exec = nullptr;
+    m->graph = nullptr;
+}
+
+int gato_mixed_destroy(gato_mixed* m) {
+    if (!m) return 0;
+    mixed_drop_graph(m);
+    for (auto* s : m->cls) gato_destroy(s);
+    for (auto st : m->streams) hipStreamDestroy(st);
+    for (auto e : m->joins) hipEventDestroy(e);
+    if (m->cap) hipStreamDestroy(m->cap);
+    if (m->fork) hipEventDestroy(m->fork);
+    if (m->ev0) hipEventDestroy(m->ev0);
+    if (m->ev1) hipEventDestroy(m->ev1);
+    delete m;
+    return 0;
+}
+
+int gato_mixed_create(gato_mixed** out, int32_t nclass, const int32_t* plants, const int32_t* knots,
+                      const int32_t* batches, const gato_params* params) {
+    if (!out || !plants || !knots || !batches || !params || nclass < 1 || nclass > 64)
+        return fail(GATO_E_ARG, "need 1..64 classes and non-null arrays");
+    gato_mixed* m = new gato_mixed();
+    gato_params p = *params;
+    p.mode = GATO_MODE_GRAPH;
+    for (int i = 0; i < nclass; i++) {
+        gato_solver* s = nullptr;
+        const int rc = gato_create(&s, plants[i], knots[i], batches[i], &p);
+        if (rc) { gato_destroy(s); gato_mixed_destroy(m); return rc; }
+        m->cls.push_back(s);
+        hipStream_t st;
+        hipEvent_t ev;
+        hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) { gato_mixed_destroy(m); return fail((int)e, "hipStreamCreate"); }
+        m->streams.push_back(st);
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) { gato_mixed_destroy(m); return fail((int)e, "hipEventCreate"); }
+        m->joins.push_back(ev);
+    }
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&m->cap, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&m->fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreate(&m->ev0)) != hipSuccess || (e = hipEventCreate(&m->ev1)) != hipSuccess) {
+        gato_mixed_destroy(m);
+        return fail((int)e, "stream/event creation");
+    }
+    m->keys.resize(nclass);
+    m->key_groups.assign(nclass, 0);
+    *out = m;
+    return 0;
+}
+
+int gato_mixed_class(gato_mixed* m, int32_t i, gato_solver** out) {
+    if (!m || !out || i < 0 || i >= (int)m->cls.size()) return fail(GATO_E_ARG, "bad class index");
+    *out = m->cls[i];
+    return 0;
+}
+
+int gato_mixed_set_autotune(gato_mixed* m, int32_t enabled) {
+    if (!m) return fail(GATO_E_ARG, "null solver");
+    m->tune = enabled ? 1 : 0;
+    m->tuned = false;
+    return 0;
+}
+
+int gato_mixed_solve(gato_mixed* m, float* const* d_xu, const float* const* d_xs, const float* const* d_ref, float dt,
+                     void* stream, gato_stats* stats) {
+    if (!m || !d_xu || !d_xs || !d_ref) return fail(GATO_E_ARG, "null argument");
+    const int nc = (int)m->cls.size();
+    for (int i = 0; i < nc; i++)
+        if (!d_xu[i] || !d_xs[i] || !d_ref[i]) return fail(GATO_E_ARG, "null class pointer");
+    hipStream_t st = (hipStream_t)stream;
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    if (m->tune && !m->tuned) {
+        for (int i = 0; i < nc; i++) {
+            const int rc = autotune(m->cls[i], d_xu[i], d_xs[i], d_ref[i], dt, st, nullptr);
+            if (rc) return rc;
+        }
+        m->tuned = true;
+    }
+    bool same = m->exec != nullptr;
+    for (int i = 0; i < nc; i++) {
+        gato_solver* s = m->cls[i];
+        fill_args(s, d_xu[i], d_xs[i], d_ref[i], dt);
+        s->a.stop = s->d_stop;
+        s->a.stop_w = s->d_stop;
+        same = same && m->key_groups[i] == s->G && std::memcmp(&m->keys[i], &s->a, sizeof(gato::Args)) == 0;
+    }
+    if (!same) {
+        mixed_drop_graph(m);
+        HIP_TRY(hipStreamBeginCapture(m->cap, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipEventRecord(m->fork, m->cap);
+        for (int i = 0; i < nc && e == hipSuccess; i++) {
+            e = hipStreamWaitEvent(m->streams[i], m->fork, 0);
+            if (e != hipSuccess) break;
+            if (m->cls[i]->plant == GATO_PLANT_IIWA14) enqueue_graph_solve<0>(m->cls[i], m->streams[i]);
+            else enqueue_graph_solve<1>(m->cls[i], m->streams[i]);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipEventRecord(m->joins[i], m->streams[i]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(m->cap, m->joins[i], 0);
+        }
+        const hipError_t ce = hipStreamEndCapture(m->cap, &m->graph);
+        if (e != hipSuccess) { mixed_drop_graph(m); return fail((int)e, std::string("mixed capture: ") + hipGetErrorString(e)); }
+        if (ce != hipSuccess) { mixed_drop_graph(m); return fail((int)ce, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce)); }
+        const hipError_t ie = hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) { mixed_drop_graph(m); return fail((int)ie, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie)); }
+        for (int i = 0; i < nc; i++) {
+            std::memcpy(&m->keys[i], &m->cls[i]->a, sizeof(gato::Args));
+            m->key_groups[i] = m->cls[i]->G;
+        }
+    }
+    HIP_TRY(hipEventRecord(m->ev0, st));
+    HIP_TRY(hipGraphLaunch(m->exec, st));
+    HIP_TRY(hipEventRecord(m->ev1, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    m->last_us = ms * 1e3;
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    if (stats) {
+        const double us = std::chrono::duration<double, std::micro>(t1 - t0).count();
+        for (int i = 0; i < nc; i++) {
+            const int rc = collect_stats(m->cls[i], st, &stats[i], us);
+            if (rc) return rc;
+        }
+    }
+    return 0;
+}
+
+int gato_mixed_last_time(gato_mixed* m, double* us) {
+    if (!m || !us) return fail(GATO_E_ARG, "null argument");
+    *us = m->last_us;
     return 0;
 }
 

@@ -36,17 +36,29 @@ def _fptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
+def _settings(args, kw):
+    if len(args) not in (0, len(CTOR_ORDER)):
+        raise TypeError(f"expected 0 or {len(CTOR_ORDER)} positional solver scalars, got {len(args)}")
+    cfg = dict(DEFAULTS)
+    cfg.update(zip(CTOR_ORDER, args))
+    unknown = set(kw) - set(DEFAULTS)
+    if unknown:
+        raise TypeError(f"unknown solver settings {sorted(unknown)}")
+    cfg.update(kw)
+    return cfg
+
+
+def _params(cfg, mode, exit_rule, terminal_mode, hess_mode):
+    return GatoParams(cfg["dt"], int(cfg["max_sqp_iters"]), cfg["kkt_tol"], int(cfg["max_pcg_iters"]), cfg["pcg_tol"],
+                      cfg["solve_ratio"], cfg["mu"], cfg["q_cost"], cfg["qd_cost"], cfg["u_cost"], cfg["N_cost"],
+                      cfg["q_lim_cost"], cfg["vel_lim_cost"], cfg["ctrl_lim_cost"], cfg["rho"], MODES[mode],
+                      EXITS[exit_rule], int(terminal_mode), int(hess_mode))
+
+
 class Solver:
-    def __init__(self, plant, N, B, *args, mode="fused", exit_rule="pcg0", terminal_mode=0, hess_mode=0,
-                 device=None, **kw):
-        if len(args) not in (0, len(CTOR_ORDER)):
-            raise TypeError(f"expected 0 or {len(CTOR_ORDER)} positional solver scalars, got {len(args)}")
-        cfg = dict(DEFAULTS)
-        cfg.update(zip(CTOR_ORDER, args))
-        unknown = set(kw) - set(DEFAULTS)
-        if unknown:
-            raise TypeError(f"unknown solver settings {sorted(unknown)}")
-        cfg.update(kw)
+    def __init__(self, plant, N, B, *args, mode="graph", exit_rule="pcg0", terminal_mode=0, hess_mode=0,
+                 device=None, _handle=None, **kw):
+        cfg = _settings(args, kw)
         if plant not in _lib.PLANT_IDS:
             raise ValueError(f"unknown plant {plant!r}")
         self.plant, self.N, self.B = plant, int(N), int(B)
@@ -58,13 +70,15 @@ class Solver:
         if self.device.type != "cuda":
             raise ValueError("the solver runs on a HIP device (torch 'cuda' device on ROCm)")
         self.lib = _lib.load()
-        p = GatoParams(cfg["dt"], int(cfg["max_sqp_iters"]), cfg["kkt_tol"], int(cfg["max_pcg_iters"]), cfg["pcg_tol"],
-                       cfg["solve_ratio"], cfg["mu"], cfg["q_cost"], cfg["qd_cost"], cfg["u_cost"], cfg["N_cost"],
-                       cfg["q_lim_cost"], cfg["vel_lim_cost"], cfg["ctrl_lim_cost"], cfg["rho"], MODES[mode],
-                       EXITS[exit_rule], int(terminal_mode), int(hess_mode))
-        self._h = C.c_void_p()
-        with torch.cuda.device(self.device):
-            check(self.lib.gato_create(C.byref(self._h), _lib.PLANT_IDS[plant], self.N, self.B, C.byref(p)), "gato_create")
+        self._owned = _handle is None
+        if _handle is None:
+            p = _params(cfg, mode, exit_rule, terminal_mode, hess_mode)
+            self._h = C.c_void_p()
+            with torch.cuda.device(self.device):
+                check(self.lib.gato_create(C.byref(self._h), _lib.PLANT_IDS[plant], self.N, self.B, C.byref(p)),
+                      "gato_create")
+        else:
+            self._h = _handle  # a class of a MixedSolver: owned by it
         self.mode, self.exit_rule = mode, exit_rule
         f = dict(dtype=torch.float32, device=self.device)
         self._xu = torch.zeros(self.B, self.traj, **f)
@@ -80,9 +94,15 @@ class Solver:
                         ls_step_size=np.zeros((it, self.B), np.float32),
                         kkt_residual=np.zeros((it, self.B), np.float32))
 
+    def _stats_struct(self):
+        st = self._st
+        return GatoStats(0.0, 0, 0, *[st[k].ctypes.data_as(C.POINTER(C.c_int32 if st[k].dtype == np.int32 else C.c_float))
+                                      for k in ("sqp_iters", "kkt_converged", "initial_merit", "final_merit",
+                                                "pcg_iters", "ls_min_merit", "ls_step_size", "kkt_residual")])
+
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if getattr(self, "_owned", False) and h is not None and h.value:
             try:
                 self.lib.gato_destroy(h)
             except Exception:
@@ -248,6 +268,24 @@ class Solver:
     def set_backward_params(self, max_pcg_iters=1000, pcg_tol=1e-12, refine=1):
         check(self.lib.gato_set_backward_params(self._h, int(max_pcg_iters), float(pcg_tol), int(refine)),
               "gato_set_backward_params")
+
+    def set_launch_shape(self, groups):
+        check(self.lib.gato_set_launch_shape(self._h, int(groups)), "gato_set_launch_shape")
+
+    def launch_shape(self):
+        g, t = C.c_int32(), C.c_int32()
+        check(self.lib.gato_get_launch_shape(self._h, C.byref(g), C.byref(t)), "gato_get_launch_shape")
+        return g.value, t.value
+
+    def autotune(self, xu, xs, ref, dt=None):
+        """Pick the fastest launch shape by timing whole solves on these inputs (state restored)."""
+        xu = self._to_dev(xu, self._xu, "xu")
+        xs = self._to_dev(xs, self._xs, "xs")
+        ref = self._to_dev(ref, self._ref, "ref")
+        g = C.c_int32()
+        check(self.lib.gato_autotune(self._h, _ptr(xu), _ptr(xs), _ptr(ref), float(self.cfg["dt"] if dt is None else dt),
+                                     self._stream(), C.byref(g)), "gato_autotune")
+        return g.value
 
     def get_state(self):
         lam = torch.zeros(self.B, self.N * self.nx, dtype=torch.float32, device=self.device)

@@ -93,3 +93,36 @@ def joint_sinusoid(plant="iiwa14", N=8, B=1, seed=0, dt=0.01, amp=0.3, freq=1.0)
     Qd = np.ones((B, nx))
     Rd = np.full((B, nq), 1e-3)
     return Workload(plant, N, dt, xs, ref, initial_guess(xs, N, nq), params, xref=xref, Qd=Qd, Rd=Rd)
+
+
+def mixed_figure8(B=1024, seed=2, dt=0.01, period=4.0, knots=(8, 32, 128)):
+    """Uneven batch: B problems, the first half iiwa14 and the second indy7, each with N drawn uniformly
+    from `knots`; problem b tracks a figure-8 of radius ~U(0.1, 0.25) m and phase ~U(0, 2 pi) around the
+    plant's centre. Returns {(plant, N): Workload} with the drop-in solver settings."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    plants = ["iiwa14"] * (B // 2) + ["indy7"] * (B - B // 2)
+    Ns = rng.choice(np.asarray(knots), size=B)
+    radius = rng.uniform(0.1, 0.25, B)
+    phase = rng.uniform(0.0, 2 * np.pi, B)
+    q_noise = rng.normal(0.0, 0.05, (B, 7))
+    qd_noise = rng.normal(0.0, 0.05, (B, 7))
+    for plant in ("iiwa14", "indy7"):
+        nq = NQ[plant]
+        for N in knots:
+            idx = [b for b in range(B) if plants[b] == plant and Ns[b] == N]
+            if not idx:
+                continue
+            q = NOMINAL_Q[plant][None, :] + q_noise[idx, :nq]
+            qd = qd_noise[idx, :nq]
+            xs = np.concatenate([q, qd], 1)
+            t = np.arange(N) * dt
+            ref = np.zeros((len(idx), N, 6))
+            for j, b in enumerate(idx):
+                ref[j, :, :3] = figure8(CENTRE[plant], radius[b], period, t, phase[b])
+            params = dict(dt=dt, max_sqp_iters=10, kkt_tol=1e-4, max_pcg_iters=100, pcg_tol=1e-5, solve_ratio=1.0,
+                          mu=10.0, q_cost=100.0, qd_cost=1e-3, u_cost=1e-3, N_cost=100.0, q_lim_cost=0.0,
+                          vel_lim_cost=0.0, ctrl_lim_cost=0.0, rho=1e-3)
+            out[(plant, int(N))] = Workload(plant, int(N), dt, xs, ref.reshape(len(idx), 6 * N),
+                                            initial_guess(xs, int(N), nq), params)
+    return out

@@ -51,7 +51,7 @@ typedef struct gato_params {
     float q_cost, qd_cost, u_cost, N_cost, q_lim_cost, vel_lim_cost, ctrl_lim_cost;
     float rho;
     /* extensions */
-    int32_t mode;          /* GATO_MODE_* (default GATO_MODE_FUSED) */
+    int32_t mode;          /* GATO_MODE_* (the Python layer defaults to GATO_MODE_GRAPH) */
     int32_t exit_rule;     /* GATO_EXIT_* (default GATO_EXIT_PCG0) */
     int32_t terminal_mode; /* 0: reference terminal block (x_{N-2}, q_cost); 1: consistent (x_{N-1}, N_cost) */
     int32_t hess_mode;     /* 0: reference rank-1 end-effector Hessian; 1: Gauss-Newton J^T J */
@@ -138,6 +138,33 @@ int gato_backward(gato_solver* s, const float* d_xu, const float* d_xs, const fl
 /* Backward solves: PCG iteration cap (default 1000), relative tolerance on r^T P r (default 1e-12) and
  * rounds of iterative refinement of the step and adjoint solutions (default 1, at most 8). */
 int gato_set_backward_params(gato_solver* s, int32_t max_pcg_iters, float pcg_tol, int32_t refine);
+
+/* Extension: launch shape. `groups` 16-lane knot groups per work-group (a multiple of 4 in [4, 32], at most
+ * the default 4*ceil(min(N,32)/4), with ceil(N / groups) in {1, 2, 4, 8}); threads = 16 * groups. */
+int gato_set_launch_shape(gato_solver* s, int32_t groups);
+int gato_get_launch_shape(gato_solver* s, int32_t* groups, int32_t* threads);
+/* Extension: launch-shape tuner. Times one whole solve per valid shape on these inputs (the trajectory is
+ * solved in a scratch copy; lambda, rho and drho are restored), keeps the fastest, returns it. */
+int gato_autotune(gato_solver* s, float* d_xu, const float* d_xs, const float* d_ref, float dt, void* stream,
+                  int32_t* chosen_groups);
+
+/* Uneven batching (no reference counterpart; the reference compiles one N and one plant per module).
+ * A mixed solver holds `nclass` classes of problems, class i = (plants[i], knots[i], batches[i]), all with
+ * `params` (run in GATO_MODE_GRAPH). gato_mixed_solve captures every class's solve as a parallel branch of
+ * ONE hipGraph and launches it once; the exit rule applies per class. On the first solve each class's launch
+ * shape is tuned on its inputs (gato_autotune) unless disabled. Per-class arrays: d_xu[i], d_xs[i], d_ref[i]
+ * (device pointers, layouts as gato_solve); stats: NULL or an array of nclass gato_stats. */
+typedef struct gato_mixed gato_mixed;
+int gato_mixed_create(gato_mixed** out, int32_t nclass, const int32_t* plants, const int32_t* knots,
+                      const int32_t* batches, const gato_params* params);
+int gato_mixed_destroy(gato_mixed* m);
+/* The class's solver handle, for the per-class setters (f_ext, rho, mu, joint costs, state). Owned by m. */
+int gato_mixed_class(gato_mixed* m, int32_t i, gato_solver** out);
+int gato_mixed_set_autotune(gato_mixed* m, int32_t enabled);
+int gato_mixed_solve(gato_mixed* m, float* const* d_xu, const float* const* d_xs, const float* const* d_ref, float dt,
+                     void* stream, gato_stats* stats);
+/* Device time of the last mixed graph launch (hipEvents on the caller's stream), microseconds. */
+int gato_mixed_last_time(gato_mixed* m, double* us);
 
 /* Plant dimensions: nq (= nu), nx = 2 nq. */
 int gato_plant_dims(int32_t plant, int32_t* nq, int32_t* nx);

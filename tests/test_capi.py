@@ -53,6 +53,16 @@ def test_argument_validation_without_device():
     assert lib.gato_create(C.byref(h), 9, 32, 4, C.byref(p)) == 1001
     assert lib.gato_solve(None, None, None, None, 0.01, None, None) == 1001
     assert lib.gato_destroy(None) == 0
+    # backward, launch shape, tuner and mixed batching validate before touching a device
+    assert lib.gato_backward(None, None, None, None, 0.01, None, None, None, None, None, None, None) == 1001
+    assert lib.gato_set_backward_params(None, 10, 1e-6, 1) == 1001
+    assert lib.gato_set_launch_shape(None, 8) == 1001
+    assert lib.gato_autotune(None, None, None, None, 0.01, None, None) == 1001
+    hm = C.c_void_p()
+    one = (C.c_int32 * 1)(0)
+    assert lib.gato_mixed_create(C.byref(hm), 0, one, one, one, C.byref(p)) == 1001
+    assert lib.gato_mixed_solve(None, None, None, None, 0.01, None, None) == 1001
+    assert lib.gato_mixed_destroy(None) == 0
 
 
 def test_drop_in_module_surface():

@@ -81,14 +81,15 @@ def test_one_iteration_figure8_consistent_with_oracle(gpu, O, plant, mode):
 
 
 @pytest.mark.parametrize("mode", ["plain", "fused", "graph"])
-@pytest.mark.parametrize("pcg_tol,bound", [(1e-7, 1e-4), (1e-5, 5e-4)])
+@pytest.mark.parametrize("pcg_tol,bound", [(1e-7, 2e-4), (1e-5, 5e-4)])
 def test_joint_space_iterations_match_oracle_in_lockstep(gpu, O, mode, pcg_tol, bound):
     # the plain SQP case: iiwa14, N=8, joint sinusoid, Q = I, R = 1e-3 I. Each SQP iteration of the GPU
     # is compared with one oracle iteration started from the GPU's own state (trajectory, lambda, rho),
     # so every comparison is one iteration of the same function on the same inputs. The PCG's relative
     # stopping tolerance bounds the agreement: fp32 and fp64 CG stop an iteration apart near the threshold,
-    # which moves dz by ~sqrt(tol) of its size - about 3e-4 of the trajectory at the configuration's 1e-5,
-    # below 1e-4 at 1e-7.
+    # which moves dz by ~sqrt(tol) of its size - about 3e-4 of the trajectory at the configuration's 1e-5.
+    # At 1e-7 the fp32 Schur blocks set the floor instead (relative error ~1e-4, amplified by the
+    # conditioning of S): the typical problem agrees to 1e-5, the worst of these four to 1.5e-4.
     wl = W.joint_sinusoid("iiwa14", N=8, B=4, seed=0)
     wl.params["pcg_tol"] = pcg_tol
     s = make(wl, mode, max_sqp_iters=1)
@@ -98,7 +99,9 @@ def test_joint_space_iterations_match_oracle_in_lockstep(gpu, O, mode, pcg_tol, 
         lam, rho, _ = s.get_state()
         r = s.solve(xu, wl.dt, wl.xs, wl.ref)
         o = oracle_solve(O, wl, 1, joint=True, xu=xu, lam=lam, rho=rho)
-        assert xu_rel(r["XU"], o["XU"]).max() < bound, it
+        e = xu_rel(r["XU"], o["XU"])
+        assert e.max() < bound, it
+        assert np.median(e) < 2e-5, it
         np.testing.assert_array_equal(r["ls_step_size"][0], o["ls_step"][0])
         assert np.all(np.abs(r["all_pcg_iters"][0] - o["pcg_iters"][0]) <= 2)
         # the merit's defect terms mu |x_{k+1} - f(x_k)|_1 are differences of nearly equal states, so the
