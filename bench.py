@@ -34,6 +34,9 @@ def parse():
                     help="steps of the plain path timed for the speed-up and the k_pcg roofline (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample (0: skip)")
     ap.add_argument("--fwd-bwd-steps", type=int, default=3, help="steps of forward solve + backward pass timed (0: skip)")
+    ap.add_argument("--scale-steps", type=int, default=5,
+                    help="Adam steps of the scale workload (iiwa14 N=128 B=4096, shared diag Q/R learned through "
+                         "the backward pass) timed (0: skip)")
     ap.add_argument("--mixed-steps", type=int, default=3,
                     help="steps of the uneven batch (1024 problems, both plants, N in {8, 32, 128}) timed (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -160,6 +163,64 @@ def time_mixed(steps):
             "problem_solves_per_s": 1024 * steps / el, "ms_per_step": 1e3 * el / steps,
             "graph_launch_us": dev_us / steps,
             "launch_shapes": {f"{k[0]} N={k[1]}": m.launch_shapes()[i][1] for i, k in enumerate(keys)}}
+
+
+def time_scale(steps):
+    """iiwa14 N=128 B=4096 figure-8s of seeded radius and phase. Shared log-weights theta = log diag(Q, R) are
+    learned with Adam (lr 1e-2) so that the solutions reproduce those of hidden teacher weights; one step is a
+    forward solve of all 4096 problems, the backward pass, and the sum over problems of the per-problem weight
+    gradients (the one reduction the path has)."""
+    import numpy as np
+    import torch
+
+    from gato_amd import workloads as W
+    from gato_amd.autograd import solve
+    from gato_amd.solver import Solver
+
+    wl = W.scale_figure8(N=128, B=4096, seed=4)
+    p = wl.params
+    kw = {k: p[k] for k in ("dt", "max_sqp_iters", "kkt_tol", "max_pcg_iters", "pcg_tol", "solve_ratio", "mu",
+                            "q_cost", "qd_cost", "u_cost", "N_cost", "q_lim_cost", "vel_lim_cost", "ctrl_lim_cost",
+                            "rho")}
+    s = Solver(wl.plant, wl.N, wl.B, mode="graph", **kw)
+    dev = s.device
+    t = lambda a: torch.tensor(a, dtype=torch.float32, device=dev).contiguous()
+    xs, ref, xu0 = t(wl.xs), t(wl.ref), t(wl.xu)
+    B, nx, nu = wl.B, s.nx, s.nu
+    rng = np.random.default_rng(7)
+    teacher = torch.tensor(rng.uniform(np.log(1e-2), np.log(1e1), nx + nu), dtype=torch.float32, device=dev)
+    theta = torch.zeros(nx + nu, dtype=torch.float32, device=dev, requires_grad=True)
+    opt = torch.optim.Adam([theta], lr=1e-2)
+
+    def forward(th):
+        w = torch.exp(th)
+        s.reset_dual()
+        s.reset_rho()
+        return solve(s, xs, ref, w[:nx].expand(B, nx).contiguous(), w[nx:].expand(B, nu).contiguous(), xu0)
+
+    with torch.no_grad():
+        target = forward(teacher).detach()
+    losses = []
+
+    def step():
+        opt.zero_grad()
+        XU = forward(theta)
+        loss = ((XU - target) ** 2).sum(1).mean()
+        loss.backward()
+        opt.step()
+        losses.append(float(loss.detach()))
+
+    step()  # graph capture and warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"workload": "iiwa14 N=128 B=4096 figure-8s (seeded radius/phase), shared diag Q/R learned by Adam",
+            "adam_steps": steps, "fwd_bwd_solves_per_s": B * steps / el, "ms_per_step": 1e3 * el / steps,
+            "loss_first": losses[0], "loss_last": losses[-1],
+            "compulsory_hbm_GBs": 4 * (nx + 6 * wl.N + 2 * s.traj + 2 * nx * wl.N + 7) * B * steps / el / 1e9}
 
 
 def algorithmic_bytes(wl):
@@ -305,6 +366,8 @@ def main():
         out["adjoint_pcg_iters_mean"] = float(g["adj_pcg_iters"].mean())
     if args.mixed_steps > 0:
         out["mixed_batch"] = time_mixed(args.mixed_steps)
+    if args.scale_steps > 0:
+        out["scale"] = time_scale(args.scale_steps)
     nq = 7 if wl.plant == "iiwa14" else 6
     nx = 2 * nq
     pcg_src = None  # (avg launch us, stats) of the k_pcg launches, from a timed plain-path region

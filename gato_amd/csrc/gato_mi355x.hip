@@ -111,8 +111,8 @@ __device__ __forceinline__ void phase_schur(const Args& a, int b, float rho, flo
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
     View<PL> v(a, b);
     const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
-    float* piv = sm + g * (2 * NX + NX * NX);
-    float* Tsh = piv + 2 * NX;
+    float* tile = sm + g * SchurTile<PL>::SIZE;
+    float* piv = tile + SchurTile<PL>::PIV_OFF;
     for (int k = g; k < N; k += G)
         invert_q_knot<PL>(lane, k < N - 1, rho, v.Qp + k * NQ * NQ, v.Qv + k * NQ, v.Rg + k * NU,
                           v.Qip + k * NQ * NQ, v.Qiv + k * NQ, v.Ri + k * NU, piv);
@@ -124,7 +124,7 @@ __device__ __forceinline__ void phase_schur(const Args& a, int b, float rho, flo
                           v.Qiv + k * NQ, v.Ri + k * NU, v.Qip + (k + 1) * NQ * NQ, v.Qiv + (k + 1) * NQ,
                           v.q + k * NX, v.q + (k + 1) * NX, v.r + k * NU, v.c + (k + 1) * NX,
                           v.Sd + (size_t)(k + 1) * NX * NX, v.So + (size_t)(k + 1) * NX * NX,
-                          v.Pd + (size_t)(k + 1) * NX * NX, v.gam + (k + 1) * NX, piv, phi, prow);
+                          v.Pd + (size_t)(k + 1) * NX * NX, v.gam + (k + 1) * NX, tile, phi, prow);
         } else {
             schur_row0<PL>(lane, rho, v.Qp, v.Qv, v.Qip, v.Qiv, v.q, v.c, v.Sd, v.Pd, v.gam);
         }
@@ -140,7 +140,7 @@ __device__ __forceinline__ void phase_schur(const Args& a, int b, float rho, flo
             phi[m] = So1[rr * NX + m];
             prow[m] = Pd1[rr * NX + m];
         }
-        precond_offdiag<PL>(lane, phi, prow, v.Pd + (size_t)k * NX * NX, Tsh, v.Po + (size_t)(k + 1) * NX * NX);
+        precond_offdiag<PL>(lane, phi, prow, v.Pd + (size_t)k * NX * NX, tile, v.Po + (size_t)(k + 1) * NX * NX);
     }
 }
 
@@ -1006,7 +1006,7 @@ size_t lds_bytes(const gato_solver* s) {
     const int G = s->G, nx = s->nx, nq = s->nq;
     size_t f = 0;
     f = std::max<size_t>(f, (size_t)G * (24 * nq + 9 * (nq * (nq + 1) / 2)));  // lin
-    f = std::max<size_t>(f, (size_t)G * (2 * nx + nx * nx));        // schur
+    f = std::max<size_t>(f, (size_t)G * (nx * 16 + nx * 8 + nq * 8 + 2 * nx));  // schur (SchurTile)
     f = std::max<size_t>(f, (size_t)2 * (s->N + 2) * 16 + 32);      // pcg
     f = std::max<size_t>(f, (size_t)G * gato::KG + 16);             // dz
     f = std::max<size_t>(f, (size_t)2 * gato::NUM_ALPHAS + 16);     // merit + ls

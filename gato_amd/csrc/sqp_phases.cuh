@@ -295,41 +295,64 @@ __device__ __forceinline__ void qinv_row(const float* Qip, const float* Qiv, int
 // theta_k = A Q~_k^-1 A^T + B R^-1 B^T + Q~_{k+1}^-1, phi_k = A Q~_k^-1,
 // gamma_{k+1} = c_{k+1} - Q~_{k+1}^-1 q_{k+1} + phi_k q_k + B R^-1 r_k,
 // S_{k+1,k+1} = -theta_k, S_{k+1,k} = phi_k, P_{k+1,k+1} = -(theta_k + rho diag(1_nq, 0))^-1 (quirk Q8).
-// Lane r keeps its phi_k row and P_{k+1,k+1} row for step 3.
+// The knot's A, B and position block of Q~_k^-1 are staged in the knot group's LDS tile (SchurTile) so the
+// products read them as broadcasts from LDS instead of from global memory.
+template <int PL>
+struct SchurTile {
+    static constexpr int NQ = Plant<PL>::NQ, NX = Plant<PL>::NX;
+    static constexpr int AS = 16, BS = 8, QS = 8;            // row strides (floats)
+    static constexpr int A_OFF = 0, B_OFF = NX * AS, Q_OFF = B_OFF + NX * BS, PIV_OFF = Q_OFF + NQ * QS;
+    static constexpr int SIZE = PIV_OFF + 2 * NX;            // floats per knot group
+};
+
 template <int PL>
 __device__ __forceinline__ void schur_row(int lane, float rho, const float* A, const float* Bm, const float* Qip0,
                                           const float* Qiv0, const float* Ri0, const float* Qip1, const float* Qiv1,
                                           const float* q0, const float* q1, const float* r0, const float* c1,
-                                          float* Sd1, float* So1, float* Pd1, float* g1, float* piv, float* phi,
+                                          float* Sd1, float* So1, float* Pd1, float* g1, float* tile, float* phi,
                                           float* prow) {
     using P = Plant<PL>;
+    using T = SchurTile<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
+    float* As = tile + T::A_OFF;
+    float* Bs = tile + T::B_OFF;
+    float* Qs = tile + T::Q_OFF;
+    float* piv = tile + T::PIV_OFF;
     const int r = lane < NX ? lane : 0;
-    float arow[NX];
+    if (lane < NX) {
 #pragma unroll
-    for (int l = 0; l < NX; l++) arow[l] = A[r * NX + l];
+        for (int l = 0; l < NX; l++) As[r * T::AS + l] = A[r * NX + l];
+#pragma unroll
+        for (int l = 0; l < NU; l++) Bs[r * T::BS + l] = Bm[r * NU + l];
+        if (lane < NQ) {
+#pragma unroll
+            for (int l = 0; l < NQ; l++) Qs[r * T::QS + l] = Qip0[r * NQ + l];
+        }
+    }
+    wave_sync();
 #pragma unroll
     for (int c = 0; c < NQ; c++) {
         float s = 0.f;
 #pragma unroll
-        for (int l = 0; l < NQ; l++) s += arow[l] * Qip0[l * NQ + c];
+        for (int l = 0; l < NQ; l++) s += As[r * T::AS + l] * Qs[l * T::QS + c];
         phi[c] = s;
-        phi[NQ + c] = arow[NQ + c] * Qiv0[c];
+        phi[NQ + c] = As[r * T::AS + NQ + c] * Qiv0[c];
     }
     float brr[NU];
 #pragma unroll
-    for (int c = 0; c < NU; c++) brr[c] = Bm[r * NU + c] * Ri0[c];
+    for (int c = 0; c < NU; c++) brr[c] = Bs[r * T::BS + c] * Ri0[c];
     float th[NX];
     qinv_row<PL>(Qip1, Qiv1, r, th);
     float g = c1[r];
 #pragma unroll
     for (int m = 0; m < NX; m++) {
+        asm volatile("" ::: "memory");  // one row of A and B at a time: their LDS loads are not hoisted
         float s = th[m];
         g -= s * q1[m];
 #pragma unroll
-        for (int l = 0; l < NX; l++) s += phi[l] * A[m * NX + l];
+        for (int l = 0; l < NX; l++) s += phi[l] * As[m * T::AS + l];
 #pragma unroll
-        for (int l = 0; l < NU; l++) s += brr[l] * Bm[m * NU + l];
+        for (int l = 0; l < NU; l++) s += brr[l] * Bs[m * T::BS + l];
         th[m] = s;
         g += phi[m] * q0[m];
     }
@@ -352,6 +375,7 @@ __device__ __forceinline__ void schur_row(int lane, float rho, const float* A, c
 #pragma unroll
         for (int m = 0; m < NX; m++) Pd1[r * NX + m] = prow[m];
     }
+    wave_sync();
 }
 
 // Block row 0: S_00 = -Q~_0^-1, P_00 = -Q~_0, gamma_0 = c_0 - Q~_0^-1 q_0 (schur_linsys.cuh:164-210)
@@ -380,21 +404,30 @@ __device__ __forceinline__ void schur_row0(int lane, float rho, const float* Qp0
 }
 
 // ---------------------------------------------------------------- Schur step 3: P_{k+1,k} = -P_{k+1,k+1} phi_k P_{k,k}
+// P_{k,k} is staged in the knot group's tile; the product phi_k P_{k,k} then replaces it there.
 template <int PL>
 __device__ __forceinline__ void precond_offdiag(int lane, const float* phi, const float* prow, const float* Pd0,
-                                                float* Tsh, float* Po1) {
-    constexpr int NX = Plant<PL>::NX;
+                                                float* tile, float* Po1) {
+    constexpr int NX = Plant<PL>::NX, AS = SchurTile<PL>::AS;
+    const int r = lane < NX ? lane : 0;
+    if (lane < NX) {
+#pragma unroll
+        for (int c = 0; c < NX; c++) tile[r * AS + c] = Pd0[r * NX + c];
+    }
+    wave_sync();
     float t[NX];
 #pragma unroll
     for (int c = 0; c < NX; c++) {
+        asm volatile("" ::: "memory");
         float s = 0.f;
 #pragma unroll
-        for (int m = 0; m < NX; m++) s += phi[m] * Pd0[m * NX + c];
-        t[c] = s;
+        for (int m = 0; m < NX; m++) s += phi[m] * tile[m * AS + c];
+        t[c] = opaque(s);  // computed here, not sunk into the lane-predicated store below
     }
+    wave_sync();
     if (lane < NX) {
 #pragma unroll
-        for (int c = 0; c < NX; c++) Tsh[lane * NX + c] = t[c];
+        for (int c = 0; c < NX; c++) tile[lane * AS + c] = t[c];
     }
     wave_sync();
     if (lane < NX) {
@@ -402,7 +435,7 @@ __device__ __forceinline__ void precond_offdiag(int lane, const float* phi, cons
         for (int c = 0; c < NX; c++) {
             float s = 0.f;
 #pragma unroll
-            for (int m = 0; m < NX; m++) s += prow[m] * Tsh[m * NX + c];
+            for (int m = 0; m < NX; m++) s += prow[m] * tile[m * AS + c];
             Po1[lane * NX + c] = -s;
         }
     }

@@ -126,3 +126,23 @@ def mixed_figure8(B=1024, seed=2, dt=0.01, period=4.0, knots=(8, 32, 128)):
             out[(plant, int(N))] = Workload(plant, int(N), dt, xs, ref.reshape(len(idx), 6 * N),
                                             initial_guess(xs, int(N), nq), params)
     return out
+
+
+def scale_figure8(N=128, B=4096, seed=4, dt=0.01, period=4.0, plant="iiwa14"):
+    """Scale workload: B problems of horizon N, problem b tracking a figure-8 of seeded radius ~U(0.1, 0.25) m
+    and phase ~U(0, 2 pi); weights are set by the caller (the learning loop owns diag Q and diag R)."""
+    rng = np.random.default_rng(seed)
+    nq = NQ[plant]
+    radius = rng.uniform(0.1, 0.25, B)
+    phase = rng.uniform(0.0, 2 * np.pi, B)
+    q = NOMINAL_Q[plant][None, :] + rng.normal(0.0, 0.05, (B, nq))
+    qd = rng.normal(0.0, 0.05, (B, nq))
+    xs = np.concatenate([q, qd], 1)
+    t = np.arange(N) * dt
+    ref = np.zeros((B, N, 6))
+    for b in range(B):
+        ref[b, :, :3] = figure8(CENTRE[plant], radius[b], period, t, phase[b])
+    params = dict(dt=dt, max_sqp_iters=10, kkt_tol=1e-4, max_pcg_iters=100, pcg_tol=1e-5, solve_ratio=1.0, mu=10.0,
+                  q_cost=10.0, qd_cost=1e-3, u_cost=1e-3, N_cost=10.0, q_lim_cost=0.0, vel_lim_cost=0.0,
+                  ctrl_lim_cost=0.0, rho=1e-2)
+    return Workload(plant, N, dt, xs, ref.reshape(B, 6 * N), initial_guess(xs, N, nq), params)

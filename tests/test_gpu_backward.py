@@ -34,9 +34,11 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
 
 
-@pytest.mark.parametrize("plant,hess,term", [("iiwa14", 0, 0), ("iiwa14", 1, 1), ("indy7", 0, 0)])
-def test_backward_matches_oracle_step_gradient(gpu, O, plant, hess, term):
-    N, B = 16, 4
+@pytest.mark.parametrize("plant,hess,term,N", [("iiwa14", 0, 0, 16), ("iiwa14", 1, 1, 16), ("indy7", 0, 0, 16),
+                                                ("iiwa14", 0, 0, 32), ("iiwa14", 0, 0, 128)])
+def test_backward_matches_oracle_step_gradient(gpu, O, plant, hess, term, N):
+    # N = 32: the differentiation configuration's horizon; N = 128: the scale configuration's
+    B = 4 if N <= 32 else 2
     s, wl, Qd, Rd, kw = _problem(plant, N, B, seed=21, hess_mode=hess, terminal_mode=term)
     dev = s.device
     xu = torch.tensor(wl.xu, dtype=torch.float32, device=dev)

