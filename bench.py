@@ -93,6 +93,25 @@ def time_gpu(solver, wl, steps, warmup):
     return (t1 - t0), kern, st
 
 
+def time_dropin_numpy(solver, wl, steps):
+    """The drop-in call as the reference's callers make it: numpy in, numpy out (host<->device copies of the
+    trajectory, initial states and references, and of the statistics, included)."""
+    import numpy as np
+
+    xu = wl.xu.astype(np.float32)
+    xs = wl.xs.astype(np.float32)
+    ref = wl.ref.astype(np.float32)
+    solver.reset_dual()
+    solver.reset_rho()
+    solver.solve(xu, wl.dt, xs, ref)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        solver.reset_dual()
+        solver.reset_rho()
+        solver.solve(xu, wl.dt, xs, ref)
+    return time.perf_counter() - t0
+
+
 def time_fwd_bwd(solver, wl, steps, warmup):
     """Forward solve plus the adjoint backward pass for a fixed loss gradient dL/dXU (seeded)."""
     import numpy as np
@@ -359,6 +378,9 @@ def main():
         "mean_pcg_iters": float(st["pcg_iters"].sum() / max(1, st["sqp_iters"].sum())),
         "fwd_bwd_solves_per_s": None,
     }
+    if args.steps > 0:
+        de = time_dropin_numpy(solver, wl, 3)
+        out["dropin_numpy_solves_per_s"] = B * 3 / de
     if args.fwd_bwd_steps > 0:
         fe, g = time_fwd_bwd(solver, wl, args.fwd_bwd_steps, 1)
         out["fwd_bwd_solves_per_s"] = B * args.fwd_bwd_steps / fe

@@ -490,28 +490,37 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_dz(Args a, int it) {
 }
 
 // host-side convergence bookkeeping of bsqp.cuh:142-165, per problem, plus the converged count
-__global__ void k_check(Args a, int it) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= a.B || stopped(a)) return;
-    int conv = a.conv[b];
-    if (a.exit_rule == GATO_EXIT_PCG0) {
-        if (a.st_pcg[it * a.B + b] == 0) conv = 1;
-        a.st_sqp[b] += 1;  // quirk Q10: batch-uniform count
-    } else if (a.active[b]) {
-        a.st_sqp[b] += 1;
-        if (a.st_kres[it * a.B + b] <= a.kkt_tol) { conv = 1; a.active[b] = 0; }
+// Host-side convergence bookkeeping of bsqp.cuh:142-165 for every problem (one work-group looping over the
+// batch), the converged count, and in graph mode the batch-level exit test (bsqp.cuh:165-167) that raises the
+// device stop flag.
+__global__ void __launch_bounds__(1024) k_check(Args a, int it) {
+    __shared__ int count;
+    if (stopped(a)) return;
+    if (threadIdx.x == 0) count = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int b = threadIdx.x; b < a.B; b += blockDim.x) {
+        int conv = a.conv[b];
+        if (a.exit_rule == GATO_EXIT_PCG0) {
+            if (a.st_pcg[it * a.B + b] == 0) conv = 1;
+            a.st_sqp[b] += 1;  // quirk Q10: batch-uniform count
+        } else if (a.active[b]) {
+            a.st_sqp[b] += 1;
+            if (a.st_kres[it * a.B + b] <= a.kkt_tol) { conv = 1; a.active[b] = 0; }
+        }
+        if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) {  // frozen: no line search this iteration
+            a.st_step[it * a.B + b] = 0.f;
+            a.st_lsm[it * a.B + b] = a.phi[b];
+        }
+        a.conv[b] = conv;
+        mine += conv;
     }
-    if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) {  // frozen: no line search this iteration
-        a.st_step[it * a.B + b] = 0.f;
-        a.st_lsm[it * a.B + b] = a.phi[b];
+    if (mine) atomicAdd(&count, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.solved[it] = count;
+        if (a.stop_w && (double)count >= (double)a.B * (double)a.solve_ratio) *a.stop_w = 1;
     }
-    a.conv[b] = conv;
-    if (conv) atomicAdd(&a.solved[it], 1);
-}
-
-// graph mode: the host loop's exit test (bsqp.cuh:165-167) on the device, after the iteration's check
-__global__ void k_decide(Args a, int it) {
-    if (threadIdx.x == 0 && !*a.stop && (double)a.solved[it] >= (double)a.B * (double)a.solve_ratio) *a.stop_w = 1;
 }
 
 __global__ void k_begin(Args a) {
@@ -984,6 +993,7 @@ bool valid_groups(int N, int G) {
     return (np == 1 || np == 2 || np == 4 || np == 8) && G <= default_groups(N);
 }
 int grid_threads(const gato_solver* s) { return gato::KG * s->G; }
+int check_threads(int B) { return std::min(1024, 64 * ((B + 63) / 64)); }
 int npass(const gato_solver* s) { return (s->N + s->G - 1) / s->G; }
 
 template <int PL>
@@ -1044,7 +1054,7 @@ void fill_args(gato_solver* s, float* d_xu, const float* d_xs, const float* d_re
     a.stop_w = nullptr;
 }
 
-// Every SQP iteration of the plain path enqueued without a host round trip: k_decide raises the device
+// Every SQP iteration of the plain path enqueued without a host round trip: k_check raises the device
 // stop flag when the batch-level exit fires and the remaining launches return at once.
 template <int PL>
 void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
@@ -1063,8 +1073,7 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
         hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, a);
         launch_pcg<PL>(np, B, blk, lds, st, a, it);
         hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, a, it);
-        hipLaunchKernelGGL(gato::k_check, dim3(sb), dim3(256), 0, st, a, it);
-        hipLaunchKernelGGL(gato::k_decide, dim3(1), dim3(64), 0, st, a, it);
+        hipLaunchKernelGGL(gato::k_check, dim3(1), dim3(check_threads(B)), 0, st, a, it);
         hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, gato::NUM_ALPHAS, 2, it);
     }
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
@@ -1149,7 +1158,7 @@ int launch_solve(gato_solver* s, hipStream_t st) {
         launch_pcg<PL>(np, B, blk, lds, st, a, it);
         HIP_TRY(hipEventRecord(s->ev1, st));
         hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, a, it);
-        hipLaunchKernelGGL(gato::k_check, dim3(sb), dim3(256), 0, st, a, it);
+        hipLaunchKernelGGL(gato::k_check, dim3(1), dim3(check_threads(B)), 0, st, a, it);
         HIP_TRY(hipGetLastError());
         int solved = 0;
         HIP_TRY(hipMemcpyAsync(&solved, a.solved + it, sizeof(int), hipMemcpyDeviceToHost, st));
