@@ -443,48 +443,6 @@ __device__ __forceinline__ void rnea_publish(const Kin<PL>& K0, const float* qd0
     });
 }
 
-// Tangent of ID along q_j (dir 0) or qd_j (dir 1), qdd fixed, reading v_i, a_i and the joint-j force
-// from the shared RNEA table (same equations as rnea_tangent below).
-template <int PL>
-__device__ __forceinline__ void rnea_tangent_tab(const Kin<PL>& K, const float* qd, const float* qdd,
-                                                 const float* tab, int j, int dir, float* dtau) {
-    constexpr int NQ = Plant<PL>::NQ;
-    using T = RneaTab<PL>;
-    const V6 fj = T::get(tab, 2 * NQ + j);
-    V6 dv = v6zero(), da = v6zero();
-    V6 df[NQ];
-    Unroll<0, NQ>::run([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        __builtin_amdgcn_sched_barrier(0);  // one link at a time: table loads are not hoisted
-        const V6 v = T::get(tab, i);
-        const V6 a = T::get(tab, NQ + i);
-        V6 xv = v;  // X_i v_{i-1}
-        xv.w2 -= qd[i];
-        V6 xa = a - crm_s(v, qd[i]);  // X_i a_{i-1}
-        xa.w2 -= qdd[i];
-        dv = xmot<PL, i>(K.s[i], K.c[i], dv);
-        da = xmot<PL, i>(K.s[i], K.c[i], da);
-        const bool hit = (i == j);
-        if (hit && dir == 0) {
-            dv = dv - s_crm(xv);
-            da = da - s_crm(xa);
-        }
-        if (hit && dir == 1) dv.w2 += 1.f;
-        da = da + crm_s(dv, qd[i]);
-        if (hit && dir == 1) da = da + V6{v.w1, -v.w0, 0.f, v.l1, -v.l0, 0.f};  // v_i x S
-        df[i] = imul<PL, i>(da) + crf(dv, imul<PL, i>(v)) + crf(v, imul<PL, i>(dv));
-    });
-    UnrollDown<0, NQ>::run([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        dtau[i] = df[i].w2;
-        if constexpr (i > 0) {
-            V6 g = df[i];
-            if (i == j && dir == 0) g = g + s_crf(fj);
-            df[i - 1] = df[i - 1] + xfrc<PL, i>(K.s[i], K.c[i], g);
-        }
-    });
-}
-
 // Joint axes carried into every later link frame: S_i^(k) = X_k ... X_{i+1} S_i for i <= k, stored at
 // slot k(k+1)/2 + i of an 8-float-stride table (identical in every lane of a knot group).
 template <int PL>
@@ -507,8 +465,8 @@ __device__ __forceinline__ float dot6(const V6& m, const V6& f) {
 
 // Tangent of ID along q_j (dir 0) or qd_j (dir 1), qdd fixed, as a single forward sweep:
 //   d tau_i = sum_{k >= i} (S_i^(k))^T g_k,  g_k = dI_k-force of link k  (+ S x* F_j at k = j for dir 0),
-// which equals the backward accumulation dF_{k-1} += X_k^T (dF_k + [k = j] S x* F_j) of rnea_tangent
-// (the extra term contributes (S_j)^T (S x* F_j) = 0 to d tau_j itself).
+// which equals the backward force accumulation of the differentiated RNEA, dF_{k-1} += X_k^T (dF_k + [k = j]
+// S x* F_j), unrolled (the extra term contributes (S_j)^T (S x* F_j) = 0 to d tau_j itself).
 template <int PL>
 __device__ __forceinline__ void rnea_tangent_ax(const Kin<PL>& K, const float* qd, const float* qdd,
                                                 const float* tab0, const float* ax0, int j, int dir, float* dtau) {
@@ -547,79 +505,6 @@ __device__ __forceinline__ void rnea_tangent_ax(const Kin<PL>& K, const float* q
             constexpr int i = decltype(ic)::value;
             dtau[i] += dot6(T::get(ax, tri(k, i)), g);
         });
-    });
-}
-
-// Force transmitted across joint j by the RNEA at (q, qd, qdd) (f_ext included), for run-time j.
-template <int PL>
-__device__ __forceinline__ V6 joint_force(const Kin<PL>& K, const float* qd, const float* qdd, const float* fext,
-                                          int j) {
-    constexpr int NQ = Plant<PL>::NQ;
-    V6 f[NQ];
-    V6 v = v6zero(), a = V6{0.f, 0.f, 0.f, 0.f, 0.f, kGravity};
-    Unroll<0, NQ>::run([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        v = xmot<PL, i>(K.s[i], K.c[i], v);
-        a = xmot<PL, i>(K.s[i], K.c[i], a);
-        v.w2 += qd[i];
-        a = a + crm_s(v, qd[i]);
-        a.w2 += qdd[i];
-        f[i] = imul<PL, i>(a) + crf(v, imul<PL, i>(v));
-    });
-    if (fext) {
-        f[NQ - 1].w0 -= fext[0]; f[NQ - 1].w1 -= fext[1]; f[NQ - 1].w2 -= fext[2];
-        f[NQ - 1].l0 -= fext[3]; f[NQ - 1].l1 -= fext[4]; f[NQ - 1].l2 -= fext[5];
-    }
-    V6 fj = v6zero();
-    UnrollDown<0, NQ>::run([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        if (i == j) fj = f[i];
-        if constexpr (i > 0) f[i - 1] = f[i - 1] + xfrc<PL, i>(K.s[i], K.c[i], f[i]);
-    });
-    return fj;
-}
-
-// Tangent of ID(q, qd, qdd) along q_j (dir = 0) or qd_j (dir = 1) with qdd held fixed:
-//   dX_j/dq_j = -crm(S) X_j, so  dv_j += -S x (X_j v_{j-1}), da_j += -S x (X_j a_{j-1}),
-//   and the transmitted force picks up X_j^T (S x* f_j) on its way to the parent.
-// The per-link v, a are recomputed alongside; fj is joint_force(..., j). The column j is a
-// run-time value: every link is visited with predicated terms, so the lanes of a wave that own
-// different columns still follow one instruction stream.
-template <int PL>
-__device__ __forceinline__ void rnea_tangent(const Kin<PL>& K, const float* qd, const float* qdd, const V6& fj,
-                                             int j, int dir, float* dtau) {
-    constexpr int NQ = Plant<PL>::NQ;
-    V6 v = v6zero(), a = V6{0.f, 0.f, 0.f, 0.f, 0.f, kGravity};
-    V6 dv = v6zero(), da = v6zero();
-    V6 df[NQ];
-    Unroll<0, NQ>::run([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        const V6 xv = xmot<PL, i>(K.s[i], K.c[i], v);  // X_i v_{i-1}
-        const V6 xa = xmot<PL, i>(K.s[i], K.c[i], a);  // X_i a_{i-1}
-        dv = xmot<PL, i>(K.s[i], K.c[i], dv);
-        da = xmot<PL, i>(K.s[i], K.c[i], da);
-        v = xv;
-        v.w2 += qd[i];
-        a = xa + crm_s(v, qd[i]);
-        a.w2 += qdd[i];
-        const bool hit = (i == j);
-        if (hit && dir == 0) {
-            dv = dv - s_crm(xv);
-            da = da - s_crm(xa);
-        }
-        if (hit && dir == 1) dv.w2 += 1.f;
-        da = da + crm_s(dv, qd[i]);
-        if (hit && dir == 1) da = da + V6{v.w1, -v.w0, 0.f, v.l1, -v.l0, 0.f};  // v_i x S
-        df[i] = imul<PL, i>(da) + crf(dv, imul<PL, i>(v)) + crf(v, imul<PL, i>(dv));
-    });
-    UnrollDown<0, NQ>::run([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        dtau[i] = df[i].w2;
-        if constexpr (i > 0) {
-            V6 g = df[i];
-            if (i == j && dir == 0) g = g + s_crf(fj);
-            df[i - 1] = df[i - 1] + xfrc<PL, i>(K.s[i], K.c[i], g);
-        }
     });
 }
 
