@@ -132,3 +132,36 @@ def test_schur_pcg_dz_merit_match_oracle_on_gpu_inputs(gpu, O, plant):
         ms = np.array([O.merit(cfg, xu[b], dz[b].astype(np.float64), 1.0 / (1 << j), xs32[b], ref32[b],
                                wl.params["mu"]) for j in range(8)])
         assert rel_err(mp[b].sum(1), ms) < 1e-5
+
+
+@pytest.mark.parametrize("plant,iters,tol", [("iiwa14", 5, 1e-4), ("iiwa14", 20, 1e-3), ("indy7", 5, 1e-4)])
+def test_pcg_fixed_iteration_count_matches_oracle(gpu, O, plant, iters, tol):
+    # With the relative tolerance at 0 both PCGs run exactly `iters` iterations from lambda = 0 on the GPU's
+    # own S, P and gamma, so their iterates are compared without the stopping rule's sensitivity.
+    from gato_amd.solver import Solver
+
+    N, B = 32, 4
+    nq = W.NQ[plant]
+    nx = 2 * nq
+    s0, wl, xu = _solver_and_inputs(plant, N, B)
+    p = wl.params
+    kw = {k: p[k] for k in ("dt", "max_sqp_iters", "kkt_tol", "solve_ratio", "mu", "q_cost", "qd_cost", "u_cost",
+                            "N_cost", "q_lim_cost", "vel_lim_cost", "ctrl_lim_cost", "rho")}
+    s = Solver(plant, N, B, mode="plain", max_pcg_iters=iters, pcg_tol=0.0, **kw)
+    s.debug_phase(0, xu, wl.xs, wl.ref)
+    s.debug_phase(1, xu, wl.xs, wl.ref)
+    Sd = s.debug_array("Sd").reshape(B, N, nx, nx)
+    So = s.debug_array("So").reshape(B, N, nx, nx)
+    Pd = s.debug_array("Pd").reshape(B, N, nx, nx)
+    Po = s.debug_array("Po").reshape(B, N, nx, nx)
+    g = s.debug_array("gamma").reshape(B, N, nx)
+    s.reset_dual()
+    s.debug_phase(2, xu, wl.xs, wl.ref)
+    lam = s.debug_array("lambda").reshape(B, N * nx)
+    pit = s.debug_array("pcg_iters")[:B]
+    for b in range(B):
+        gs = dict(Sd=Sd[b].astype(np.float64), So=So[b].astype(np.float64), Pd=Pd[b].astype(np.float64),
+                  Po=Po[b].astype(np.float64), gamma=g[b].astype(np.float64).reshape(-1))
+        l64, it = O.pcg(nx, N, gs, np.zeros(N * nx), 0.0, iters)
+        assert it == iters and int(pit[b]) == iters
+        assert rel_err(lam[b], l64) < tol
