@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--scale-steps", type=int, default=5,
                     help="Adam steps of the scale workload (iiwa14 N=128 B=4096, shared diag Q/R learned through "
                          "the backward pass) timed (0: skip)")
+    ap.add_argument("--ref-mpc-steps", type=int, default=40,
+                    help="MPC steps of the reference's published configuration (indy7 N=32 B=512, 1 SQP "
+                         "iteration per solve) timed (0: skip)")
     ap.add_argument("--mixed-steps", type=int, default=3,
                     help="steps of the uneven batch (1024 problems, both plants, N in {8, 32, 128}) timed (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -242,6 +245,63 @@ def time_scale(steps):
             "compulsory_hbm_GBs": 4 * (nx + 6 * wl.N + 2 * s.traj + 2 * nx * wl.N + 7) * B * steps / el / 1e9}
 
 
+REFERENCE_HEATMAP_MS = 2.84  # indy7 N=32 B=512, plots/fig8_benchmark_heatmap.ipynb (GPU not stated)
+
+
+def time_reference_mpc(steps):
+    """The reference's published benchmark point: figure-8 MPC of indy7 with N=32, B=512 force hypotheses and
+    its default MPC settings (one SQP iteration, <= 200 PCG iterations, pcg_tol 1e-4, warm start from the
+    previous solution). Between solves the plant advances one dt with the first control of hypothesis 0 (zero
+    wrench) through the library's own forward step; every problem restarts from the chosen solution with
+    x_0 = current state, as the reference's MPC loop does (mpc_controller.py:186-262). Reported: the solve's
+    wall clock as the module reports it (sqp_time_us, the reference's metric) and the device time."""
+    import numpy as np
+    import torch
+
+    from gato_amd import workloads as W
+    from gato_amd.solver import Solver
+
+    N, B, dt = 32, 512, 0.01
+    prm = dict(W.REFERENCE_MPC_PARAMS)
+    s = Solver("indy7", N, B, mode="graph", dt=dt, **prm)
+    nx, nu = s.nx, s.nu
+    rng = np.random.default_rng(6)
+    fext = rng.normal(0.0, 5.0 / np.sqrt(3), (B, 6)).astype(np.float32)
+    fext[0] = 0.0
+    fext[:, 3:] = 0.0
+    s.set_f_ext_batch(fext)
+    path = W.reference_figure8_path(dt)
+    x = np.concatenate([W.INDY7_READY, np.zeros(6)]).astype(np.float32)
+    xu = np.zeros((B, s.traj), np.float32)
+    for k in range(N):
+        xu[:, k * (nx + nu): k * (nx + nu) + nx] = x
+    dev = s.device
+    t = lambda a: torch.tensor(a, dtype=torch.float32, device=dev).contiguous()
+    xu_d = t(xu)
+    s.reset_dual()
+    wall, devt, pcg = [], [], []
+    for step in range(steps + 3):
+        ref = np.tile(path[6 * step: 6 * (step + N)], (B, 1)).astype(np.float32)
+        xs_d = t(np.tile(x, (B, 1)))
+        xu_d[:, :nx] = xs_d
+        s.reset_rho()
+        st = s.solve_device(xu_d, xs_d, t(ref), stats=True)
+        if step >= 3:
+            wall.append(st["sqp_time_us"])
+            devt.append(s.last_kernel_time()[0])
+            pcg.append(float(st["pcg_iters"][0].mean()))
+        best = xu_d[0].clone()
+        u0 = best[nx:nx + nu].cpu().numpy()
+        x = s.sim_forward(x, u0, dt)[0].astype(np.float32)
+        xu_d[:] = best
+    ms_wall = float(np.mean(wall)) / 1e3
+    return {"workload": "indy7 N=32 B=512 figure-8 MPC (reference's published point), 1 SQP iteration per solve",
+            "solves": steps, "ms_per_solve_wall": ms_wall, "ms_per_solve_device": float(np.mean(devt)) / 1e3,
+            "mean_pcg_iters": float(np.mean(pcg)), "reference_ms": REFERENCE_HEATMAP_MS,
+            "speedup_vs_reference": REFERENCE_HEATMAP_MS / ms_wall,
+            "note": "reference hardware not stated; same configuration, different GPU"}
+
+
 def algorithmic_bytes(wl):
     """Compulsory HBM bytes of one batched solve: x_s, reference, XU in and out, the warm-start
     multipliers in and out, and the per-problem scalars (rho, drho, mu, tol, merit, stats)."""
@@ -386,6 +446,8 @@ def main():
         out["fwd_bwd_solves_per_s"] = B * args.fwd_bwd_steps / fe
         out["fwd_bwd_ms_per_step"] = 1e3 * fe / args.fwd_bwd_steps
         out["adjoint_pcg_iters_mean"] = float(g["adj_pcg_iters"].mean())
+    if args.ref_mpc_steps > 0:
+        out["reference_heatmap_point"] = time_reference_mpc(args.ref_mpc_steps)
     if args.mixed_steps > 0:
         out["mixed_batch"] = time_mixed(args.mixed_steps)
     if args.scale_steps > 0:

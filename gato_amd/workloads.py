@@ -146,3 +146,25 @@ def scale_figure8(N=128, B=4096, seed=4, dt=0.01, period=4.0, plant="iiwa14"):
                   q_cost=10.0, qd_cost=1e-3, u_cost=1e-3, N_cost=10.0, q_lim_cost=0.0, vel_lim_cost=0.0,
                   ctrl_lim_cost=0.0, rho=1e-2)
     return Workload(plant, N, dt, xs, ref.reshape(B, 6 * N), initial_guess(xs, N, nq), params)
+
+
+# The reference's published solve-time heatmap: indy7 figure-8 MPC with its default MPC solver settings
+# (python/bsqp/config.py:35-50) from the "ready" configuration (config.py:22-26).
+REFERENCE_MPC_PARAMS = dict(max_sqp_iters=1, kkt_tol=0.001, max_pcg_iters=200, pcg_tol=1e-4, solve_ratio=1.0,
+                            mu=10.0, q_cost=2.0, qd_cost=1e-2, u_cost=2e-6, N_cost=50.0, q_lim_cost=0.01,
+                            vel_lim_cost=0.0, ctrl_lim_cost=0.0, rho=0.01)
+INDY7_READY = np.array([-1.096711, -0.09903229, 0.83125766, -0.10907673, 0.49704404, 0.01499449])
+
+
+def reference_figure8_path(dt, A_x=0.4, A_z=0.4, offset=(0.0, 0.5, 0.6), period=6.0, cycles=5, theta=np.pi / 4):
+    """End-effector path of the reference's figure-8 benchmark (python/bsqp/common.py:10-46): the curve
+    (A_x sin t, 0, A_z sin(2t)/2 + A_z/2) + offset rotated by theta about z, sampled at period/dt points of
+    t in [0, 2 pi] and repeated `cycles` times; 6 floats per point (xyz, zero rpy)."""
+    t = np.linspace(0.0, 2 * np.pi, int(period / dt))
+    p = np.stack([offset[0] + A_x * np.sin(t), np.full_like(t, offset[1]),
+                  offset[2] + A_z * np.sin(2 * t) / 2 + A_z / 2], 1)
+    c, s_ = np.cos(theta), np.sin(theta)
+    R = np.array([[c, -s_, 0.0], [s_, c, 0.0], [0.0, 0.0, 1.0]])
+    pts = np.zeros((len(t), 6))
+    pts[:, :3] = p @ R.T
+    return np.tile(pts.reshape(-1), int(cycles))
