@@ -17,11 +17,12 @@ from gato_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-def _solver_and_inputs(plant, N, B, seed=5):
+def _solver_and_inputs(plant, N, B, seed=5, barriers=0.0):
     from gato_amd.solver import Solver
 
     wl = W.drop_in_figure8(plant, N=N, B=B)
     p = wl.params
+    p.update(q_lim_cost=barriers, vel_lim_cost=barriers, ctrl_lim_cost=barriers)
     rng = np.random.default_rng(seed)
     nx = 2 * W.NQ[plant]
     xu = wl.xu + rng.normal(0, 0.02, wl.xu.shape)
@@ -37,7 +38,7 @@ def _cfg(O, plant, wl):
     p = wl.params
     return O.make_cfg(plant, N=wl.N, dt=p["dt"], max_sqp_iters=p["max_sqp_iters"], max_pcg_iters=p["max_pcg_iters"],
                       q_cost=p["q_cost"], qd_cost=p["qd_cost"], u_cost=p["u_cost"], N_cost=p["N_cost"],
-                      q_lim_cost=p["q_lim_cost"])
+                      q_lim_cost=p["q_lim_cost"], vel_lim_cost=p["vel_lim_cost"], ctrl_lim_cost=p["ctrl_lim_cost"])
 
 
 def _gpu_kkt(s, B, N, nq):
@@ -53,11 +54,13 @@ def _gpu_kkt(s, B, N, nq):
     return A, Bm, c, Qp, Qv, Rg, q, r
 
 
-@pytest.mark.parametrize("plant", ["iiwa14", "indy7"])
-def test_kkt_setup_matches_oracle(gpu, O, plant):
+@pytest.mark.parametrize("plant,barriers", [("iiwa14", 0.0), ("indy7", 0.0), ("iiwa14", 0.01), ("indy7", 0.01)])
+def test_kkt_setup_matches_oracle(gpu, O, plant, barriers):
+    # barriers > 0: joint-limit barrier terms on positions, velocities and torques (the reference's MPC
+    # settings use q_lim_cost = 0.01), i.e. iiwa14's sign-preserving clamps and indy7's outer products
     N, B = 32, 4
     nq = W.NQ[plant]
-    s, wl, xu = _solver_and_inputs(plant, N, B)
+    s, wl, xu = _solver_and_inputs(plant, N, B, barriers=barriers)
     cfg = _cfg(O, plant, wl)
     s.debug_phase(0, xu, wl.xs, wl.ref)
     A, Bm, c, Qp, Qv, Rg, q, r = _gpu_kkt(s, B, N, nq)
@@ -86,12 +89,12 @@ def _kkt_dict(A, Bm, c, Qp, Qv, Rg, q, r, b, nq):
                 r=r[b].astype(np.float64), c=c[b].astype(np.float64))
 
 
-@pytest.mark.parametrize("plant", ["iiwa14", "indy7"])
-def test_schur_pcg_dz_merit_match_oracle_on_gpu_inputs(gpu, O, plant):
+@pytest.mark.parametrize("plant,barriers", [("iiwa14", 0.0), ("indy7", 0.0), ("iiwa14", 0.01), ("indy7", 0.01)])
+def test_schur_pcg_dz_merit_match_oracle_on_gpu_inputs(gpu, O, plant, barriers):
     N, B = 32, 4
     nq = W.NQ[plant]
     nx, nu = 2 * nq, nq
-    s, wl, xu = _solver_and_inputs(plant, N, B)
+    s, wl, xu = _solver_and_inputs(plant, N, B, barriers=barriers)
     cfg = _cfg(O, plant, wl)
     s.debug_phase(0, xu, wl.xs, wl.ref)
     kk = _gpu_kkt(s, B, N, nq)
