@@ -246,16 +246,19 @@ __device__ __forceinline__ int phase_pcg(const Args& a, int b, float eps, float*
     return it;
 }
 
-// Sum over the work-group with one barrier: red is double-buffered by the caller (alternate
-// halves between consecutive calls), so no barrier is needed before the partial sums are written.
+// Sum over the work-group with one barrier: each 16-lane row is summed on the DPP network and its lane 0
+// writes one slot (at most 32 rows); after the barrier lanes 0..31 of every wave pick up the slots and two
+// more row sums plus one add give the total, identical in every lane. red (32 floats) is double-buffered by
+// the caller (alternate halves between consecutive calls), so no barrier is needed before the writes.
 __device__ __forceinline__ float block_sum1(float v, float* red) {
-    const int nw = (blockDim.x + 63) >> 6;
-    v = wave_sum(v);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    const int rows = blockDim.x >> 4;
+    v = row16_sum(v);
+    if ((threadIdx.x & 15) == 0) red[threadIdx.x >> 4] = v;
     __syncthreads();
-    float s = 0.f;
-    for (int i = 0; i < nw; i++) s += red[i];
-    return s;
+    const int l = threadIdx.x & 63;
+    float s = l < rows ? red[l] : 0.f;
+    s = row16_sum(s);
+    return lane_f(s, 0) + lane_f(s, 16);
 }
 
 // Same iteration as phase_pcg with the lane's rows of S and P held in registers for the whole
@@ -268,7 +271,7 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
     const int N = a.N, tid = threadIdx.x, k = tid / KG, lane = tid % KG;
     float* sp = sm;
     float* sr = sp + (N + 2) * XP;
-    float* red = sr + (N + 2) * XP;  // 4 x 16 floats: two double-buffered reduction slots
+    float* red = sr + (N + 2) * XP;  // 2 x 32 floats: two double-buffered reduction buffers
     for (int i = tid; i < 2 * (N + 2) * XP; i += blockDim.x) sm[i] = 0.f;
     const bool val = (lane < NX) && (k < N);
     const int l = lane < NX ? lane : 0;
@@ -321,7 +324,7 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
     float z = val ? mv(pd, pl, pu, sr) : 0.f;
     float p = z;
     int slot = 0;
-    float rho = block_sum1(r * z, red + 16 * slot);
+    float rho = block_sum1(r * z, red + 32 * slot);
     slot ^= 1;
     if (fabsf(rho) < a.pcg_abs_tol) return 0;
     const float rho0 = fabsf(rho);
@@ -331,17 +334,18 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
         if (val) sp[(kk + 1) * XP + l] = p;
         __syncthreads();
         const float Ap = val ? mv(sd, sl, su, sp) : 0.f;
-        const float alpha = rho / block_sum1(p * Ap, red + 16 * slot);
+        // quotients by reciprocal and product, as the reference's fast-math build divides (-use_fast_math)
+        const float alpha = rho * __builtin_amdgcn_rcpf(block_sum1(p * Ap, red + 32 * slot));
         slot ^= 1;
         x += alpha * p;
         r -= alpha * Ap;
         if (val) sr[(kk + 1) * XP + l] = r;
         __syncthreads();
         z = val ? mv(pd, pl, pu, sr) : 0.f;
-        const float rn = block_sum1(r * z, red + 16 * slot);
+        const float rn = block_sum1(r * z, red + 32 * slot);
         slot ^= 1;
         if (fabsf(rn) < a.pcg_abs_tol + eps * rho0) break;
-        const float beta = rn / rho;
+        const float beta = rn * __builtin_amdgcn_rcpf(rho);
         rho = rn;
         p = z + beta * p;
     }
@@ -1017,7 +1021,7 @@ size_t lds_bytes(const gato_solver* s) {
     size_t f = 0;
     f = std::max<size_t>(f, (size_t)G * (24 * nq + 9 * (nq * (nq + 1) / 2)));  // lin
     f = std::max<size_t>(f, (size_t)G * (nx * 16 + nx * 8 + nq * 8 + 2 * nx));  // schur (SchurTile)
-    f = std::max<size_t>(f, (size_t)2 * (s->N + 2) * 16 + 32);      // pcg
+    f = std::max<size_t>(f, (size_t)2 * (s->N + 2) * 16 + 64);      // pcg
     f = std::max<size_t>(f, (size_t)G * gato::KG + 16);             // dz
     f = std::max<size_t>(f, (size_t)2 * gato::NUM_ALPHAS + 16);     // merit + ls
     return f * sizeof(float);
