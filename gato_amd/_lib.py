@@ -55,6 +55,7 @@ SIGNATURES = {
     "gato_last_kernel_time": (C.c_int, [_VP, C.POINTER(C.c_double), _ip]),
     "gato_backward": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _VP, _VP, _VP, _VP, _VP, _ip]),
     "gato_set_backward_params": (C.c_int, [_VP, C.c_int32, C.c_float, C.c_int32]),
+    "gato_select_hypothesis": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _VP, _ip, _VP]),
     "gato_set_launch_shape": (C.c_int, [_VP, C.c_int32]),
     "gato_get_launch_shape": (C.c_int, [_VP, _ip, _ip]),
     "gato_autotune": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _ip]),

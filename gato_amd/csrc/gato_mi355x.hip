@@ -830,6 +830,63 @@ __global__ void k_bwd_begin(Args a, const float* lam_keep, float* lam_b, float* 
     }
 }
 
+
+// ------------------------------------------------------------------ MPC hypothesis selection (no reference kernel)
+// The MPC caller's per-step host work (mpc_controller.py:268-282): step every wrench hypothesis b from
+// (x_last, u_last) with its f_ext_b, err_b = |x_next_b - x_curr|_2, best = argmin (first on ties, as numpy's
+// argmin), then every problem's warm start becomes problem best's trajectory (mpc_controller.py:251-253).
+template <int PL>
+__global__ void __launch_bounds__(1024) k_select(const float* x, const float* u, const float* xcur,
+                                                 const float* fext, float dt, int B, float* err, int* best) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ;
+    __shared__ float se[1024];
+    __shared__ int si[1024];
+    float emin = 3.4e38f;
+    int imin = B;
+    for (int b = threadIdx.x; b < B; b += blockDim.x) {
+        float q[NQ], qd[NQ], uu[NQ], qdd[NQ], L[NQ * (NQ + 1) / 2];
+#pragma unroll
+        for (int i = 0; i < NQ; i++) { q[i] = x[i]; qd[i] = x[NQ + i]; uu[i] = u[i]; }
+        Kin<PL> K;
+        K.set(q);
+        forward_dynamics<PL>(K, qd, uu, fext + 6 * b, qdd, L);
+        float e = 0.f;
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+            const float dq = q[i] + dt * qd[i] + 0.5f * qdd[i] * dt * dt - xcur[i];
+            const float dv = qd[i] + dt * qdd[i] - xcur[NQ + i];
+            e += dq * dq + dv * dv;
+        }
+        e = sqrtf(e);
+        if (err) err[b] = e;
+        if (e < emin) { emin = e; imin = b; }
+    }
+    se[threadIdx.x] = emin;
+    si[threadIdx.x] = imin;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            const float e2 = se[threadIdx.x + h];
+            const int i2 = si[threadIdx.x + h];
+            if (e2 < se[threadIdx.x] || (e2 == se[threadIdx.x] && i2 < si[threadIdx.x])) {
+                se[threadIdx.x] = e2;
+                si[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *best = si[0];
+}
+
+__global__ void k_broadcast_row(float* xu, int traj, int B, const int* best) {
+    const int src = *best;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)B * traj; i += (size_t)gridDim.x * blockDim.x) {
+        const int b = (int)(i / traj);
+        if (b != src) xu[i] = xu[(size_t)src * traj + i % traj];
+    }
+}
+
 // ------------------------------------------------------------------ sim_forward (sim.cuh:14-48)
 template <int PL>
 __global__ void k_sim(float* xn, const float* x, const float* u, const float* fext, float dt, int B) {
@@ -966,7 +1023,7 @@ size_t layout(gato_solver* s, bool assign) {
     t = F(it * B); if (assign) a.st_lsm = t;
     t = F(it * B); if (assign) a.st_step = t;
     ip = I(B); if (assign) a.st_sqp = ip;
-    ip = I(1); if (assign) s->d_stop = ip;
+    ip = I(2); if (assign) s->d_stop = ip;  // [0] graph-mode stop flag, [1] selected hypothesis
     t = F(B * N * nx); if (assign) s->lam_save = t;
     t = F(B * s->traj); if (assign) s->xu_tune = t;
     t = F(B * N * nx); if (assign) s->lam_b = t;
@@ -1536,6 +1593,27 @@ int gato_sim_forward(gato_solver* s, float* d_xnext, const float* d_x, const flo
         hipLaunchKernelGGL(gato::k_sim<1>, dim3(nb), dim3(64), 0, st, d_xnext, d_x, d_u, s->d_fext, dt, s->B);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int gato_select_hypothesis(gato_solver* s, const float* d_x_last, const float* d_u_last, const float* d_x_curr,
+                           float dt, float* d_xu, float* d_errors, int32_t* h_best, void* stream) {
+    if (!s || !d_x_last || !d_u_last || !d_x_curr || !d_xu) return fail(GATO_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int threads = std::min(1024, 64 * ((s->B + 63) / 64));
+    int* d_best = s->d_stop + 1;
+    if (s->plant == GATO_PLANT_IIWA14)
+        hipLaunchKernelGGL(gato::k_select<0>, dim3(1), dim3(threads), 0, st, d_x_last, d_u_last, d_x_curr, s->d_fext, dt,
+                           s->B, d_errors, d_best);
+    else
+        hipLaunchKernelGGL(gato::k_select<1>, dim3(1), dim3(threads), 0, st, d_x_last, d_u_last, d_x_curr, s->d_fext, dt,
+                           s->B, d_errors, d_best);
+    hipLaunchKernelGGL(gato::k_broadcast_row, dim3(256), dim3(256), 0, st, d_xu, s->traj, s->B, d_best);
+    HIP_TRY(hipGetLastError());
+    int32_t best = 0;
+    HIP_TRY(hipMemcpyAsync(&best, d_best, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (h_best) *h_best = best;
     return 0;
 }
 

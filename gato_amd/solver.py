@@ -269,6 +269,23 @@ class Solver:
         check(self.lib.gato_set_backward_params(self._h, int(max_pcg_iters), float(pcg_tol), int(refine)),
               "gato_set_backward_params")
 
+    def select_hypothesis(self, x_last, u_last, x_curr, dt, xu):
+        """MPC step helper: simulate every wrench hypothesis from (x_last, u_last), pick the one closest to the
+        measured x_curr, and make its trajectory every problem's warm start in the device tensor xu [B, traj].
+        Returns (best index, per-hypothesis errors as numpy)."""
+        if not (isinstance(xu, torch.Tensor) and xu.is_cuda and xu.dtype == torch.float32 and xu.is_contiguous()
+                and xu.numel() == self.B * self.traj):
+            raise ValueError(f"xu: expected a contiguous float32 device tensor with {self.B * self.traj} elements")
+        f = dict(dtype=torch.float32, device=self.device)
+        xl = self._to_dev(x_last, torch.zeros(self.nx, **f), "x_last")
+        ul = self._to_dev(u_last, torch.zeros(self.nu, **f), "u_last")
+        xc = self._to_dev(x_curr, torch.zeros(self.nx, **f), "x_curr")
+        err = torch.zeros(self.B, **f)
+        best = C.c_int32()
+        check(self.lib.gato_select_hypothesis(self._h, _ptr(xl), _ptr(ul), _ptr(xc), float(dt), _ptr(xu), _ptr(err),
+                                              C.byref(best), self._stream()), "gato_select_hypothesis")
+        return best.value, err.cpu().numpy()
+
     def set_launch_shape(self, groups):
         check(self.lib.gato_set_launch_shape(self._h, int(groups)), "gato_set_launch_shape")
 

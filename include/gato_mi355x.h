@@ -166,6 +166,14 @@ int gato_mixed_solve(gato_mixed* m, float* const* d_xu, const float* const* d_xs
 /* Device time of the last mixed graph launch (hipEvents on the caller's stream), microseconds. */
 int gato_mixed_last_time(gato_mixed* m, double* us);
 
+/* MPC hypothesis selection (the caller's per-step host work in mpc_controller.py:268-282 and 251-253, done on
+ * the device): one trapezoidal step of (d_x_last [nx], d_u_last [nu]) under each problem's wrench
+ * (gato_set_f_ext_batch), err_b = |x_next_b - d_x_curr|_2 into d_errors [B] (device, may be NULL), best =
+ * argmin (first index on ties) into *h_best (host, may be NULL); then every row of d_xu [B * traj] is
+ * overwritten with row best (the next solve's warm start). */
+int gato_select_hypothesis(gato_solver* s, const float* d_x_last, const float* d_u_last, const float* d_x_curr,
+                           float dt, float* d_xu, float* d_errors, int32_t* h_best, void* stream);
+
 /* Plant dimensions: nq (= nu), nx = 2 nq. */
 int gato_plant_dims(int32_t plant, int32_t* nq, int32_t* nx);
 
