@@ -56,6 +56,8 @@ SIGNATURES = {
     "gato_backward": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _VP, _VP, _VP, _VP, _VP, _ip]),
     "gato_set_backward_params": (C.c_int, [_VP, C.c_int32, C.c_float, C.c_int32]),
     "gato_select_hypothesis": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _VP, _ip, _VP]),
+    "gato_simulate_rk4": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, C.c_int32, _VP]),
+    "gato_window_reference": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int32, _VP]),
     "gato_set_launch_shape": (C.c_int, [_VP, C.c_int32]),
     "gato_get_launch_shape": (C.c_int, [_VP, _ip, _ip]),
     "gato_autotune": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _ip]),

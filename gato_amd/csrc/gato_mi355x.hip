@@ -887,6 +887,62 @@ __global__ void k_broadcast_row(float* xu, int traj, int B, const int* best) {
     }
 }
 
+
+// ------------------------------------------------------------------ plant simulation and reference windows (MPC)
+// RK4 of qdd = FD(q, qd, u; f_ext) over `steps` sub-steps of h, the integration the reference's MPC harness
+// runs on the host with pinocchio (common.py:49-91), here for every problem b with its own control u_b and
+// wrench f_ext_b (x_b updated in place).
+template <int PL>
+__global__ void k_rk4(float* x, const float* u, const float* fext, float h, int steps, int B) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    float q[NQ], v[NQ], uu[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) { q[i] = x[b * 2 * NQ + i]; v[i] = x[b * 2 * NQ + NQ + i]; uu[i] = u[b * NQ + i]; }
+    const float* fe = fext + 6 * b;
+    auto accel = [&](const float* qq, const float* vv, float* a) {
+        Kin<PL> K;
+        K.set(qq);
+        float L[NQ * (NQ + 1) / 2];
+        forward_dynamics<PL>(K, vv, uu, fe, a, L);
+    };
+    for (int st = 0; st < steps; st++) {
+        float k1v[NQ], k2v[NQ], k3v[NQ], k4v[NQ], qt[NQ], vt[NQ];
+        accel(q, v, k1v);
+#pragma unroll
+        for (int i = 0; i < NQ; i++) { qt[i] = q[i] + 0.5f * h * v[i]; vt[i] = v[i] + 0.5f * h * k1v[i]; }
+        accel(qt, vt, k2v);
+        float v2[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; i++) { v2[i] = vt[i]; qt[i] = q[i] + 0.5f * h * v2[i]; vt[i] = v[i] + 0.5f * h * k2v[i]; }
+        accel(qt, vt, k3v);
+        float v3[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; i++) { v3[i] = vt[i]; qt[i] = q[i] + h * v3[i]; vt[i] = v[i] + h * k3v[i]; }
+        accel(qt, vt, k4v);
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+            q[i] += h / 6.f * (v[i] + 2.f * v2[i] + 2.f * v3[i] + vt[i]);
+            v[i] += h / 6.f * (k1v[i] + 2.f * k2v[i] + 2.f * k3v[i] + k4v[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NQ; i++) { x[b * 2 * NQ + i] = q[i]; x[b * 2 * NQ + NQ + i] = v[i]; }
+}
+
+// ref_b[k] = path[offset + k] (6 floats per point) for every problem: the reference window the MPC caller
+// tiles on the host each step (mpc_controller.py:226-229), from a device-resident path.
+__global__ void k_window_ref(float* ref, const float* path, int npoints, int offset, int N, int B) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)B * N * 6; i += (size_t)gridDim.x * blockDim.x) {
+        const int j = (int)(i % (6 * N));
+        const int k = j / 6;
+        const int pt = std::min(offset + k, npoints - 1);
+        ref[i] = path[(size_t)pt * 6 + j % 6];
+    }
+}
+
 // ------------------------------------------------------------------ sim_forward (sim.cuh:14-48)
 template <int PL>
 __global__ void k_sim(float* xn, const float* x, const float* u, const float* fext, float dt, int B) {
@@ -1614,6 +1670,31 @@ int gato_select_hypothesis(gato_solver* s, const float* d_x_last, const float* d
     HIP_TRY(hipMemcpyAsync(&best, d_best, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (h_best) *h_best = best;
+    return 0;
+}
+
+int gato_simulate_rk4(gato_solver* s, float* d_x, const float* d_u, const float* d_f_ext, float h, int32_t steps,
+                      void* stream) {
+    if (!s || !d_x || !d_u || !d_f_ext) return fail(GATO_E_ARG, "null argument");
+    if (steps < 0 || !(h > 0.f)) return fail(GATO_E_ARG, "need steps >= 0 and h > 0");
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = (s->B + 63) / 64;
+    if (s->plant == GATO_PLANT_IIWA14)
+        hipLaunchKernelGGL(gato::k_rk4<0>, dim3(nb), dim3(64), 0, st, d_x, d_u, d_f_ext, h, steps, s->B);
+    else
+        hipLaunchKernelGGL(gato::k_rk4<1>, dim3(nb), dim3(64), 0, st, d_x, d_u, d_f_ext, h, steps, s->B);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int gato_window_reference(gato_solver* s, float* d_ref, const float* d_path, int32_t npoints, int32_t offset,
+                          void* stream) {
+    if (!s || !d_ref || !d_path) return fail(GATO_E_ARG, "null argument");
+    if (npoints < 1 || offset < 0) return fail(GATO_E_ARG, "need npoints >= 1 and offset >= 0");
+    hipLaunchKernelGGL(gato::k_window_ref, dim3(64), dim3(256), 0, (hipStream_t)stream, d_ref, d_path, npoints, offset,
+                       s->N, s->B);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 

@@ -286,6 +286,26 @@ class Solver:
                                               C.byref(best), self._stream()), "gato_select_hypothesis")
         return best.value, err.cpu().numpy()
 
+    def simulate_rk4(self, x, u, f_ext, h, steps):
+        """RK4 plant simulation of every problem's state x [B, nx] (device tensor, in place) under controls
+        u [B, nu] and wrenches f_ext [B, 6] (device tensors), `steps` sub-steps of length h."""
+        for t, n, name in ((x, self.B * self.nx, "x"), (u, self.B * self.nu, "u"), (f_ext, self.B * 6, "f_ext")):
+            if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+                    and t.numel() == n):
+                raise ValueError(f"{name}: expected a contiguous float32 device tensor with {n} elements")
+        check(self.lib.gato_simulate_rk4(self._h, _ptr(x), _ptr(u), _ptr(f_ext), float(h), int(steps), self._stream()),
+              "gato_simulate_rk4")
+
+    def window_reference(self, ref, path, offset):
+        """ref [B, 6N] (device tensor) <- the N points of the device path [npoints, 6] from `offset` on."""
+        if not (isinstance(path, torch.Tensor) and path.is_cuda and path.dtype == torch.float32 and path.is_contiguous()
+                and path.numel() % 6 == 0):
+            raise ValueError("path: expected a contiguous float32 device tensor of 6-float points")
+        if not (isinstance(ref, torch.Tensor) and ref.is_cuda and ref.numel() == self.B * 6 * self.N):
+            raise ValueError(f"ref: expected a float32 device tensor with {self.B * 6 * self.N} elements")
+        check(self.lib.gato_window_reference(self._h, _ptr(ref), _ptr(path), path.numel() // 6, int(offset),
+                                             self._stream()), "gato_window_reference")
+
     def set_launch_shape(self, groups):
         check(self.lib.gato_set_launch_shape(self._h, int(groups)), "gato_set_launch_shape")
 

@@ -174,6 +174,16 @@ int gato_mixed_last_time(gato_mixed* m, double* us);
 int gato_select_hypothesis(gato_solver* s, const float* d_x_last, const float* d_u_last, const float* d_x_curr,
                            float dt, float* d_xu, float* d_errors, int32_t* h_best, void* stream);
 
+/* Plant simulation for MPC loops (the caller's host integration, common.py:49-91, on the device): `steps` RK4
+ * sub-steps of length h of every problem's state d_x [B * nx] (in place) under its control d_u [B * nu] and
+ * wrench d_f_ext [B * 6] (device pointers; the wrench here is the "true" one, independent of the solver's). */
+int gato_simulate_rk4(gato_solver* s, float* d_x, const float* d_u, const float* d_f_ext, float h, int32_t steps,
+                      void* stream);
+/* Reference window for every problem from a device-resident path d_path [npoints * 6]:
+ * d_ref[b][k] = path[min(offset + k, npoints - 1)] (mpc_controller.py:226-229 tiles it on the host). Asynchronous. */
+int gato_window_reference(gato_solver* s, float* d_ref, const float* d_path, int32_t npoints, int32_t offset,
+                          void* stream);
+
 /* Plant dimensions: nq (= nu), nx = 2 nq. */
 int gato_plant_dims(int32_t plant, int32_t* nq, int32_t* nx);
 

@@ -300,3 +300,42 @@ def test_select_hypothesis_matches_host_selection(gpu, plant):
     np.testing.assert_allclose(err, expect, rtol=1e-4, atol=1e-6)
     assert best == int(np.argmin(expect)) == 5
     assert torch.equal(xu, row5.expand(B, -1))
+
+
+@pytest.mark.parametrize("plant", ["iiwa14", "indy7"])
+def test_rk4_simulation_and_reference_window(gpu, O, plant):
+    import torch
+
+    from gato_amd.solver import Solver
+
+    n = W.NQ[plant]
+    B, N = 8, 8
+    s = Solver(plant, N, B)
+    rng = np.random.default_rng(32)
+    x0 = rng.uniform(-1, 1, (B, 2 * n)).astype(np.float32)
+    u = rng.normal(0, 5, (B, n)).astype(np.float32)
+    fe = rng.normal(0, 2, (B, 6)).astype(np.float32)
+    t = lambda a: torch.tensor(a, dtype=torch.float32, device=s.device).contiguous()
+    x = t(x0)
+    h, steps = 1e-3, 10
+    s.simulate_rk4(x, t(u), t(fe), h, steps)
+    f = lambda a: a.astype(np.float64)
+    for b in range(B):  # RK4 on the oracle's forward dynamics
+        q, v = f(x0[b, :n]), f(x0[b, n:])
+        acc = lambda qq, vv: O.fd(plant, qq, vv, f(u[b]), f(fe[b]))
+        for _ in range(steps):
+            k1 = acc(q, v)
+            k2 = acc(q + 0.5 * h * v, v + 0.5 * h * k1)
+            v2 = v + 0.5 * h * k1
+            k3 = acc(q + 0.5 * h * v2, v + 0.5 * h * k2)
+            v3 = v + 0.5 * h * k2
+            k4 = acc(q + h * v3, v + h * k3)
+            v4 = v + h * k3
+            q, v = q + h / 6 * (v + 2 * v2 + 2 * v3 + v4), v + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        assert rel_err(x[b].cpu().numpy(), np.concatenate([q, v])) < 1e-5
+    path = rng.normal(0, 1, (20, 6)).astype(np.float32)
+    ref = torch.zeros(B, 6 * N, dtype=torch.float32, device=s.device)
+    s.window_reference(ref, t(path), 15)
+    torch.cuda.synchronize()
+    idx = np.minimum(15 + np.arange(N), 19)
+    np.testing.assert_array_equal(ref.cpu().numpy(), np.tile(path[idx].reshape(-1), (B, 1)))
