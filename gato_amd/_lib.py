@@ -10,6 +10,7 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libgato_mi355x.so"
 
 PLANT_IDS = {"iiwa14": 0, "indy7": 1}
 MODE_PLAIN, MODE_FUSED, MODE_GRAPH = 0, 1, 2
+FORCE_EST_STATE_LEN = 30  # estimate[6] momentum[6] smoothed[6] rotation[9] radius confidence updates
 EXIT_PCG0, EXIT_KKT = 0, 1
 
 
@@ -24,6 +25,7 @@ class GatoParams(C.Structure):
 
 _ip = C.POINTER(C.c_int32)
 _fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
 
 
 class GatoStats(C.Structure):
@@ -58,6 +60,14 @@ SIGNATURES = {
     "gato_select_hypothesis": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _VP, _ip, _VP]),
     "gato_simulate_rk4": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, C.c_int32, _VP]),
     "gato_window_reference": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int32, _VP]),
+    "gato_figure8_path": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_double, C.c_double, _dp, C.c_double, _VP]),
+    "gato_force_est_create": (C.c_int, [C.POINTER(_VP), C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float,
+                                        C.c_uint64]),
+    "gato_force_est_destroy": (C.c_int, [_VP]),
+    "gato_force_est_reset": (C.c_int, [_VP, C.c_float, _VP]),
+    "gato_force_est_generate": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "gato_force_est_update": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, C.c_float, _dp, _VP]),
+    "gato_force_est_state": (C.c_int, [_VP, _dp, _VP]),
     "gato_set_launch_shape": (C.c_int, [_VP, C.c_int32]),
     "gato_get_launch_shape": (C.c_int, [_VP, _ip, _ip]),
     "gato_autotune": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _ip]),

@@ -943,6 +943,208 @@ __global__ void k_window_ref(float* ref, const float* path, int npoints, int off
     }
 }
 
+// Figure-8 end-effector path of the MPC examples (common.py:10-46), generated on the device: point j of a cycle
+// of m points sits at t_j = 2 pi j / (m - 1) (the last one exactly at 2 pi, as numpy.linspace places it) on
+// (ax sin t, 0, az sin(2t)/2 + az/2) + o, rotated by theta about z; rpy = 0; the cycle repeats `cycles` times.
+// Evaluated in double and rounded once, as the callers round the float64 path.
+__global__ void k_figure8(float* path, int m, int cycles, double ax, double az, double ox, double oy, double oz,
+                          double theta) {
+    const double c = cos(theta), s = sin(theta);
+    const double step = m > 1 ? 2.0 * M_PI / (double)(m - 1) : 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)m * cycles; i += (size_t)gridDim.x * blockDim.x) {
+        const int j = (int)(i % m);
+        const double t = (m > 1 && j == m - 1) ? 2.0 * M_PI : (double)j * step;
+        const double xu = ox + ax * sin(t), yu = oy, zu = oz + az * sin(2.0 * t) / 2.0 + az / 2.0;
+        float* pt = path + 6 * i;
+        pt[0] = (float)(c * xu - s * yu);
+        pt[1] = (float)(s * xu + c * yu);
+        pt[2] = (float)zu;
+        pt[3] = pt[4] = pt[5] = 0.f;
+    }
+}
+
+// ------------------------------------------------------------------ force hypotheses for MPC (force_estimator.py)
+// State of the MPC examples' wrench estimator (force_estimator.py:4-31) kept on the device. Wrenches here are
+// world-frame [force; moment]. radius and confidence stay in double as the reference's Python floats do.
+struct ForceEst {
+    float est[6], mom[6], smooth[6];
+    float R[9];      // rotation applied to the exploration directions (row-major)
+    double radius, confidence;
+    double hist[5];  // last five minimum prediction errors, oldest first
+    int count;       // updates so far (the length of the reference's error history)
+};
+
+// Hypothesis i of the batch (force_estimator.py:82-97): 0 the smoothed estimate, 1 zero, 2 the estimate pushed
+// along its momentum, i >= 3 the blended estimate plus radius times the rotated exploration direction i - 3.
+// Float arithmetic is rounded per operation like numpy's float32 arrays (no contraction).
+__device__ __forceinline__ void force_hypothesis(const ForceEst& st, const float* dirs, int i, float* f) {
+#pragma clang fp contract(off)
+    if (i == 0) {
+        for (int c = 0; c < 6; c++) f[c] = st.smooth[c];
+    } else if (i == 1) {
+        for (int c = 0; c < 6; c++) f[c] = 0.f;
+    } else if (i == 2) {
+        for (int c = 0; c < 6; c++) f[c] = st.smooth[c] + 0.5f * st.mom[c];
+    } else {
+        const float* d = dirs + 3 * (i - 3);
+        for (int r = 0; r < 3; r++) {
+            const float dir = (st.R[3 * r] * d[0] + st.R[3 * r + 1] * d[1]) + st.R[3 * r + 2] * d[2];
+            const float base = 0.7f * st.smooth[r] + 0.3f * st.est[r];
+            f[r] = (float)((double)base + st.radius * (double)dir);
+        }
+        for (int c = 3; c < 6; c++) f[c] = st.smooth[c];
+    }
+}
+
+// World placements (R row-major, origin) of the last two joint frames at q, in double: the frames whose relative
+// placement the wrench transform of the MPC caller uses (the end-effector's parent joint and the joint before it).
+template <int PL>
+__device__ void last_joint_frames(const float* q, double* R1, double* o1, double* R0, double* o0) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ;
+    double W[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, o[3] = {0, 0, 0};
+    Unroll<0, NQ>::run([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        constexpr auto& T = P::J::T[i];  // R_tree (row-major), p_tree of joint i in double
+#pragma unroll
+        for (int r = 0; r < 3; r++) o[r] += W[3 * r] * T[9] + W[3 * r + 1] * T[10] + W[3 * r + 2] * T[11];
+        double WT[9];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                WT[3 * r + k] = W[3 * r] * T[k] + W[3 * r + 1] * T[3 + k] + W[3 * r + 2] * T[6 + k];
+        const double s = sin((double)q[i]), c = cos((double)q[i]);
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            W[3 * r] = WT[3 * r] * c + WT[3 * r + 1] * s;
+            W[3 * r + 1] = -WT[3 * r] * s + WT[3 * r + 1] * c;
+            W[3 * r + 2] = WT[3 * r + 2];
+        }
+        if constexpr (i == NQ - 2) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) R0[k] = W[k];
+#pragma unroll
+            for (int k = 0; k < 3; k++) o0[k] = o[k];
+        }
+    });
+    for (int k = 0; k < 9; k++) R1[k] = W[k];
+    for (int k = 0; k < 3; k++) o1[k] = o[k];
+}
+
+// Inverse action of the placement (R, p) on a wrench [f; n]: (R^T f, R^T (n - p x f)).
+__device__ __forceinline__ void wrench_act_inv(const double* R, const double* p, const double* w, double* out) {
+    const double n0 = w[3] - (p[1] * w[2] - p[2] * w[1]);
+    const double n1 = w[4] - (p[2] * w[0] - p[0] * w[2]);
+    const double n2 = w[5] - (p[0] * w[1] - p[1] * w[0]);
+    for (int k = 0; k < 3; k++) {
+        out[k] = R[k] * w[0] + R[3 + k] * w[1] + R[6 + k] * w[2];
+        out[3 + k] = R[k] * n0 + R[3 + k] * n1 + R[6 + k] * n2;
+    }
+}
+
+// Every hypothesis b < B in the world frame (optional output) and mapped to the wrench the solver takes, the way
+// mpc_controller.py:311-337 maps it: inverse action of the last joint's world placement, then inverse action of
+// that joint's placement relative to the joint before it. Frames at q are computed once per work-group.
+template <int PL>
+__global__ void __launch_bounds__(256) k_force_hypotheses(const ForceEst* st, const float* dirs, const float* q,
+                                                          int B, float* fext, float* world) {
+    __shared__ double fr[12];  // R' (relative rotation), p' (relative origin) of the last joint
+    __shared__ double M1[12];  // world rotation and origin of the last joint
+    __shared__ ForceEst S;
+    if (threadIdx.x == 0) {
+        double R1[9], o1[3], R0[9], o0[3];
+        last_joint_frames<PL>(q, R1, o1, R0, o0);
+        const double d[3] = {o1[0] - o0[0], o1[1] - o0[1], o1[2] - o0[2]};
+        for (int r = 0; r < 3; r++) {
+            for (int k = 0; k < 3; k++)
+                fr[3 * r + k] = R0[r] * R1[k] + R0[3 + r] * R1[3 + k] + R0[6 + r] * R1[6 + k];  // R0^T R1
+            fr[9 + r] = R0[r] * d[0] + R0[3 + r] * d[1] + R0[6 + r] * d[2];                 // R0^T (o1 - o0)
+        }
+        for (int k = 0; k < 9; k++) M1[k] = R1[k];
+        for (int k = 0; k < 3; k++) M1[9 + k] = o1[k];
+        S = *st;
+    }
+    __syncthreads();
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < B; b += gridDim.x * blockDim.x) {
+        float f[6];
+        force_hypothesis(S, dirs, b, f);
+        if (world)
+            for (int c = 0; c < 6; c++) world[6 * b + c] = f[c];
+        double w[6], wl[6], wj[6];
+        for (int c = 0; c < 6; c++) w[c] = f[c];
+        wrench_act_inv(M1, M1 + 9, w, wl);
+        wrench_act_inv(fr, fr + 9, wl, wj);
+        for (int c = 0; c < 6; c++) fext[6 * b + c] = (float)wj[c];
+    }
+}
+
+// Estimator update after a solve (force_estimator.py:99-136) by one thread: the selected hypothesis (*best) is
+// regenerated from the unchanged state, momentum / estimate / smoothed estimate move toward it, the search
+// radius shrinks when an exploitation hypothesis (< 3) won and grows otherwise, the last five minimum
+// prediction errors adjust it further, and the exploration directions get the rotation of the unit quaternion
+// drawn from the uniforms u (the reference draws them with numpy.random.rand).
+__global__ void k_force_update(ForceEst* st, const float* dirs, const int* best, const float* err, float alpha,
+                               float beta, double smoothing, double r_min, double r_max, double u1, double u2,
+                               double u3) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    ForceEst S = *st;
+    const int bi = *best;
+    const double emin = (double)err[bi];  // the minimum error is the selected hypothesis's (first argmin)
+    if (S.count < 5) {
+        S.hist[S.count] = emin;
+    } else {
+        for (int k = 0; k < 4; k++) S.hist[k] = S.hist[k + 1];
+        S.hist[4] = emin;
+    }
+    S.count++;
+    float bf[6];
+    force_hypothesis(*st, dirs, bi, bf);
+    const float b32 = beta, omb = (float)(1.0 - (double)beta);
+    const float a32 = alpha, oma = (float)(1.0 - (double)alpha);
+    const float sm = (float)smoothing, oms = (float)(1.0 - smoothing);
+    for (int c = 0; c < 6; c++) {
+        const float delta = bf[c] - S.est[c];
+        S.mom[c] = b32 * S.mom[c] + omb * delta;
+        const float raw = a32 * bf[c] + oma * S.est[c];
+        S.est[c] = 0.8f * S.est[c] + 0.2f * (raw + 0.5f * S.mom[c]);
+        S.smooth[c] = oms * S.smooth[c] + sm * S.est[c];
+    }
+    if (bi < 3) {
+        S.radius *= 0.95;
+        S.confidence = fmin(1.0, S.confidence + 0.05);
+    } else {
+        S.radius *= 1.05;
+        S.confidence = fmax(0.0, S.confidence - 0.1);
+    }
+    S.radius = fmin(fmax(S.radius, r_min), r_max);
+    if (S.count > 5) {
+        double mean = 0.0;
+        for (int k = 0; k < 5; k++) mean += S.hist[k];
+        mean /= 5.0;
+        double var = 0.0;
+        for (int k = 0; k < 5; k++) var += (S.hist[k] - mean) * (S.hist[k] - mean);
+        const double sd = sqrt(var / 5.0);
+        const double mean4 = (((S.hist[0] + S.hist[1]) + S.hist[2]) + S.hist[3]) / 4.0;
+        if (sd < 0.01) {
+            S.radius *= 0.9;
+        } else if (S.hist[4] > 1.5 * mean4) {
+            S.radius *= 1.3;
+            S.confidence *= 0.5;
+        }
+        S.radius = fmin(fmax(S.radius, r_min), r_max);
+    }
+    // unit quaternion (x, y, z, w) from three uniforms, then its rotation matrix
+    const double x = sqrt(1.0 - u1) * sin(2.0 * M_PI * u2), y = sqrt(1.0 - u1) * cos(2.0 * M_PI * u2);
+    const double z = sqrt(u1) * sin(2.0 * M_PI * u3), w = sqrt(u1) * cos(2.0 * M_PI * u3);
+    const double R[9] = {1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - w * z), 2.0 * (x * z + w * y),
+                         2.0 * (x * y + w * z), 1.0 - 2.0 * (x * x + z * z), 2.0 * (y * z - w * x),
+                         2.0 * (x * z - w * y), 2.0 * (y * z + w * x), 1.0 - 2.0 * (x * x + y * y)};
+    for (int k = 0; k < 9; k++) S.R[k] = (float)R[k];
+    *st = S;
+}
+
 // ------------------------------------------------------------------ sim_forward (sim.cuh:14-48)
 template <int PL>
 __global__ void k_sim(float* xn, const float* x, const float* u, const float* fext, float dt, int B) {
@@ -1698,6 +1900,151 @@ int gato_window_reference(gato_solver* s, float* d_ref, const float* d_path, int
     return 0;
 }
 
+int gato_figure8_path(float* d_path, int32_t points_per_cycle, int32_t cycles, double amp_x, double amp_z,
+                      const double* offset, double theta, void* stream) {
+    if (!d_path || !offset) return fail(GATO_E_ARG, "null argument");
+    if (points_per_cycle < 1 || cycles < 1) return fail(GATO_E_ARG, "need points_per_cycle >= 1 and cycles >= 1");
+    hipLaunchKernelGGL(gato::k_figure8, dim3(64), dim3(256), 0, (hipStream_t)stream, d_path, points_per_cycle, cycles,
+                       amp_x, amp_z, offset[0], offset[1], offset[2], theta);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+struct gato_force_est {
+    int B = 0;
+    double smoothing = 0.3, r_min = 1.0, r_max = 100.0;
+    uint64_t rng = 0;
+    gato::ForceEst* d_st = nullptr;
+    float* d_dirs = nullptr;
+};
+
+namespace {
+// SplitMix64 (Steele, Lea and Flood, 2014): the library's own uniforms for the exploration rotations
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double uniform01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+gato::ForceEst initial_force_state(double radius) {
+    gato::ForceEst st{};
+    st.R[0] = st.R[4] = st.R[8] = 1.f;
+    st.radius = radius;
+    return st;
+}
+}  // namespace
+
+int gato_force_est_create(gato_force_est** out, int32_t batch, float initial_radius, float min_radius,
+                          float max_radius, float smoothing_factor, uint64_t seed) {
+    if (!out) return fail(GATO_E_ARG, "null argument");
+    *out = nullptr;
+    if (batch <= 3) return fail(GATO_E_ARG, "batch must exceed 3 (three exploitation hypotheses plus exploration)");
+    if (!(min_radius > 0.f) || !(max_radius >= min_radius) || !(initial_radius > 0.f))
+        return fail(GATO_E_ARG, "need 0 < min_radius <= max_radius and initial_radius > 0");
+    if (!(smoothing_factor >= 0.f && smoothing_factor <= 1.f)) return fail(GATO_E_ARG, "smoothing_factor outside [0, 1]");
+    // exploration directions: Fibonacci spiral on the unit sphere (force_estimator.py:33-61), double then float
+    const int n = batch - 3;
+    std::vector<float> dirs(3 * (size_t)n);
+    const double phi = (1.0 + std::sqrt(5.0)) / 2.0;
+    for (int i = 0; i < n; i++) {
+        const double y = n > 1 ? 1.0 - 2.0 * i / (double)(n - 1) : 0.0;
+        const double r = std::sqrt(1.0 - y * y), th = 2.0 * M_PI * i / phi;
+        dirs[3 * i] = (float)(r * std::cos(th));
+        dirs[3 * i + 1] = (float)y;
+        dirs[3 * i + 2] = (float)(r * std::sin(th));
+    }
+    auto* e = new gato_force_est();
+    e->B = batch;
+    e->smoothing = smoothing_factor;
+    e->r_min = min_radius;
+    e->r_max = max_radius;
+    e->rng = seed;
+    const gato::ForceEst st = initial_force_state(initial_radius);
+    if (hipMalloc(&e->d_st, sizeof(gato::ForceEst)) != hipSuccess ||
+        hipMalloc(&e->d_dirs, sizeof(float) * std::max<size_t>(dirs.size(), 1)) != hipSuccess) {
+        if (e->d_st) (void)hipFree(e->d_st);
+        delete e;
+        return fail(GATO_E_ALLOC, "device allocation of the force estimator failed");
+    }
+    if (hipMemcpy(e->d_st, &st, sizeof(st), hipMemcpyHostToDevice) != hipSuccess ||
+        (n > 0 && hipMemcpy(e->d_dirs, dirs.data(), sizeof(float) * dirs.size(), hipMemcpyHostToDevice) != hipSuccess)) {
+        (void)hipFree(e->d_st);
+        (void)hipFree(e->d_dirs);
+        delete e;
+        return fail(GATO_E_LAUNCH, "upload of the force estimator state failed");
+    }
+    *out = e;
+    return 0;
+}
+
+int gato_force_est_destroy(gato_force_est* e) {
+    if (!e) return fail(GATO_E_ARG, "null handle");
+    (void)hipFree(e->d_st);
+    (void)hipFree(e->d_dirs);
+    delete e;
+    return 0;
+}
+
+int gato_force_est_reset(gato_force_est* e, float radius, void* stream) {
+    if (!e) return fail(GATO_E_ARG, "null handle");
+    if (!(radius > 0.f)) return fail(GATO_E_ARG, "radius must be positive");
+    const gato::ForceEst st = initial_force_state(radius);
+    hipStream_t sm = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(e->d_st, &st, sizeof(st), hipMemcpyHostToDevice, sm));
+    HIP_TRY(hipStreamSynchronize(sm));
+    return 0;
+}
+
+int gato_force_est_generate(gato_force_est* e, gato_solver* s, const float* d_q, float* d_world, void* stream) {
+    if (!e || !s || !d_q) return fail(GATO_E_ARG, "null argument");
+    if (e->B != s->B) return fail(GATO_E_ARG, "estimator and solver batch sizes differ");
+    const dim3 grid((e->B + 255) / 256);
+    hipStream_t st = (hipStream_t)stream;
+    if (s->plant == GATO_PLANT_IIWA14)
+        hipLaunchKernelGGL(gato::k_force_hypotheses<0>, grid, dim3(256), 0, st, e->d_st, e->d_dirs, d_q, e->B, s->d_fext,
+                           d_world);
+    else
+        hipLaunchKernelGGL(gato::k_force_hypotheses<1>, grid, dim3(256), 0, st, e->d_st, e->d_dirs, d_q, e->B, s->d_fext,
+                           d_world);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gato_force_est_update(gato_force_est* e, gato_solver* s, const int32_t* d_best, const float* d_errors, float alpha,
+                          float beta, const double* h_uniforms, void* stream) {
+    if (!e || !s || !d_errors) return fail(GATO_E_ARG, "null argument");
+    if (e->B != s->B) return fail(GATO_E_ARG, "estimator and solver batch sizes differ");
+    double u[3];
+    for (int k = 0; k < 3; k++) u[k] = h_uniforms ? h_uniforms[k] : uniform01(e->rng);
+    for (int k = 0; k < 3; k++)
+        if (!(u[k] >= 0.0 && u[k] <= 1.0)) return fail(GATO_E_ARG, "uniforms must lie in [0, 1]");
+    const int* best = d_best ? d_best : s->d_stop + 1;  // the solver's last gato_select_hypothesis result
+    hipLaunchKernelGGL(gato::k_force_update, dim3(1), dim3(64), 0, (hipStream_t)stream, e->d_st, e->d_dirs, best,
+                       d_errors, alpha, beta, e->smoothing, e->r_min, e->r_max, u[0], u[1], u[2]);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gato_force_est_state(gato_force_est* e, double* h_state, void* stream) {
+    if (!e || !h_state) return fail(GATO_E_ARG, "null argument");
+    gato::ForceEst st;
+    hipStream_t sm = (hipStream_t)stream;
+    HIP_TRY(hipMemcpyAsync(&st, e->d_st, sizeof(st), hipMemcpyDeviceToHost, sm));
+    HIP_TRY(hipStreamSynchronize(sm));
+    for (int c = 0; c < 6; c++) {
+        h_state[c] = st.est[c];
+        h_state[6 + c] = st.mom[c];
+        h_state[12 + c] = st.smooth[c];
+    }
+    for (int k = 0; k < 9; k++) h_state[18 + k] = st.R[k];
+    h_state[27] = st.radius;
+    h_state[28] = st.confidence;
+    h_state[29] = st.count;
+    return 0;
+}
+
 int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_xs, const float* d_ref, float dt,
                      void* stream) {
     if (!s || !d_xu || !d_xs || !d_ref) return fail(GATO_E_ARG, "null argument");
@@ -1752,7 +2099,7 @@ int gato_debug_copy(gato_solver* s, const char* name, void* d_dst, int64_t cap, 
                      {"So", a.So, B * N * nx * nx},       {"Pd", a.Pd, B * N * nx * nx},  {"Po", a.Po, B * N * nx * nx},
                      {"gamma", a.gam, B * N * nx},        {"lambda", a.lam, B * N * nx},  {"dz", a.dz, B * (s->traj + nu)},
                      {"merit_parts", a.mpart, B * gato::NUM_ALPHAS * N}, {"pcg_iters", a.st_pcg, it * B},
-                     {"kkt_residual", a.st_kres, it * B}};
+                     {"kkt_residual", a.st_kres, it * B}, {"f_ext", s->d_fext, B * 6}};
     for (const E& e : tab) {
         if (std::strcmp(e.n, name) != 0) continue;
         if (count) *count = (int64_t)e.c;

@@ -183,6 +183,38 @@ int gato_simulate_rk4(gato_solver* s, float* d_x, const float* d_u, const float*
  * d_ref[b][k] = path[min(offset + k, npoints - 1)] (mpc_controller.py:226-229 tiles it on the host). Asynchronous. */
 int gato_window_reference(gato_solver* s, float* d_ref, const float* d_path, int32_t npoints, int32_t offset,
                           void* stream);
+/* The MPC examples' figure-8 end-effector path generated on the device (common.py:10-46, which builds it with
+ * numpy on the host): points_per_cycle points of t in [0, 2 pi] (numpy.linspace spacing) on
+ * (amp_x sin t, 0, amp_z sin(2t)/2 + amp_z/2) + offset[3], rotated by theta about z, zero rpy, repeated `cycles`
+ * times, into d_path [points_per_cycle * cycles * 6] (evaluated in double, stored as float). Asynchronous. */
+int gato_figure8_path(float* d_path, int32_t points_per_cycle, int32_t cycles, double amp_x, double amp_z,
+                      const double* offset, double theta, void* stream);
+
+/* Wrench-hypothesis estimator of the MPC examples (force_estimator.py:4-155 and its use in
+ * mpc_controller.py:279-337) with its state on the device. Hypotheses: 0 the smoothed estimate, 1 zero, 2 the
+ * estimate plus half its momentum, 3.. exploration around the estimate on a rotated Fibonacci sphere of the
+ * current radius; world frame, [force; moment]. */
+typedef struct gato_force_est gato_force_est;
+#define GATO_FORCE_EST_STATE_LEN 30 /* estimate[6] momentum[6] smoothed[6] rotation[9] radius confidence updates */
+/* batch > 3 (force_estimator.py:8); the radius starts at initial_radius and stays in [min_radius, max_radius];
+ * seed drives the library's own uniforms for the exploration rotations (used when an update passes none). */
+int gato_force_est_create(gato_force_est** e, int32_t batch, float initial_radius, float min_radius,
+                          float max_radius, float smoothing_factor, uint64_t seed);
+int gato_force_est_destroy(gato_force_est* e);
+/* Back to the initial state (force_estimator.py:150-155) with the given radius. Synchronous. */
+int gato_force_est_reset(gato_force_est* e, float radius, void* stream);
+/* update_force_batch (mpc_controller.py:279-292) on the device: every hypothesis mapped to the solver's wrench
+ * frame at joint positions d_q [nq] (device) as transform_force_to_gato_frame does (mpc_controller.py:311-337)
+ * and written to the solver's per-problem wrenches (what gato_set_f_ext_batch sets); d_world [B * 6] receives
+ * the world-frame hypotheses when not NULL. Batch sizes must match. Asynchronous. */
+int gato_force_est_generate(gato_force_est* e, gato_solver* s, const float* d_q, float* d_world, void* stream);
+/* ForceEstimator.update (force_estimator.py:99-136) on the device for the winning hypothesis *d_best (NULL: the
+ * solver's last gato_select_hypothesis result) with that call's prediction errors d_errors [B]; h_uniforms: three
+ * numbers in [0, 1] for the next exploration rotation (NULL: the estimator's own generator). Asynchronous. */
+int gato_force_est_update(gato_force_est* e, gato_solver* s, const int32_t* d_best, const float* d_errors, float alpha,
+                          float beta, const double* h_uniforms, void* stream);
+/* Copy of the state (GATO_FORCE_EST_STATE_LEN doubles, layout above). Synchronous. */
+int gato_force_est_state(gato_force_est* e, double* h_state, void* stream);
 
 /* Plant dimensions: nq (= nu), nx = 2 nq. */
 int gato_plant_dims(int32_t plant, int32_t* nq, int32_t* nx);
@@ -199,7 +231,7 @@ int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_
                      void* stream);
 /* Extension (phase-level tests): copy workspace array `name` ("A", "B", "c", "Qp", "Qv", "Rg", "q", "r",
  * "Qip", "Qiv", "Ri", "Sd", "So", "Pd", "Po", "gamma", "lambda", "dz", "merit_parts", "pcg_iters",
- * "kkt_residual") into the device buffer d_dst of capacity `cap` elements; *count = its length. */
+ * "kkt_residual", "f_ext") into the device buffer d_dst of capacity `cap` elements; *count = its length. */
 int gato_debug_copy(gato_solver* s, const char* name, void* d_dst, int64_t cap, int64_t* count);
 
 /* Description of the last failure on the calling thread ("" if none). */

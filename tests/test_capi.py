@@ -63,6 +63,17 @@ def test_argument_validation_without_device():
     assert lib.gato_mixed_create(C.byref(hm), 0, one, one, one, C.byref(p)) == 1001
     assert lib.gato_mixed_solve(None, None, None, None, 0.01, None, None) == 1001
     assert lib.gato_mixed_destroy(None) == 0
+    # MPC helpers: path generation and the wrench estimator check their arguments first
+    off = (C.c_double * 3)(0.0, 0.5, 0.6)
+    assert lib.gato_figure8_path(None, 600, 5, 0.4, 0.4, off, 0.7, None) == 1001
+    assert lib.gato_figure8_path(C.c_void_p(16), 0, 5, 0.4, 0.4, off, 0.7, None) == 1001
+    he = C.c_void_p()
+    assert lib.gato_force_est_create(C.byref(he), 3, 5.0, 2.0, 20.0, 0.5, 0) == 1001  # needs batch > 3
+    assert lib.gato_force_est_create(C.byref(he), 16, 5.0, 30.0, 20.0, 0.5, 0) == 1001  # min > max radius
+    assert lib.gato_force_est_create(C.byref(he), 16, 5.0, 2.0, 20.0, 1.5, 0) == 1001  # smoothing outside [0, 1]
+    assert lib.gato_force_est_generate(None, None, None, None, None) == 1001
+    assert lib.gato_force_est_update(None, None, None, None, 0.5, 0.8, None, None) == 1001
+    assert lib.gato_force_est_destroy(None) == 1001
 
 
 def test_drop_in_module_surface():
