@@ -90,10 +90,15 @@ class ForceEstimator:
               "gato_force_est_generate")
         return self._world.view(self.batch_size, 6)
 
-    def update(self, prediction_errors, alpha=0.5, beta=0.8, best=None, uniforms=None):
+    def update(self, prediction_errors=None, alpha=0.5, beta=0.8, best=None, uniforms=None):
         """Move the estimate toward the winning hypothesis (force_estimator.py:99-136). prediction_errors: the
-        per-hypothesis errors [B]; best: the winning index (device int32 tensor or int; default the solver's
-        last select_hypothesis choice)."""
+        per-hypothesis errors [B] (default: those of the solver's last select_hypothesis, still on the device);
+        best: the winning index (device int32 tensor or int; default the solver's last select_hypothesis
+        choice)."""
+        if prediction_errors is None:
+            prediction_errors = getattr(self.solver, "last_selection_errors", None)
+            if prediction_errors is None:
+                raise ValueError("no prediction errors: pass them or run solver.select_hypothesis first")
         err = _dev_f32(prediction_errors, self.batch_size, "prediction_errors", self.device)
         bptr = None
         if best is not None:

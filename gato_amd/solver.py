@@ -284,6 +284,7 @@ class Solver:
         best = C.c_int32()
         check(self.lib.gato_select_hypothesis(self._h, _ptr(xl), _ptr(ul), _ptr(xc), float(dt), _ptr(xu), _ptr(err),
                                               C.byref(best), self._stream()), "gato_select_hypothesis")
+        self.last_selection_errors = err  # device copy, for gato_amd.mpc.ForceEstimator.update
         return best.value, err.cpu().numpy()
 
     def simulate_rk4(self, x, u, f_ext, h, steps):
