@@ -893,7 +893,7 @@ __global__ void k_broadcast_row(float* xu, int traj, int B, const int* best) {
 // runs on the host with pinocchio (common.py:49-91), here for every problem b with its own control u_b and
 // wrench f_ext_b (x_b updated in place).
 template <int PL>
-__global__ void k_rk4(float* x, const float* u, const float* fext, float h, int steps, int B) {
+__global__ void __launch_bounds__(64) k_rk4(float* x, const float* u, const float* fext, float h, int steps, int B) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1335,7 +1335,8 @@ size_t lds_bytes(const gato_solver* s) {
     const int G = s->G, nx = s->nx, nq = s->nq;
     size_t f = 0;
     f = std::max<size_t>(f, (size_t)G * (24 * nq + 9 * (nq * (nq + 1) / 2)));  // lin
-    f = std::max<size_t>(f, (size_t)G * (nx * 16 + nx * 8 + nq * 8 + 2 * nx));  // schur (SchurTile)
+    const size_t tile = s->plant == GATO_PLANT_IIWA14 ? gato::SchurTile<0>::SIZE : gato::SchurTile<1>::SIZE;
+    f = std::max<size_t>(f, (size_t)G * tile);                                  // schur (SchurTile)
     f = std::max<size_t>(f, (size_t)2 * (s->N + 2) * 16 + 64);      // pcg
     f = std::max<size_t>(f, (size_t)G * gato::KG + 16);             // dz
     f = std::max<size_t>(f, (size_t)2 * gato::NUM_ALPHAS + 16);     // merit + ls
