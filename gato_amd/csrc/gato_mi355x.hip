@@ -832,9 +832,9 @@ __global__ void k_bwd_begin(Args a, const float* lam_keep, float* lam_b, float* 
 
 
 // ------------------------------------------------------------------ MPC hypothesis selection (no reference kernel)
-// The MPC caller's per-step host work (mpc_controller.py:268-282): step every wrench hypothesis b from
+// The MPC caller's per-step host work (mpc_controller.py:294-309): step every wrench hypothesis b from
 // (x_last, u_last) with its f_ext_b, err_b = |x_next_b - x_curr|_2, best = argmin (first on ties, as numpy's
-// argmin), then every problem's warm start becomes problem best's trajectory (mpc_controller.py:251-253).
+// argmin), then every problem's warm start becomes problem best's trajectory (mpc_controller.py:240-242).
 template <int PL>
 __global__ void __launch_bounds__(1024) k_select(const float* x, const float* u, const float* xcur,
                                                  const float* fext, float dt, int B, float* err, int* best) {
@@ -842,6 +842,12 @@ __global__ void __launch_bounds__(1024) k_select(const float* x, const float* u,
     constexpr int NQ = P::NQ;
     __shared__ float se[1024];
     __shared__ int si[1024];
+    // numpy.argmin's order (mpc_controller.py:303-304): the first NaN if there is one, else the first minimum
+    auto better = [](float e2, int i2, float e1, int i1) {
+        const bool n2 = e2 != e2, n1 = e1 != e1;
+        if (n2 != n1) return n2;
+        return (!n2 && e2 < e1) || ((n2 || e2 == e1) && i2 < i1);
+    };
     float emin = 3.4e38f;
     int imin = B;
     for (int b = threadIdx.x; b < B; b += blockDim.x) {
@@ -860,7 +866,7 @@ __global__ void __launch_bounds__(1024) k_select(const float* x, const float* u,
         }
         e = sqrtf(e);
         if (err) err[b] = e;
-        if (e < emin) { emin = e; imin = b; }
+        if (better(e, b, emin, imin)) { emin = e; imin = b; }
     }
     se[threadIdx.x] = emin;
     si[threadIdx.x] = imin;
@@ -869,14 +875,14 @@ __global__ void __launch_bounds__(1024) k_select(const float* x, const float* u,
         if ((int)threadIdx.x < h) {
             const float e2 = se[threadIdx.x + h];
             const int i2 = si[threadIdx.x + h];
-            if (e2 < se[threadIdx.x] || (e2 == se[threadIdx.x] && i2 < si[threadIdx.x])) {
+            if (better(e2, i2, se[threadIdx.x], si[threadIdx.x])) {
                 se[threadIdx.x] = e2;
                 si[threadIdx.x] = i2;
             }
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *best = si[0];
+    if (threadIdx.x == 0) *best = si[0] < B ? si[0] : 0;  // B = 0 aside, si[0] < B always holds here
 }
 
 __global__ void k_broadcast_row(float* xu, int traj, int B, const int* best) {
@@ -1084,13 +1090,14 @@ __global__ void __launch_bounds__(256) k_force_hypotheses(const ForceEst* st, co
 // radius shrinks when an exploitation hypothesis (< 3) won and grows otherwise, the last five minimum
 // prediction errors adjust it further, and the exploration directions get the rotation of the unit quaternion
 // drawn from the uniforms u (the reference draws them with numpy.random.rand).
-__global__ void k_force_update(ForceEst* st, const float* dirs, const int* best, const float* err, float alpha,
+__global__ void k_force_update(ForceEst* st, const float* dirs, const int* best, const float* err, int B, float alpha,
                                float beta, double smoothing, double r_min, double r_max, double u1, double u2,
                                double u3) {
 #pragma clang fp contract(off)
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     ForceEst S = *st;
     const int bi = *best;
+    if (bi < 0 || bi >= B) return;  // an index outside the batch leaves the estimator unchanged
     const double emin = (double)err[bi];  // the minimum error is the selected hypothesis's (first argmin)
     if (S.count < 5) {
         S.hist[S.count] = emin;
@@ -2023,7 +2030,7 @@ int gato_force_est_update(gato_force_est* e, gato_solver* s, const int32_t* d_be
         if (!(u[k] >= 0.0 && u[k] <= 1.0)) return fail(GATO_E_ARG, "uniforms must lie in [0, 1]");
     const int* best = d_best ? d_best : s->d_stop + 1;  // the solver's last gato_select_hypothesis result
     hipLaunchKernelGGL(gato::k_force_update, dim3(1), dim3(64), 0, (hipStream_t)stream, e->d_st, e->d_dirs, best,
-                       d_errors, alpha, beta, e->smoothing, e->r_min, e->r_max, u[0], u[1], u[2]);
+                       d_errors, e->B, alpha, beta, e->smoothing, e->r_min, e->r_max, u[0], u[1], u[2]);
     HIP_TRY(hipGetLastError());
     return 0;
 }

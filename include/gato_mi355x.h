@@ -166,10 +166,11 @@ int gato_mixed_solve(gato_mixed* m, float* const* d_xu, const float* const* d_xs
 /* Device time of the last mixed graph launch (hipEvents on the caller's stream), microseconds. */
 int gato_mixed_last_time(gato_mixed* m, double* us);
 
-/* MPC hypothesis selection (the caller's per-step host work in mpc_controller.py:268-282 and 251-253, done on
+/* MPC hypothesis selection (the caller's per-step host work in mpc_controller.py:294-309 and 240-242, done on
  * the device): one trapezoidal step of (d_x_last [nx], d_u_last [nu]) under each problem's wrench
  * (gato_set_f_ext_batch), err_b = |x_next_b - d_x_curr|_2 into d_errors [B] (device, may be NULL), best =
- * argmin (first index on ties) into *h_best (host, may be NULL); then every row of d_xu [B * traj] is
+ * argmin in numpy.argmin's order (the first NaN if any, else the first minimum) into *h_best (host, may be
+ * NULL); then every row of d_xu [B * traj] is
  * overwritten with row best (the next solve's warm start). */
 int gato_select_hypothesis(gato_solver* s, const float* d_x_last, const float* d_u_last, const float* d_x_curr,
                            float dt, float* d_xu, float* d_errors, int32_t* h_best, void* stream);
@@ -210,7 +211,8 @@ int gato_force_est_reset(gato_force_est* e, float radius, void* stream);
 int gato_force_est_generate(gato_force_est* e, gato_solver* s, const float* d_q, float* d_world, void* stream);
 /* ForceEstimator.update (force_estimator.py:99-136) on the device for the winning hypothesis *d_best (NULL: the
  * solver's last gato_select_hypothesis result) with that call's prediction errors d_errors [B]; h_uniforms: three
- * numbers in [0, 1] for the next exploration rotation (NULL: the estimator's own generator). Asynchronous. */
+ * numbers in [0, 1] for the next exploration rotation (NULL: the estimator's own generator). An index outside
+ * [0, B) leaves the estimator unchanged. Asynchronous. */
 int gato_force_est_update(gato_force_est* e, gato_solver* s, const int32_t* d_best, const float* d_errors, float alpha,
                           float beta, const double* h_uniforms, void* stream);
 /* Copy of the state (GATO_FORCE_EST_STATE_LEN doubles, layout above). Synchronous. */

@@ -315,7 +315,7 @@ def test_full_size_invariants(gpu):
 
 @pytest.mark.parametrize("plant", ["iiwa14", "indy7"])
 def test_select_hypothesis_matches_host_selection(gpu, plant):
-    # the MPC caller's host work (mpc_controller.py:268-282): simulate every wrench hypothesis, take the
+    # the MPC caller's host work (mpc_controller.py:294-309): simulate every wrench hypothesis, take the
     # argmin of |x_next - x_curr|, warm-start every problem from that problem's trajectory
     import torch
 
@@ -338,6 +338,17 @@ def test_select_hypothesis_matches_host_selection(gpu, plant):
     np.testing.assert_allclose(err, expect, rtol=1e-4, atol=1e-6)
     assert best == int(np.argmin(expect)) == 5
     assert torch.equal(xu, row5.expand(B, -1))
+    # a hypothesis whose prediction is NaN wins as numpy.argmin lets it (first NaN), and the index stays in
+    # the batch; with every error NaN the first hypothesis wins
+    fe[9] = np.nan
+    fe[12] = np.nan
+    s.set_f_ext_batch(fe)
+    row9 = xu[9].clone()
+    best, err = s.select_hypothesis(x, u, x_curr, 0.01, xu)
+    assert np.isnan(err[9]) and best == 9 == int(np.argmin(err))
+    assert torch.equal(xu, row9.expand(B, -1))
+    best, err = s.select_hypothesis(x, u, np.full(2 * n, np.nan, np.float32), 0.01, xu)
+    assert np.isnan(err).all() and best == 0 == int(np.argmin(err))
 
 
 @pytest.mark.parametrize("plant", ["iiwa14", "indy7"])
