@@ -407,7 +407,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (problems are independent; "
-                                   "sharding is described in DESIGN.md)"}))
+                                   "a multi-GPU run would split the batch across devices with no exchange during a solve)"}))
         sys.exit(2)
     import torch
 

@@ -240,7 +240,7 @@ def sim_forward(plant, x, u, fext, dt):
 # where [dz; lambda] solves the step's KKT system K [dz; lambda] = [-g; c] (G~ = blkdiag(Q~_k, R_k) with the
 # half-diagonal rho of linalg.cuh:85-96, C from the integrator Jacobians), for theta = (x_s, reference, diag Q,
 # diag R). At a converged solution dz = 0 and this is the implicit derivative of the solution under the
-# solver's own Hessian model. (No counterpart in the reference: the north-star adjoint.)
+# solver's own Hessian model. (No counterpart in the reference: the adjoint for differentiable MPC.)
 
 
 def dense_kkt(kkt, rho, nq):

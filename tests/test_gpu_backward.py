@@ -3,7 +3,7 @@
 The oracle's step_grad is pinned by central differences of the oracle's step map and by the implicit
 derivative at a fixed point (tests/test_oracle_adjoint.py). Here the HIP backward, called through the
 C-ABI, must reproduce it at the GPU's own solution, rho and weights: 1e-3 relative to each gradient's
-largest entry (the north-star bar for adjoint gradients).
+largest entry (the tolerance set for adjoint gradients against finite differences).
 """
 import numpy as np
 import pytest

@@ -1,6 +1,6 @@
 """End-to-end GPU parity and behaviour of the batched solve, against the FP64 oracle.
 
-Trajectory tolerance: 1e-4 relative where the iteration is well conditioned
+Trajectory tolerance: 1e-4 relative (the parity bar for trajectories) where the iteration is well conditioned
 (joint-space tracking, one SQP iteration of the figure-8 case). Beyond that the figure-8 case is
 chaotic in floating point - its PCG stops at the iteration cap every time and fp32/fp64 line-search
 accept/reject decisions flip after a few iterations - so multi-iteration runs there are checked
@@ -299,7 +299,7 @@ def test_kkt_exit_rule(gpu, mode):
 
 
 def test_full_size_invariants(gpu):
-    # BASELINE workload at full size: iiwa14, N=32, B=512, <= 10 SQP iterations
+    # the bench workload at full size: iiwa14, N=32, B=512, <= 10 SQP iterations
     wl = W.drop_in_figure8("iiwa14", N=32, B=512, seed=1)
     s = make(wl, "fused")
     r = s.solve(wl.xu, wl.dt, wl.xs, wl.ref)
