@@ -156,9 +156,10 @@ template <int PL>
 struct Kin {
     static constexpr int NQ = Plant<PL>::NQ;
     float s[NQ], c[NQ];
+    // the hardware sine and cosine, as the reference's fast-math build evaluates them (-use_fast_math)
     __device__ __forceinline__ void set(const float* q) {
 #pragma unroll
-        for (int i = 0; i < NQ; i++) sincosf(q[i], &s[i], &c[i]);
+        for (int i = 0; i < NQ; i++) __sincosf(q[i], &s[i], &c[i]);
     }
 };
 
@@ -332,14 +333,14 @@ __device__ __forceinline__ void chol_solve(const float* L, float* b) {
         float v = b[i];
 #pragma unroll
         for (int k = 0; k < i; k++) v -= L[tri(i, k)] * b[k];
-        b[i] = v / L[tri(i, i)];
+        b[i] = v * __builtin_amdgcn_rcpf(L[tri(i, i)]);  // fast-math quotient, as the reference divides
     }
 #pragma unroll
     for (int i = N - 1; i >= 0; i--) {
         float v = b[i];
 #pragma unroll
         for (int k = i + 1; k < N; k++) v -= L[tri(k, i)] * b[k];
-        b[i] = v / L[tri(i, i)];
+        b[i] = v * __builtin_amdgcn_rcpf(L[tri(i, i)]);
     }
 }
 

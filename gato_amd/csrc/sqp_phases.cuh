@@ -56,14 +56,14 @@ __device__ __forceinline__ float barrier_g(float v, float lo, float hi) {
         a = a >= 0.f ? (a < eps ? eps : a) : (a > -eps ? -eps : a);
         b = b >= 0.f ? (b < eps ? eps : b) : (b > -eps ? -eps : b);
     }
-    return -1.f / a + 1.f / b;
+    return -__builtin_amdgcn_rcpf(a) + __builtin_amdgcn_rcpf(b);  // fast-math reciprocals (-use_fast_math)
 }
 // iiwa14 curvature 1/d_lo^2 + 1/d_hi^2 with |d| >= 1e-6 (iiwa14_plant.cuh:141-155)
 __device__ __forceinline__ float barrier_h(float v, float lo, float hi) {
     float a = fabsf(v - lo), b = fabsf(hi - v);
     a = a < 1e-6f ? 1e-6f : a;
     b = b < 1e-6f ? 1e-6f : b;
-    return 1.f / (a * a) + 1.f / (b * b);
+    return __builtin_amdgcn_rcpf(a * a) + __builtin_amdgcn_rcpf(b * b);
 }
 
 template <typename Tarr>
@@ -240,7 +240,7 @@ __device__ __forceinline__ void gj_inplace(int lane, float* row, float* piv) {
 #pragma unroll
     for (int p = 0; p < NX; p++) {
         if (lane == p) {
-            const float inv = 1.f / row[p];
+            const float inv = __builtin_amdgcn_rcpf(row[p]);  // fast-math reciprocal of the pivot
             row[p] = 1.f;
 #pragma unroll
             for (int c = 0; c < NX; c++) {
@@ -272,8 +272,8 @@ __device__ __forceinline__ void invert_q_knot(int lane, bool has_r, float rho, c
     if (lane < NQ) {
 #pragma unroll
         for (int c = 0; c < NQ; c++) Qip[lane * NQ + c] = row[c];
-        Qiv[lane] = 1.f / Qv[lane];
-        if (has_r) Ri[lane] = 1.f / Rg[lane];
+        Qiv[lane] = __builtin_amdgcn_rcpf(Qv[lane]);
+        if (has_r) Ri[lane] = __builtin_amdgcn_rcpf(Rg[lane]);
     }
 }
 
