@@ -315,6 +315,25 @@ def test_full_size_invariants(gpu):
 
 
 @pytest.mark.parametrize("plant", ["iiwa14", "indy7"])
+def test_largest_reference_batch(gpu, O, plant):
+    # B = 1024, the largest batch the reference registers (bindings.cu:244-266): four problems per CU, two
+    # SQP iterations in graph mode; every problem's first iteration is checked
+    # against a plain-mode solve of the same inputs (bitwise: the same kernels) and a sample against the oracle
+    wl = W.drop_in_figure8(plant, N=32, B=1024, seed=21)
+    s = make(wl, "graph", max_sqp_iters=2)
+    r = s.solve(wl.xu, wl.dt, wl.xs, wl.ref)
+    assert np.isfinite(r["XU"]).all()
+    assert np.all(r["final_merit"] <= r["initial_merit"])
+    p = make(wl, "plain", max_sqp_iters=2).solve(wl.xu, wl.dt, wl.xs, wl.ref)
+    np.testing.assert_array_equal(r["XU"], p["XU"])
+    sub = W.drop_in_figure8(plant, N=32, B=1024, seed=21)
+    idx = np.array([0, 511, 1023])
+    sub.xu, sub.xs, sub.ref = sub.xu[idx], sub.xs[idx], sub.ref[idx]
+    o = oracle_solve(O, sub, 1)
+    assert rel_err(r["initial_merit"][idx], o["initial_merit"]) < 1e-5
+
+
+@pytest.mark.parametrize("plant", ["iiwa14", "indy7"])
 def test_select_hypothesis_matches_host_selection(gpu, plant):
     # the MPC caller's host work (mpc_controller.py:294-309): simulate every wrench hypothesis, take the
     # argmin of |x_next - x_curr|, warm-start every problem from that problem's trajectory
