@@ -440,8 +440,8 @@ __device__ __forceinline__ void phase_ls(const Args& a, int b, int it, const flo
 
 // ------------------------------------------------------------------ plain path kernels
 template <int PL>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_merit(Args a, int nal, int which /*0 init, 1 final, 2 line search*/,
-                                               int it) {
+__device__ __forceinline__ void merit_body(const Args& a, int nal, int which /*0 init, 1 final, 2 line search*/,
+                                           int it) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
     if (which == 2 && (stopped(a) || (a.exit_rule == GATO_EXIT_KKT && !a.active[b]))) return;
@@ -451,6 +451,19 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_merit(Args a, int na
         (which == 0 ? a.st_init : a.st_final)[b] = sm[0];
     }
     if (which == 2) phase_ls<PL>(a, b, it, sm, sm + NUM_ALPHAS);
+}
+
+template <int PL>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_merit(Args a, int nal, int which, int it) {
+    merit_body<PL>(a, nal, which, it);
+}
+
+// The same for N * NUM_ALPHAS <= 256: one thread per (step length, knot) item in a 256-thread work-group,
+// which leaves room for 256 VGPRs at two work-groups per CU, so the forward dynamics of an item do not spill.
+template <int PL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_merit_n(Args a, int nal, int which,
+                                                                                         int it) {
+    merit_body<PL>(a, nal, which, it);
 }
 
 template <int PL>
@@ -1323,6 +1336,14 @@ int check_threads(int B) { return std::min(1024, 64 * ((B + 63) / 64)); }
 int npass(const gato_solver* s) { return (s->N + s->G - 1) / s->G; }
 
 template <int PL>
+void launch_merit(int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a, int nal, int which, int it) {
+    if (a.N * gato::NUM_ALPHAS <= 256)
+        hipLaunchKernelGGL((gato::k_merit_n<PL>), dim3(B), dim3(256), lds, st, a, nal, which, it);
+    else
+        hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, nal, which, it);
+}
+
+template <int PL>
 void launch_pcg(int np, int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a, int it) {
     if (np == 1) hipLaunchKernelGGL((gato::k_pcg<PL, 1>), dim3(B), blk, lds, st, a, it);
     else if (np == 2) hipLaunchKernelGGL((gato::k_pcg<PL, 2>), dim3(B), blk, lds, st, a, it);
@@ -1394,17 +1415,17 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     const size_t nz = (size_t)B * (s->traj + s->nu);
     hipLaunchKernelGGL(gato::k_begin, dim3(sb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
-    hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, 1, 0, 0);
+    launch_merit<PL>(B, blk, lds, st, a, 1, 0, 0);
     for (int it = 0; it < (int)s->prm.max_sqp_iters; it++) {
         hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
         hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, a);
         launch_pcg<PL>(np, B, blk, lds, st, a, it);
         hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, a, it);
         hipLaunchKernelGGL(gato::k_check, dim3(1), dim3(check_threads(B)), 0, st, a, it);
-        hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, gato::NUM_ALPHAS, 2, it);
+        launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 2, it);
     }
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
-    hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, 1, 1, 0);
+    launch_merit<PL>(B, blk, lds, st, a, 1, 1, 0);
     hipLaunchKernelGGL(gato::k_end, dim3(sb), dim3(256), 0, st, a);
 }
 
@@ -1473,7 +1494,7 @@ int launch_solve(gato_solver* s, hipStream_t st) {
         const size_t n = (size_t)B * (s->traj + s->nu);
         hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, n);
     }
-    hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, 1, 0, 0);
+    launch_merit<PL>(B, blk, lds, st, a, 1, 0, 0);
     HIP_TRY(hipGetLastError());
     double pcg_ms = 0.0;
     int pcg_launches = 0;
@@ -1495,14 +1516,14 @@ int launch_solve(gato_solver* s, hipStream_t st) {
         pcg_ms += ms;
         pcg_launches++;
         if ((double)solved >= (double)B * s->prm.solve_ratio) { it++; break; }
-        hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, gato::NUM_ALPHAS, 2, it);
+        launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 2, it);
         HIP_TRY(hipGetLastError());
     }
     {
         const size_t n = (size_t)B * (s->traj + s->nu);
         hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, n);
     }
-    hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, 1, 1, 0);
+    launch_merit<PL>(B, blk, lds, st, a, 1, 1, 0);
     hipLaunchKernelGGL(gato::k_end, dim3(sb), dim3(256), 0, st, a);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
@@ -2086,8 +2107,8 @@ int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_
             break;
         case 4:
             // merit terms only: reuse the line-search launch with a large phi so nothing is applied
-            if (ii) hipLaunchKernelGGL(gato::k_merit<0>, dim3(s->B), blk, lds, st, a, gato::NUM_ALPHAS, 3, 0);
-            else hipLaunchKernelGGL(gato::k_merit<1>, dim3(s->B), blk, lds, st, a, gato::NUM_ALPHAS, 3, 0);
+            if (ii) launch_merit<0>(s->B, blk, lds, st, a, gato::NUM_ALPHAS, 3, 0);
+            else launch_merit<1>(s->B, blk, lds, st, a, gato::NUM_ALPHAS, 3, 0);
             break;
     }
     HIP_TRY(hipGetLastError());
