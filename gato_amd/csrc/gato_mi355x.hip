@@ -482,6 +482,16 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_schur(Args a) {
     phase_schur<PL>(a, b, a.rho[b], sm);
 }
 
+// The same in 256-thread work-groups (N <= 32: at most two knots per knot group), with room for 256 VGPRs at two
+// work-groups per CU, so the Schur products do not spill.
+template <int PL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_schur_n(Args a) {
+    extern __shared__ __align__(16) float sm[];
+    const int b = blockIdx.x;
+    if (stopped(a) || (a.exit_rule == GATO_EXIT_KKT && !a.active[b])) return;
+    phase_schur<PL>(a, b, a.rho[b], sm);
+}
+
 template <int PL, int NPASS>
 __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg(Args a, int it) {
     extern __shared__ __align__(16) float sm[];
@@ -1336,6 +1346,14 @@ int check_threads(int B) { return std::min(1024, 64 * ((B + 63) / 64)); }
 int npass(const gato_solver* s) { return (s->N + s->G - 1) / s->G; }
 
 template <int PL>
+void launch_schur(int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a) {
+    if (a.N <= 32 && blk.x >= 256)
+        hipLaunchKernelGGL((gato::k_schur_n<PL>), dim3(B), dim3(256), lds, st, a);
+    else
+        hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, a);
+}
+
+template <int PL>
 void launch_merit(int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a, int nal, int which, int it) {
     if (a.N * gato::NUM_ALPHAS <= 256)
         hipLaunchKernelGGL((gato::k_merit_n<PL>), dim3(B), dim3(256), lds, st, a, nal, which, it);
@@ -1418,7 +1436,7 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     launch_merit<PL>(B, blk, lds, st, a, 1, 0, 0);
     for (int it = 0; it < (int)s->prm.max_sqp_iters; it++) {
         hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
-        hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, a);
+        launch_schur<PL>(B, blk, lds, st, a);
         launch_pcg<PL>(np, B, blk, lds, st, a, it);
         hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, a, it);
         hipLaunchKernelGGL(gato::k_check, dim3(1), dim3(check_threads(B)), 0, st, a, it);
@@ -1501,7 +1519,7 @@ int launch_solve(gato_solver* s, hipStream_t st) {
     int it = 0;
     for (; it < (int)s->prm.max_sqp_iters; it++) {
         hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
-        hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, a);
+        launch_schur<PL>(B, blk, lds, st, a);
         HIP_TRY(hipEventRecord(s->ev0, st));
         launch_pcg<PL>(np, B, blk, lds, st, a, it);
         HIP_TRY(hipEventRecord(s->ev1, st));
@@ -1555,7 +1573,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->dz_v, (size_t)B * ndz);
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->lam_v, (size_t)B * nlam);
     hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, ab);
-    hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, ab);
+    launch_schur<PL>(B, blk, lds, st, ab);
     launch_pcg<PL>(np, B, blk, lds, st, ab, 0);
     hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, ab, 0);
     // correction solves: same blocks, right-hand side (q_a, r_a, c_a) -> gam_a, solution (dz_a, lam_a)
@@ -2094,8 +2112,8 @@ int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_
             else hipLaunchKernelGGL(gato::k_lin<1>, dim3(s->B), blk, lds, st, a);
             break;
         case 1:
-            if (ii) hipLaunchKernelGGL(gato::k_schur<0>, dim3(s->B), blk, lds, st, a);
-            else hipLaunchKernelGGL(gato::k_schur<1>, dim3(s->B), blk, lds, st, a);
+            if (ii) launch_schur<0>(s->B, blk, lds, st, a);
+            else launch_schur<1>(s->B, blk, lds, st, a);
             break;
         case 2:
             if (ii) launch_pcg<0>(np, s->B, blk, lds, st, a, 0);
