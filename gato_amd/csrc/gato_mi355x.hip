@@ -1241,8 +1241,8 @@ struct gato_solver {
           *bst_kres = nullptr;
     int* bst_pcg = nullptr;
     int bwd_max_pcg = 1000;
-    int bwd_refine = 1;
-    float bwd_tol = 1e-12f;
+    int bwd_refine = 2;
+    float bwd_tol = 1e-6f;
     // graph mode: the captured solve, valid for the launch arguments it was captured with
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;

@@ -135,8 +135,8 @@ int gato_sim_forward(gato_solver* s, float* d_xnext, const float* d_x, const flo
 int gato_backward(gato_solver* s, const float* d_xu, const float* d_xs, const float* d_ref, float dt,
                   const float* d_dL_dxu, float* d_grad_xs, float* d_grad_ref, float* d_grad_Qd, float* d_grad_Rd,
                   void* stream, int32_t* h_adj_pcg_iters);
-/* Backward solves: PCG iteration cap (default 1000), relative tolerance on r^T P r (default 1e-12) and
- * rounds of iterative refinement of the step and adjoint solutions (default 1, at most 8). */
+/* Backward solves: PCG iteration cap (default 1000), relative tolerance on r^T P r (default 1e-6) and
+ * rounds of iterative refinement of the step and adjoint solutions (default 2, at most 8). */
 int gato_set_backward_params(gato_solver* s, int32_t max_pcg_iters, float pcg_tol, int32_t refine);
 
 /* Extension: launch shape. `groups` 16-lane knot groups per work-group (a multiple of 4 in [4, 32], at most
