@@ -353,6 +353,162 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
     return it;
 }
 
+// Work-group sums of two values with one barrier, each summed exactly as block_sum1 sums one. red: 64 floats per
+// call, double-buffered by the caller like block_sum1's.
+__device__ __forceinline__ float2 block_sum2(float v1, float v2, float* red) {
+    const int rows = blockDim.x >> 4;
+    v1 = row16_sum(v1);
+    v2 = row16_sum(v2);
+    if ((threadIdx.x & 15) == 0) {
+        red[threadIdx.x >> 4] = v1;
+        red[32 + (threadIdx.x >> 4)] = v2;
+    }
+    __syncthreads();
+    const int l = threadIdx.x & 63;
+    float s1 = l < rows ? red[l] : 0.f, s2 = l < rows ? red[32 + l] : 0.f;
+    s1 = row16_sum(s1);
+    s2 = row16_sum(s2);
+    return make_float2(lane_f(s1, 0) + lane_f(s1, 16), lane_f(s2, 0) + lane_f(s2, 16));
+}
+
+// Two PCG solves on the same S and P in lockstep (the backward pass's step and adjoint chains): the rows of S
+// and P are loaded once, every iteration does both mat-vecs and each pair of dot products shares one barrier.
+// Solve i has its own right-hand side a_i.gam, warm start and result a_i.lam, tolerance a_i.tol[b], absolute
+// tolerance, iteration cap and count, and the iteration of phase_pcg_reg; once it stops (its test or its cap)
+// its vectors are left alone while the other goes on. run_i = false skips solve i (count 0, lambda untouched).
+// One knot per knot group (N <= 32).
+template <int PL>
+__device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, int b, bool run1, bool run2,
+                                               float* sm, int* n1, int* n2) {
+    constexpr int NX = Plant<PL>::NX, XP = 16;
+    View<PL> v1(a1, b), v2(a2, b);
+    const int N = a1.N, tid = threadIdx.x, k = tid / KG, lane = tid % KG;
+    float* sp1 = sm;
+    float* sr1 = sp1 + (N + 2) * XP;
+    float* sp2 = sr1 + (N + 2) * XP;
+    float* sr2 = sp2 + (N + 2) * XP;
+    float* red = sr2 + (N + 2) * XP;  // 2 x 64 floats: two double-buffered pairs of reduction slots
+    for (int i = tid; i < 4 * (N + 2) * XP; i += blockDim.x) sm[i] = 0.f;
+    const bool val = (lane < NX) && (k < N);
+    const int l = lane < NX ? lane : 0;
+    const int kk = k < N ? k : 0;
+    float sd[NX], sl[NX], su[NX], pd[NX], pl[NX], pu[NX];
+    {
+        const float* D = v1.Sd + (size_t)kk * NX * NX + l * NX;
+        const float* L = v1.So + (size_t)kk * NX * NX + l * NX;
+        const float* U = v1.So + (size_t)(kk + 1 < N ? kk + 1 : kk) * NX * NX + l;
+        const float* PD = v1.Pd + (size_t)kk * NX * NX + l * NX;
+        const float* PL_ = v1.Po + (size_t)kk * NX * NX + l * NX;
+        const float* PU = v1.Po + (size_t)(kk + 1 < N ? kk + 1 : kk) * NX * NX + l;
+        const bool hasl = kk > 0, hasu = kk < N - 1;
+#pragma unroll
+        for (int c = 0; c < NX; c++) {
+            sd[c] = D[c];
+            pd[c] = PD[c];
+            sl[c] = hasl ? L[c] : 0.f;
+            pl[c] = hasl ? PL_[c] : 0.f;
+            su[c] = hasu ? U[c * NX] : 0.f;
+            pu[c] = hasu ? PU[c * NX] : 0.f;
+        }
+    }
+    auto blk = [&](const float* M, const float4* xb, float s) {
+        float xs[XP];
+#pragma unroll
+        for (int q = 0; q < XP / 4; q++) {
+            const float4 t = xb[q];
+            xs[4 * q] = t.x; xs[4 * q + 1] = t.y; xs[4 * q + 2] = t.z; xs[4 * q + 3] = t.w;
+        }
+#pragma unroll
+        for (int c = 0; c < NX; c++) s += M[c] * xs[c];
+        return s;
+    };
+    auto mv = [&](const float* M0, const float* M1, const float* M2, const float* x) {
+        const float4* xk = reinterpret_cast<const float4*>(x + (kk + 1) * XP);
+        float s = blk(M0, xk, 0.f);
+        s = blk(M1, xk - XP / 4, s);
+        return blk(M2, xk + XP / 4, s);
+    };
+    const int o = (kk + 1) * XP + l;
+    float x1 = (val && run1) ? v1.lam[kk * NX + l] : 0.f, x2 = (val && run2) ? v2.lam[kk * NX + l] : 0.f;
+    __syncthreads();
+    if (val) {
+        sp1[o] = x1;
+        sp2[o] = x2;
+    }
+    __syncthreads();
+    float r1 = (val && run1) ? v1.gam[kk * NX + l] - mv(sd, sl, su, sp1) : 0.f;
+    float r2 = (val && run2) ? v2.gam[kk * NX + l] - mv(sd, sl, su, sp2) : 0.f;
+    if (val) {
+        sr1[o] = r1;
+        sr2[o] = r2;
+    }
+    __syncthreads();
+    float z1 = (val && run1) ? mv(pd, pl, pu, sr1) : 0.f, z2 = (val && run2) ? mv(pd, pl, pu, sr2) : 0.f;
+    float p1 = z1, p2 = z2;
+    int slot = 0;
+    const float2 rho = block_sum2(r1 * z1, r2 * z2, red + 64 * slot);
+    slot ^= 1;
+    float rho1 = rho.x, rho2 = rho.y;
+    bool act1 = run1 && a1.max_pcg > 0 && !(fabsf(rho1) < a1.pcg_abs_tol);
+    bool act2 = run2 && a2.max_pcg > 0 && !(fabsf(rho2) < a2.pcg_abs_tol);
+    const float lim1 = a1.pcg_abs_tol + a1.tol[b] * fabsf(rho1), lim2 = a2.pcg_abs_tol + a2.tol[b] * fabsf(rho2);
+    int it1 = 0, it2 = 0;
+    while (act1 || act2) {  // act1, act2 come from work-group sums: uniform, so are the branches below
+        if (act1) it1++;
+        if (act2) it2++;
+        if (val) {
+            if (act1) sp1[o] = p1;
+            if (act2) sp2[o] = p2;
+        }
+        __syncthreads();
+        const float Ap1 = (val && act1) ? mv(sd, sl, su, sp1) : 0.f;
+        const float Ap2 = (val && act2) ? mv(sd, sl, su, sp2) : 0.f;
+        const float2 pap = block_sum2(p1 * Ap1, p2 * Ap2, red + 64 * slot);
+        slot ^= 1;
+        if (act1) {
+            const float alpha = rho1 * __builtin_amdgcn_rcpf(pap.x);
+            x1 += alpha * p1;
+            r1 -= alpha * Ap1;
+            if (val) sr1[o] = r1;
+        }
+        if (act2) {
+            const float alpha = rho2 * __builtin_amdgcn_rcpf(pap.y);
+            x2 += alpha * p2;
+            r2 -= alpha * Ap2;
+            if (val) sr2[o] = r2;
+        }
+        __syncthreads();
+        z1 = (val && act1) ? mv(pd, pl, pu, sr1) : 0.f;
+        z2 = (val && act2) ? mv(pd, pl, pu, sr2) : 0.f;
+        const float2 rz = block_sum2(r1 * z1, r2 * z2, red + 64 * slot);
+        slot ^= 1;
+        if (act1) {
+            if (fabsf(rz.x) < lim1) {
+                act1 = false;
+            } else {
+                const float beta = rz.x * __builtin_amdgcn_rcpf(rho1);
+                rho1 = rz.x;
+                p1 = z1 + beta * p1;
+                act1 = it1 < a1.max_pcg;
+            }
+        }
+        if (act2) {
+            if (fabsf(rz.y) < lim2) {
+                act2 = false;
+            } else {
+                const float beta = rz.y * __builtin_amdgcn_rcpf(rho2);
+                rho2 = rz.y;
+                p2 = z2 + beta * p2;
+                act2 = it2 < a2.max_pcg;
+            }
+        }
+    }
+    if (val && run1) v1.lam[kk * NX + l] = x1;
+    if (val && run2) v2.lam[kk * NX + l] = x2;
+    *n1 = it1;
+    *n2 = it2;
+}
+
 // register-resident PCG when every knot group owns a single knot (N <= 32), else S and P stream
 template <int PL, int NPASS>
 __device__ __forceinline__ int pcg_dispatch(const Args& a, int b, float eps, float* sm) {
@@ -501,6 +657,22 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg(Args a, int it) 
     int n = 0;
     if (!skip) n = pcg_dispatch<PL, NPASS>(a, b, a.tol[b], sm);
     if (threadIdx.x == 0) a.st_pcg[it * a.B + b] = n;
+}
+
+// Two PCG solves on one problem's S and P in lockstep (phase_pcg_reg2), each with the k_pcg skip rule (quirk Q9);
+// iteration counts to a1.st_pcg[b] and a2.st_pcg[b] when those are set.
+template <int PL>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_dual(Args a1, Args a2) {
+    extern __shared__ __align__(16) float sm[];
+    const int b = blockIdx.x;
+    const bool run1 = !((a1.exit_rule == GATO_EXIT_KKT) ? !a1.active[b] : a1.conv[b]);
+    const bool run2 = !((a2.exit_rule == GATO_EXIT_KKT) ? !a2.active[b] : a2.conv[b]);
+    int n1 = 0, n2 = 0;
+    phase_pcg_reg2<PL>(a1, a2, b, run1, run2, sm, &n1, &n2);
+    if (threadIdx.x == 0) {
+        if (a1.st_pcg) a1.st_pcg[b] = n1;
+        if (a2.st_pcg) a2.st_pcg[b] = n2;
+    }
 }
 
 template <int PL>
@@ -1234,12 +1406,14 @@ struct gato_solver {
     int gkey_G = 0;
     float *lam_save = nullptr, *xu_tune = nullptr;  // launch-shape tuner: saved multipliers, scratch trajectory
     hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
-    // backward workspace: the step at the solution (lam_b, dz_b) and the adjoint system (q_a, r_a, gam_a,
-    // lam_a, dz_a), the adjoint PCG settings and its iteration counts
+    // backward workspace: the step at the solution (lam_b, dz_b) and its correction solves (q_a, r_a, c_a,
+    // gam_a, lam_a, dz_a); the adjoint solution (dz_v, lam_v) and its correction solves (q_w, r_w, c_w, gam_w,
+    // lam_w, dz_w); the backward PCG settings and the iteration counts of the two chains
     float *lam_b = nullptr, *dz_b = nullptr, *q_a = nullptr, *r_a = nullptr, *c_a = nullptr, *gam_a = nullptr,
           *lam_a = nullptr, *dz_a = nullptr, *dz_v = nullptr, *lam_v = nullptr, *d_bwd_tol = nullptr,
-          *bst_kres = nullptr;
-    int* bst_pcg = nullptr;
+          *bst_kres = nullptr, *q_w = nullptr, *r_w = nullptr, *c_w = nullptr, *gam_w = nullptr, *lam_w = nullptr,
+          *dz_w = nullptr;
+    int *bst_pcg = nullptr, *bst_pcg_s = nullptr;
     int bwd_max_pcg = 1000;
     int bwd_refine = 2;
     float bwd_tol = 1e-6f;
@@ -1327,6 +1501,13 @@ size_t layout(gato_solver* s, bool assign) {
     t = F(B); if (assign) s->d_bwd_tol = t;
     t = F(B); if (assign) s->bst_kres = t;
     ip = I(B); if (assign) s->bst_pcg = ip;
+    ip = I(B); if (assign) s->bst_pcg_s = ip;
+    t = F(B * N * nx); if (assign) s->q_w = t;
+    t = F(B * N * nu); if (assign) s->r_w = t;
+    t = F(B * N * nx); if (assign) s->c_w = t;
+    t = F(B * N * nx); if (assign) s->gam_w = t;
+    t = F(B * N * nx); if (assign) s->lam_w = t;
+    t = F(B * (s->traj + nu)); if (assign) s->dz_w = t;
     t = F(B); if (assign) a.st_init = t;
     t = F(B); if (assign) a.st_final = t;
     return off + 256;
@@ -1558,7 +1739,9 @@ template <int PL>
 int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, float* gQ, float* gR, hipStream_t st) {
     const int B = s->B, N = s->N, np = npass(s);
     const dim3 blk(grid_threads(s));
-    const size_t lds = std::max(lds_bytes(s), (size_t)2 * N * gato::KG * sizeof(float));
+    // (the dual PCG keeps four padded vectors and two pairs of reduction slots in LDS)
+    const size_t lds = std::max({lds_bytes(s), (size_t)2 * N * gato::KG * sizeof(float),
+                                 (size_t)(4 * (N + 2) * 16 + 128) * sizeof(float)});
     const int nlam = N * s->nx, ndz = s->traj + s->nu;
     gato::Args ab = s->a;  // the step at the solution, on its own multipliers and dz
     ab.exit_rule = GATO_EXIT_PCG0;
@@ -1572,11 +1755,10 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
                        s->dz_a, nlam, ndz);
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->dz_v, (size_t)B * ndz);
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->lam_v, (size_t)B * nlam);
+    hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->lam_w, (size_t)B * nlam);
     hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, ab);
     launch_schur<PL>(B, blk, lds, st, ab);
-    launch_pcg<PL>(np, B, blk, lds, st, ab, 0);
-    hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, ab, 0);
-    // correction solves: same blocks, right-hand side (q_a, r_a, c_a) -> gam_a, solution (dz_a, lam_a)
+    // correction solves of the step: same blocks, right-hand side (q_a, r_a, c_a) -> gam_a, solution (dz_a, lam_a)
     gato::Args ac = ab;
     ac.q = s->q_a;
     ac.r = s->r_a;
@@ -1586,17 +1768,43 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     ac.max_pcg = s->bwd_max_pcg;
     ac.tol = s->d_bwd_tol;
     ac.pcg_abs_tol = 1e-30f;
-    auto correct = [&](const float* rz, const float* gq, const float* gr, const float* cc, float* z, float* l) {
-        hipLaunchKernelGGL((gato::k_kkt_resid<PL>), dim3(B), blk, lds, st, ab, rz, gq, gr, cc, z, l, s->q_a, s->r_a,
-                           s->c_a);
-        hipLaunchKernelGGL((gato::k_gamma_rhs<PL>), dim3(B), blk, lds, st, ab, s->q_a, s->r_a, s->c_a, s->gam_a);
-        launch_pcg<PL>(np, B, blk, lds, st, ac, 0);
-        hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, ac, 0);
-        hipLaunchKernelGGL(gato::k_accumulate, dim3(B), dim3(256), 0, st, z, s->dz_a, l, s->lam_a, ndz, nlam);
+    ac.st_pcg = s->bst_pcg_s;
+    ab.st_pcg = s->bst_pcg_s;
+    // solves of the adjoint K [v; v_l] = [dL/dz; 0] (from zero, then corrections): (q_w, r_w, c_w) -> gam_w,
+    // solution (dz_w, lam_w), accumulated into (dz_v, lam_v)
+    gato::Args aw = ac;
+    aw.q = s->q_w;
+    aw.r = s->r_w;
+    aw.gam = s->gam_w;
+    aw.lam = s->lam_w;
+    aw.dz = s->dz_w;
+    aw.st_pcg = s->bst_pcg;
+    auto residual = [&](const float* rz, const float* gq, const float* gr, const float* cc, const float* z,
+                        const float* l, float* q, float* r, float* c, float* gam) {
+        hipLaunchKernelGGL((gato::k_kkt_resid<PL>), dim3(B), blk, lds, st, ab, rz, gq, gr, cc, z, l, q, r, c);
+        hipLaunchKernelGGL((gato::k_gamma_rhs<PL>), dim3(B), blk, lds, st, ab, q, r, c, gam);
     };
-    for (int i = 0; i < s->bwd_refine; i++) correct(nullptr, s->a.q, s->a.r, s->a.c, s->dz_b, s->lam_b);
-    // adjoint: K [v; v_l] = [dL/dz; 0], from zero
-    for (int i = 0; i <= s->bwd_refine; i++) correct(dL, nullptr, nullptr, nullptr, s->dz_v, s->lam_v);
+    // the two chains are independent and share K's Schur complement and preconditioner: round i solves the
+    // step's correction i (round 0: the step itself) and the adjoint's correction i side by side
+    const bool dual = (np == 1);
+    for (int i = 0; i <= s->bwd_refine; i++) {
+        const gato::Args& as = i == 0 ? ab : ac;
+        if (i > 0) residual(nullptr, s->a.q, s->a.r, s->a.c, s->dz_b, s->lam_b, s->q_a, s->r_a, s->c_a, s->gam_a);
+        residual(dL, nullptr, nullptr, nullptr, s->dz_v, s->lam_v, s->q_w, s->r_w, s->c_w, s->gam_w);
+        if (dual) {
+            hipLaunchKernelGGL((gato::k_pcg_dual<PL>), dim3(B), blk, lds, st, as, aw);
+        } else {
+            launch_pcg<PL>(np, B, blk, lds, st, as, 0);
+            launch_pcg<PL>(np, B, blk, lds, st, aw, 0);
+        }
+        hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, as, 0);
+        if (i > 0)
+            hipLaunchKernelGGL(gato::k_accumulate, dim3(B), dim3(256), 0, st, s->dz_b, s->dz_a, s->lam_b, s->lam_a, ndz,
+                               nlam);
+        hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, aw, 0);
+        hipLaunchKernelGGL(gato::k_accumulate, dim3(B), dim3(256), 0, st, s->dz_v, s->dz_w, s->lam_v, s->lam_w, ndz,
+                           nlam);
+    }
     hipLaunchKernelGGL((gato::k_grad<PL>), dim3(B), blk, lds, st, ab, s->dz_v, s->lam_v, gxs, gref, gQ, gR);
     HIP_TRY(hipGetLastError());
     return 0;
