@@ -1786,14 +1786,13 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     };
     // the two chains are independent and share K's Schur complement and preconditioner: round i solves the
     // step's correction i (round 0: the step itself) and the adjoint's correction i side by side
-    const bool dual = (np == 1);
     for (int i = 0; i <= s->bwd_refine; i++) {
         const gato::Args& as = i == 0 ? ab : ac;
         if (i > 0) residual(nullptr, s->a.q, s->a.r, s->a.c, s->dz_b, s->lam_b, s->q_a, s->r_a, s->c_a, s->gam_a);
         residual(dL, nullptr, nullptr, nullptr, s->dz_v, s->lam_v, s->q_w, s->r_w, s->c_w, s->gam_w);
-        if (dual) {
+        if (np == 1) {
             hipLaunchKernelGGL((gato::k_pcg_dual<PL>), dim3(B), blk, lds, st, as, aw);
-        } else {
+        } else {  // S and P stream: a paired streamed PCG measured slower than two launches (more VALU, spills)
             launch_pcg<PL>(np, B, blk, lds, st, as, 0);
             launch_pcg<PL>(np, B, blk, lds, st, aw, 0);
         }
