@@ -2122,10 +2122,12 @@ int gato_select_hypothesis(gato_solver* s, const float* d_x_last, const float* d
                            s->B, d_errors, d_best);
     hipLaunchKernelGGL(gato::k_broadcast_row, dim3(256), dim3(256), 0, st, d_xu, s->traj, s->B, d_best);
     HIP_TRY(hipGetLastError());
-    int32_t best = 0;
-    HIP_TRY(hipMemcpyAsync(&best, d_best, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    if (h_best) *h_best = best;
+    if (h_best) {  // only a host copy of the index waits for the device
+        int32_t best = 0;
+        HIP_TRY(hipMemcpyAsync(&best, d_best, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        *h_best = best;
+    }
     return 0;
 }
 
@@ -2140,7 +2142,6 @@ int gato_simulate_rk4(gato_solver* s, float* d_x, const float* d_u, const float*
     else
         hipLaunchKernelGGL(gato::k_rk4<1>, dim3(nb), dim3(64), 0, st, d_x, d_u, d_f_ext, h, steps, s->B);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(st));
     return 0;
 }
 

@@ -171,13 +171,14 @@ int gato_mixed_last_time(gato_mixed* m, double* us);
  * (gato_set_f_ext_batch), err_b = |x_next_b - d_x_curr|_2 into d_errors [B] (device, may be NULL), best =
  * argmin in numpy.argmin's order (the first NaN if any, else the first minimum) into *h_best (host, may be
  * NULL); then every row of d_xu [B * traj] is
- * overwritten with row best (the next solve's warm start). */
+ * overwritten with row best (the next solve's warm start). Asynchronous unless h_best is given. */
 int gato_select_hypothesis(gato_solver* s, const float* d_x_last, const float* d_u_last, const float* d_x_curr,
                            float dt, float* d_xu, float* d_errors, int32_t* h_best, void* stream);
 
 /* Plant simulation for MPC loops (the caller's host integration, common.py:49-91, on the device): `steps` RK4
  * sub-steps of length h of every problem's state d_x [B * nx] (in place) under its control d_u [B * nu] and
- * wrench d_f_ext [B * 6] (device pointers; the wrench here is the "true" one, independent of the solver's). */
+ * wrench d_f_ext [B * 6] (device pointers; the wrench here is the "true" one, independent of the solver's).
+ * Asynchronous. */
 int gato_simulate_rk4(gato_solver* s, float* d_x, const float* d_u, const float* d_f_ext, float h, int32_t steps,
                       void* stream);
 /* Reference window for every problem from a device-resident path d_path [npoints * 6]:
