@@ -688,6 +688,26 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_dz(Args a, int it) {
     if (threadIdx.x == 0) a.st_kres[it * a.B + b] = kr;
 }
 
+// k_pcg followed by k_dz in one launch (the graph path's step): after the problem's PCG has written lambda, the
+// same work-group computes dz and the KKT residual from it, saving a launch and the reload of the batch.
+template <int PL, int NPASS>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_dz(Args a, int it) {
+    extern __shared__ __align__(16) float sm[];
+    const int b = blockIdx.x;
+    if (stopped(a)) return;
+    const bool skip = (a.exit_rule == GATO_EXIT_KKT) ? !a.active[b] : a.conv[b];  // quirk Q9 (pcg.cuh:30-33)
+    int n = 0;
+    if (!skip) n = pcg_dispatch<PL, NPASS>(a, b, a.tol[b], sm);
+    if (threadIdx.x == 0) a.st_pcg[it * a.B + b] = n;
+    if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) {
+        if (threadIdx.x == 0) a.st_kres[it * a.B + b] = 0.f;
+        return;
+    }
+    __syncthreads();  // lambda (global) and the LDS of the PCG are settled for the dz phase
+    const float kr = phase_dz<PL>(a, b, sm);
+    if (threadIdx.x == 0) a.st_kres[it * a.B + b] = kr;
+}
+
 // host-side convergence bookkeeping of bsqp.cuh:142-165, per problem, plus the converged count
 // Host-side convergence bookkeeping of bsqp.cuh:142-165 for every problem (one work-group looping over the
 // batch), the converged count, and in graph mode the batch-level exit test (bsqp.cuh:165-167) that raises the
@@ -1551,6 +1571,14 @@ void launch_pcg(int np, int B, dim3 blk, size_t lds, hipStream_t st, const gato:
 }
 
 template <int PL>
+void launch_pcg_dz(int np, int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a, int it) {
+    if (np == 1) hipLaunchKernelGGL((gato::k_pcg_dz<PL, 1>), dim3(B), blk, lds, st, a, it);
+    else if (np == 2) hipLaunchKernelGGL((gato::k_pcg_dz<PL, 2>), dim3(B), blk, lds, st, a, it);
+    else if (np == 4) hipLaunchKernelGGL((gato::k_pcg_dz<PL, 4>), dim3(B), blk, lds, st, a, it);
+    else hipLaunchKernelGGL((gato::k_pcg_dz<PL, 8>), dim3(B), blk, lds, st, a, it);
+}
+
+template <int PL>
 void launch_fused(int np, int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a) {
     if (np == 1) hipLaunchKernelGGL((gato::k_fused<PL, 1>), dim3(B), blk, lds, st, a);
     else if (np == 2) hipLaunchKernelGGL((gato::k_fused<PL, 2>), dim3(B), blk, lds, st, a);
@@ -1618,8 +1646,7 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     for (int it = 0; it < (int)s->prm.max_sqp_iters; it++) {
         hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
         launch_schur<PL>(B, blk, lds, st, a);
-        launch_pcg<PL>(np, B, blk, lds, st, a, it);
-        hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, a, it);
+        launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
         hipLaunchKernelGGL(gato::k_check, dim3(1), dim3(check_threads(B)), 0, st, a, it);
         launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 2, it);
     }
