@@ -595,18 +595,25 @@ __device__ __forceinline__ void phase_ls(const Args& a, int b, int it, const flo
 }
 
 // ------------------------------------------------------------------ plain path kernels
+// which: 0 initial merit, 1 final merit, 2 line search, 3 line search preceded by the batch-level exit test
+// (bsqp.cuh:165-167) on the converged count solved[it] that k_pcg_dz accumulated: every work-group makes the
+// same decision from it, and work-group 0 raises the device stop flag for the launches that follow.
 template <int PL>
-__device__ __forceinline__ void merit_body(const Args& a, int nal, int which /*0 init, 1 final, 2 line search*/,
-                                           int it) {
+__device__ __forceinline__ void merit_body(const Args& a, int nal, int which, int it) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
-    if (which == 2 && (stopped(a) || (a.exit_rule == GATO_EXIT_KKT && !a.active[b]))) return;
+    if (which >= 2 && stopped(a)) return;
+    if (which == 3 && (double)a.solved[it] >= (double)a.B * (double)a.solve_ratio) {
+        if (b == 0 && threadIdx.x == 0) *a.stop_w = 1;
+        return;
+    }
+    if (which >= 2 && a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
     phase_merit<PL>(a, b, a.mu[b], nal, sm);
     if (threadIdx.x == 0 && which < 2) {
         a.phi[b] = sm[0];
         (which == 0 ? a.st_init : a.st_final)[b] = sm[0];
     }
-    if (which == 2) phase_ls<PL>(a, b, it, sm, sm + NUM_ALPHAS);
+    if (which >= 2) phase_ls<PL>(a, b, it, sm, sm + NUM_ALPHAS);
 }
 
 template <int PL>
@@ -688,52 +695,61 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_dz(Args a, int it) {
     if (threadIdx.x == 0) a.st_kres[it * a.B + b] = kr;
 }
 
-// k_pcg followed by k_dz in one launch (the graph path's step): after the problem's PCG has written lambda, the
-// same work-group computes dz and the KKT residual from it, saving a launch and the reload of the batch.
+// Host-side convergence bookkeeping of bsqp.cuh:142-165 for problem b after SQP iteration it, given its PCG
+// iteration count and KKT residual: SQP count, converged and active flags, and the line-search record of a
+// frozen problem. Returns the converged flag.
+__device__ __forceinline__ int check_problem(const Args& a, int b, int it, int npcg, float kres) {
+    int conv = a.conv[b];
+    if (a.exit_rule == GATO_EXIT_PCG0) {
+        if (npcg == 0) conv = 1;
+        a.st_sqp[b] += 1;  // quirk Q10: batch-uniform count
+    } else if (a.active[b]) {
+        a.st_sqp[b] += 1;
+        if (kres <= a.kkt_tol) { conv = 1; a.active[b] = 0; }
+    }
+    if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) {  // frozen: no line search this iteration
+        a.st_step[it * a.B + b] = 0.f;
+        a.st_lsm[it * a.B + b] = a.phi[b];
+    }
+    a.conv[b] = conv;
+    return conv;
+}
+
+// k_pcg, k_dz and the per-problem part of k_check in one launch (the graph path's step): after the problem's PCG
+// has written lambda, the same work-group computes dz and the KKT residual from it and does the problem's
+// convergence bookkeeping, adding it to the converged count solved[it] (zero at the start of the solve). The
+// batch-level exit test on that count runs at the start of the line search (merit_body, which == 3).
 template <int PL, int NPASS>
 __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_dz(Args a, int it) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
     if (stopped(a)) return;
     const bool skip = (a.exit_rule == GATO_EXIT_KKT) ? !a.active[b] : a.conv[b];  // quirk Q9 (pcg.cuh:30-33)
+    const bool frozen = a.exit_rule == GATO_EXIT_KKT && !a.active[b];
     int n = 0;
     if (!skip) n = pcg_dispatch<PL, NPASS>(a, b, a.tol[b], sm);
-    if (threadIdx.x == 0) a.st_pcg[it * a.B + b] = n;
-    if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) {
-        if (threadIdx.x == 0) a.st_kres[it * a.B + b] = 0.f;
-        return;
+    float kr = 0.f;
+    if (!frozen) {
+        __syncthreads();  // lambda (global) and the LDS of the PCG are settled for the dz phase
+        kr = phase_dz<PL>(a, b, sm);
     }
-    __syncthreads();  // lambda (global) and the LDS of the PCG are settled for the dz phase
-    const float kr = phase_dz<PL>(a, b, sm);
-    if (threadIdx.x == 0) a.st_kres[it * a.B + b] = kr;
+    if (threadIdx.x == 0) {
+        a.st_pcg[it * a.B + b] = n;
+        a.st_kres[it * a.B + b] = kr;
+        if (check_problem(a, b, it, n, kr)) atomicAdd(&a.solved[it], 1);
+    }
 }
 
-// host-side convergence bookkeeping of bsqp.cuh:142-165, per problem, plus the converged count
-// Host-side convergence bookkeeping of bsqp.cuh:142-165 for every problem (one work-group looping over the
-// batch), the converged count, and in graph mode the batch-level exit test (bsqp.cuh:165-167) that raises the
-// device stop flag.
+// The convergence bookkeeping (check_problem) for every problem of the batch in one work-group, the converged
+// count, and, when a device stop flag is set, the batch-level exit test (bsqp.cuh:165-167) that raises it.
 __global__ void __launch_bounds__(1024) k_check(Args a, int it) {
     __shared__ int count;
     if (stopped(a)) return;
     if (threadIdx.x == 0) count = 0;
     __syncthreads();
     int mine = 0;
-    for (int b = threadIdx.x; b < a.B; b += blockDim.x) {
-        int conv = a.conv[b];
-        if (a.exit_rule == GATO_EXIT_PCG0) {
-            if (a.st_pcg[it * a.B + b] == 0) conv = 1;
-            a.st_sqp[b] += 1;  // quirk Q10: batch-uniform count
-        } else if (a.active[b]) {
-            a.st_sqp[b] += 1;
-            if (a.st_kres[it * a.B + b] <= a.kkt_tol) { conv = 1; a.active[b] = 0; }
-        }
-        if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) {  // frozen: no line search this iteration
-            a.st_step[it * a.B + b] = 0.f;
-            a.st_lsm[it * a.B + b] = a.phi[b];
-        }
-        a.conv[b] = conv;
-        mine += conv;
-    }
+    for (int b = threadIdx.x; b < a.B; b += blockDim.x)
+        mine += check_problem(a, b, it, a.st_pcg[it * a.B + b], a.st_kres[it * a.B + b]);
     if (mine) atomicAdd(&count, mine);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1629,8 +1645,9 @@ void fill_args(gato_solver* s, float* d_xu, const float* d_xs, const float* d_re
     a.stop_w = nullptr;
 }
 
-// Every SQP iteration of the plain path enqueued without a host round trip: k_check raises the device
-// stop flag when the batch-level exit fires and the remaining launches return at once.
+// Every SQP iteration of the plain path enqueued without a host round trip, three launches per iteration after
+// the linearization: Schur, PCG with dz and the convergence bookkeeping, and the line search, whose batch-level
+// exit test raises the device stop flag so that the remaining launches return at once.
 template <int PL>
 void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     const int B = s->B, N = s->N;
@@ -1647,8 +1664,7 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
         hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
         launch_schur<PL>(B, blk, lds, st, a);
         launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
-        hipLaunchKernelGGL(gato::k_check, dim3(1), dim3(check_threads(B)), 0, st, a, it);
-        launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 2, it);
+        launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 3, it);
     }
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
     launch_merit<PL>(B, blk, lds, st, a, 1, 1, 0);
