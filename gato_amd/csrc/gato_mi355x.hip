@@ -629,6 +629,32 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     merit_body<PL>(a, nal, which, it);
 }
 
+// The graph path's iteration head for N <= 32 in 256-thread work-groups (room for 256 VGPRs at two work-groups
+// per CU, as k_merit_n and k_schur_n): when it_ls >= 0, the batch-level exit test and line search of SQP iteration
+// it_ls (merit_body, which == 3); then, when it >= 0, the linearization and Schur phase of iteration it on the
+// trajectory the line search has just updated. Every phase is that of one problem, so one launch replaces three.
+template <int PL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_ls_lin_schur_n(Args a, int it_ls,
+                                                                                                int it) {
+    extern __shared__ __align__(16) float sm[];
+    const int b = blockIdx.x;
+    if (stopped(a)) return;
+    if (it_ls >= 0 && (double)a.solved[it_ls] >= (double)a.B * (double)a.solve_ratio) {
+        if (b == 0 && threadIdx.x == 0) *a.stop_w = 1;
+        return;
+    }
+    if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
+    if (it_ls >= 0) {
+        phase_merit<PL>(a, b, a.mu[b], NUM_ALPHAS, sm);
+        phase_ls<PL>(a, b, it_ls, sm, sm + NUM_ALPHAS);
+    }
+    if (it < 0) return;
+    __syncthreads();  // the updated trajectory and rho are settled, and the LDS is free for the linearization
+    phase_lin<PL>(a, b, sm);
+    __syncthreads();
+    phase_schur<PL>(a, b, a.rho[b], sm);
+}
+
 template <int PL>
 __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_lin(Args a) {
     extern __shared__ __align__(16) float sm[];
@@ -1645,9 +1671,11 @@ void fill_args(gato_solver* s, float* d_xu, const float* d_xs, const float* d_re
     a.stop_w = nullptr;
 }
 
-// Every SQP iteration of the plain path enqueued without a host round trip, three launches per iteration after
-// the linearization: Schur, PCG with dz and the convergence bookkeeping, and the line search, whose batch-level
-// exit test raises the device stop flag so that the remaining launches return at once.
+// Every SQP iteration of the plain path enqueued without a host round trip: PCG with dz and the convergence
+// bookkeeping (k_pcg_dz), then the line search, whose batch-level exit test raises the device stop flag so that
+// the remaining launches return at once. For N <= 32 the line search shares a launch with the next iteration's
+// linearization and Schur phase (k_ls_lin_schur_n), two launches per iteration; otherwise those are k_lin and
+// k_schur, four.
 template <int PL>
 void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     const int B = s->B, N = s->N;
@@ -1660,11 +1688,21 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     hipLaunchKernelGGL(gato::k_begin, dim3(sb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
     launch_merit<PL>(B, blk, lds, st, a, 1, 0, 0);
-    for (int it = 0; it < (int)s->prm.max_sqp_iters; it++) {
-        hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
-        launch_schur<PL>(B, blk, lds, st, a);
-        launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
-        launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 3, it);
+    const int iters = (int)s->prm.max_sqp_iters;
+    if (N <= 32 && N * gato::NUM_ALPHAS <= 256 && blk.x >= 256) {
+        hipLaunchKernelGGL((gato::k_ls_lin_schur_n<PL>), dim3(B), dim3(256), lds, st, a, -1, 0);
+        for (int it = 0; it < iters; it++) {
+            launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
+            hipLaunchKernelGGL((gato::k_ls_lin_schur_n<PL>), dim3(B), dim3(256), lds, st, a, it,
+                               it + 1 < iters ? it + 1 : -1);
+        }
+    } else {
+        for (int it = 0; it < iters; it++) {
+            hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
+            launch_schur<PL>(B, blk, lds, st, a);
+            launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
+            launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 3, it);
+        }
     }
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
     launch_merit<PL>(B, blk, lds, st, a, 1, 1, 0);
