@@ -629,30 +629,98 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     merit_body<PL>(a, nal, which, it);
 }
 
-// The graph path's iteration head for N <= 32 in 256-thread work-groups (room for 256 VGPRs at two work-groups
-// per CU, as k_merit_n and k_schur_n): when it_ls >= 0, the batch-level exit test and line search of SQP iteration
-// it_ls (merit_body, which == 3); then, when it >= 0, the linearization and Schur phase of iteration it on the
-// trajectory the line search has just updated. Every phase is that of one problem, so one launch replaces three.
+// Solve-start state of the batch (the device stop flag, the converged counts) and of problem b (flags, SQP
+// count, the per-iteration records) for the solve of bsqp.cuh:111-114, with the records this build keeps.
+__device__ __forceinline__ void begin_batch(const Args& a) {
+    if (a.stop_w) *a.stop_w = 0;
+    for (int i = 0; i < a.max_sqp; i++) a.solved[i] = 0;
+}
+
+__device__ __forceinline__ void begin_problem(const Args& a, int b) {
+    a.conv[b] = 0;
+    a.active[b] = 1;
+    a.st_sqp[b] = 0;
+    for (int i = 0; i < a.max_sqp; i++) {
+        a.st_pcg[i * a.B + b] = 0;
+        a.st_kres[i * a.B + b] = 0.f;
+        a.st_lsm[i * a.B + b] = 0.f;
+        a.st_step[i * a.B + b] = 0.f;
+    }
+}
+
+// The graph path for N <= 32 runs a solve as k_first_n, then per SQP iteration k_pcg_dz and either
+// k_ls_lin_schur_n or, after the last, k_last_n: every launch does one problem's consecutive phases in one
+// 256-thread work-group (room for 256 VGPRs at two work-groups per CU, as k_merit_n and k_schur_n), so the
+// whole solve takes 2 * iterations + 1 launches.
+
+// Solve start (k_begin, the clear of dz and the initial merit of bsqp.cuh:111-118 for problem b), then the
+// linearization and Schur phase of SQP iteration 0.
 template <int PL>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_ls_lin_schur_n(Args a, int it_ls,
-                                                                                                int it) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_first_n(Args a) {
+    extern __shared__ __align__(16) float sm[];
+    constexpr int NU = Plant<PL>::NU;
+    const int b = blockIdx.x;
+    View<PL> v(a, b);
+    if (threadIdx.x == 0) {
+        if (b == 0) begin_batch(a);
+        begin_problem(a, b);
+    }
+    for (int i = threadIdx.x; i < a.traj + NU; i += blockDim.x) v.dz[i] = 0.f;
+    __syncthreads();
+    phase_merit<PL>(a, b, a.mu[b], 1, sm);
+    if (threadIdx.x == 0) {
+        a.phi[b] = sm[0];
+        a.st_init[b] = sm[0];
+    }
+    __syncthreads();
+    phase_lin<PL>(a, b, sm);
+    __syncthreads();
+    phase_schur<PL>(a, b, a.rho[b], sm);
+}
+
+// The batch-level exit test and line search of SQP iteration it - 1 (merit_body, which == 3), then the
+// linearization and Schur phase of iteration it on the trajectory the line search has just updated.
+template <int PL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_ls_lin_schur_n(Args a, int it) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
     if (stopped(a)) return;
-    if (it_ls >= 0 && (double)a.solved[it_ls] >= (double)a.B * (double)a.solve_ratio) {
+    if ((double)a.solved[it - 1] >= (double)a.B * (double)a.solve_ratio) {
         if (b == 0 && threadIdx.x == 0) *a.stop_w = 1;
         return;
     }
     if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
-    if (it_ls >= 0) {
-        phase_merit<PL>(a, b, a.mu[b], NUM_ALPHAS, sm);
-        phase_ls<PL>(a, b, it_ls, sm, sm + NUM_ALPHAS);
-    }
-    if (it < 0) return;
+    phase_merit<PL>(a, b, a.mu[b], NUM_ALPHAS, sm);
+    phase_ls<PL>(a, b, it - 1, sm, sm + NUM_ALPHAS);
     __syncthreads();  // the updated trajectory and rho are settled, and the LDS is free for the linearization
     phase_lin<PL>(a, b, sm);
     __syncthreads();
     phase_schur<PL>(a, b, a.rho[b], sm);
+}
+
+// The exit test and line search of the last SQP iteration it (skipped when the batch has stopped, as in
+// k_ls_lin_schur_n), then solve end for problem b: the clear of dz, the final merit (bsqp.cuh:179-182) and the
+// reset of the rho multiplier (k_end).
+template <int PL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_last_n(Args a, int it) {
+    extern __shared__ __align__(16) float sm[];
+    constexpr int NU = Plant<PL>::NU;
+    const int b = blockIdx.x;
+    View<PL> v(a, b);
+    if (!stopped(a) && (double)a.solved[it] < (double)a.B * (double)a.solve_ratio &&
+        !(a.exit_rule == GATO_EXIT_KKT && !a.active[b])) {
+        phase_merit<PL>(a, b, a.mu[b], NUM_ALPHAS, sm);
+        phase_ls<PL>(a, b, it, sm, sm + NUM_ALPHAS);
+        __syncthreads();  // the line search has read dz
+    }
+    for (int i = threadIdx.x; i < a.traj + NU; i += blockDim.x) v.dz[i] = 0.f;
+    __syncthreads();
+    phase_merit<PL>(a, b, a.mu[b], 1, sm);
+    if (threadIdx.x == 0) {
+        a.phi[b] = sm[0];
+        a.st_final[b] = sm[0];
+        a.drho[b] = a.drho_init[b];  // bsqp.cuh:189
+    }
 }
 
 template <int PL>
@@ -786,19 +854,8 @@ __global__ void __launch_bounds__(1024) k_check(Args a, int it) {
 
 __global__ void k_begin(Args a) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b == 0 && a.stop_w) *a.stop_w = 0;
-    if (b >= a.B) return;
-    a.conv[b] = 0;
-    a.active[b] = 1;
-    a.st_sqp[b] = 0;
-    if (b == 0)
-        for (int i = 0; i < a.max_sqp; i++) a.solved[i] = 0;
-    for (int i = 0; i < a.max_sqp; i++) {
-        a.st_pcg[i * a.B + b] = 0;
-        a.st_kres[i * a.B + b] = 0.f;
-        a.st_lsm[i * a.B + b] = 0.f;
-        a.st_step[i * a.B + b] = 0.f;
-    }
+    if (b == 0) begin_batch(a);
+    if (b < a.B) begin_problem(a, b);
 }
 
 __global__ void k_zero(float* p, size_t n) {
@@ -1674,8 +1731,8 @@ void fill_args(gato_solver* s, float* d_xu, const float* d_xs, const float* d_re
 // Every SQP iteration of the plain path enqueued without a host round trip: PCG with dz and the convergence
 // bookkeeping (k_pcg_dz), then the line search, whose batch-level exit test raises the device stop flag so that
 // the remaining launches return at once. For N <= 32 the line search shares a launch with the next iteration's
-// linearization and Schur phase (k_ls_lin_schur_n), two launches per iteration; otherwise those are k_lin and
-// k_schur, four.
+// linearization and Schur phase, and the solve start and end with the first and last of those (k_first_n,
+// k_ls_lin_schur_n, k_last_n); otherwise the iteration is k_lin, k_schur, k_pcg_dz and k_merit.
 template <int PL>
 void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     const int B = s->B, N = s->N;
@@ -1685,24 +1742,26 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     const int sb = (B + 255) / 256;
     const int np = npass(s);
     const size_t nz = (size_t)B * (s->traj + s->nu);
+    const int iters = (int)s->prm.max_sqp_iters;
+    if (iters >= 1 && N <= 32 && N * gato::NUM_ALPHAS <= 256 && blk.x >= 256) {
+        hipLaunchKernelGGL((gato::k_first_n<PL>), dim3(B), dim3(256), lds, st, a);
+        for (int it = 0; it < iters; it++) {
+            launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
+            if (it + 1 < iters)
+                hipLaunchKernelGGL((gato::k_ls_lin_schur_n<PL>), dim3(B), dim3(256), lds, st, a, it + 1);
+            else
+                hipLaunchKernelGGL((gato::k_last_n<PL>), dim3(B), dim3(256), lds, st, a, it);
+        }
+        return;
+    }
     hipLaunchKernelGGL(gato::k_begin, dim3(sb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
     launch_merit<PL>(B, blk, lds, st, a, 1, 0, 0);
-    const int iters = (int)s->prm.max_sqp_iters;
-    if (N <= 32 && N * gato::NUM_ALPHAS <= 256 && blk.x >= 256) {
-        hipLaunchKernelGGL((gato::k_ls_lin_schur_n<PL>), dim3(B), dim3(256), lds, st, a, -1, 0);
-        for (int it = 0; it < iters; it++) {
-            launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
-            hipLaunchKernelGGL((gato::k_ls_lin_schur_n<PL>), dim3(B), dim3(256), lds, st, a, it,
-                               it + 1 < iters ? it + 1 : -1);
-        }
-    } else {
-        for (int it = 0; it < iters; it++) {
-            hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
-            launch_schur<PL>(B, blk, lds, st, a);
-            launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
-            launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 3, it);
-        }
+    for (int it = 0; it < iters; it++) {
+        hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
+        launch_schur<PL>(B, blk, lds, st, a);
+        launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
+        launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 3, it);
     }
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
     launch_merit<PL>(B, blk, lds, st, a, 1, 1, 0);

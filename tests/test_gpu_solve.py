@@ -209,6 +209,36 @@ def test_plain_fused_and_graph_paths_agree(gpu):
         np.testing.assert_array_equal(sg.get_state()[0], sp.get_state()[0])
 
 
+@pytest.mark.parametrize("N", [16, 48])  # the graph path's launches for N <= 32, and its four-launch iteration
+def test_graph_equals_plain_when_problems_freeze_and_the_batch_stops(gpu, N):
+    # The graph path makes the exit decisions on the device (convergence bookkeeping inside the PCG launch, batch
+    # exit test at the start of the line search); the plain path makes them on the host. A KKT tolerance at the
+    # median residual of the second iteration freezes about half the batch part-way; a low solve ratio then
+    # stops the batch early, and a PCG cap of 0 iterations stops it in the first iteration under the reference
+    # rule (converged when the PCG took no iteration).
+    wl = W.drop_in_figure8("iiwa14", N=N, B=32, seed=21)
+    probe = make(wl, "plain", max_sqp_iters=5, kkt_tol=0.0, exit_rule="kkt").solve(wl.xu, wl.dt, wl.xs, wl.ref)
+    assert probe["n_iters"] == 5
+    tol = float(np.median(probe["kkt_residual"][1]))
+    cases = [dict(kkt_tol=tol, exit_rule="kkt", solve_ratio=1.0), dict(kkt_tol=tol, exit_rule="kkt", solve_ratio=0.25),
+             dict(max_pcg_iters=0)]
+    keys = ("XU", "final_merit", "initial_merit", "pcg_iters", "ls_step_size", "ls_min_merit", "sqp_iters",
+            "kkt_converged", "kkt_residual", "n_iters")
+    for i, over in enumerate(cases):
+        rp = make(wl, "plain", max_sqp_iters=5, **over).solve(wl.xu, wl.dt, wl.xs, wl.ref)
+        rg = make(wl, "graph", max_sqp_iters=5, **over).solve(wl.xu, wl.dt, wl.xs, wl.ref)
+        for key in keys:
+            np.testing.assert_array_equal(rg[key], rp[key], err_msg=f"{key} case {i}")
+        conv = int(rp["kkt_converged"].sum())
+        if i == 0:  # frozen problems (residual recorded as 0) beside active ones in the third iteration
+            kr = rp["kkt_residual"]
+            assert rp["n_iters"] >= 3 and (kr[2] == 0).any() and (kr[2] != 0).any()
+        elif i == 1:
+            assert conv >= wl.B // 4 and rp["n_iters"] < 5
+        else:
+            assert conv == wl.B and rp["n_iters"] == 1
+
+
 def test_state_persists_and_resets(gpu, O):
     wl = W.drop_in_figure8("iiwa14", N=16, B=4, seed=13)
     s = make(wl, "fused", max_sqp_iters=2)
