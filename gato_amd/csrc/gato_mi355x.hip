@@ -371,15 +371,154 @@ __device__ __forceinline__ float2 block_sum2(float v1, float v2, float* red) {
     return make_float2(lane_f(s1, 0) + lane_f(s1, 16), lane_f(s2, 0) + lane_f(s2, 16));
 }
 
-// Two PCG solves on the same S and P in lockstep (the backward pass's step and adjoint chains): the rows of S
-// and P are loaded once, every iteration does both mat-vecs and each pair of dot products shares one barrier.
-// Solve i has its own right-hand side a_i.gam, warm start and result a_i.lam, tolerance a_i.tol[b], absolute
-// tolerance, iteration cap and count, and the iteration of phase_pcg_reg; once it stops (its test or its cap)
-// its vectors are left alone while the other goes on. run_i = false skips solve i (count 0, lambda untouched).
-// One knot per knot group (N <= 32).
+// The backward pass's preconditioner (no reference counterpart): the block-tridiagonal factorization
+// S = L D L^T of the Schur complement, L unit lower block-bidiagonal with L_{k,k-1} = G_k, D = blkdiag(D_k):
+//   D_0 = S_00,  G_k = S_{k,k-1} D_{k-1}^-1,  D_k = S_kk - G_k S_{k,k-1}^T   (k = 1 .. N-1),
+// kept as G_k (k >= 1) and D_k^-1 (Gauss-Jordan, gj_inplace: S is definite and so are the D_k). M = L D L^T is S up
+// to fp32 rounding, so a PCG preconditioned with it (btd_solve) stops after one or two iterations where the
+// reference's preconditioner P takes 120 to 240 on the same systems at tolerance 1e-6, and reaches the same fp32
+// accuracy. Sequential over the knots: one 16-lane knot group per problem (lane = row), four problems per 64-thread
+// work-group. Gm, Di: B x N blocks of NX x NX floats (row-major), laid out like Sd.
+template <int PL>
+__global__ void __launch_bounds__(64) k_btd_factor(Args a, float* Gm, float* Di) {
+    constexpr int NX = Plant<PL>::NX, TS = 16;
+    __shared__ float tiles[4][NX * TS];
+    __shared__ float pivs[4][NX];
+    const int grp = threadIdx.x / KG, lane = threadIdx.x % KG, b = blockIdx.x * 4 + grp;
+    if (b >= a.B) return;
+    const int N = a.N, r = lane < NX ? lane : 0;
+    const size_t blk = (size_t)NX * NX, base = (size_t)b * N * blk;
+    const float* Sd = a.Sd + base;
+    const float* So = a.So + base;
+    float* Gp = Gm + base;
+    float* Dp = Di + base;
+    float* tile = tiles[grp];
+    float* piv = pivs[grp];
+    float row[NX];
+#pragma unroll
+    for (int c = 0; c < NX; c++) row[c] = Sd[r * NX + c];
+    gj_inplace<NX>(lane, row, piv);
+    if (lane < NX) {
+#pragma unroll
+        for (int c = 0; c < NX; c++) Dp[r * NX + c] = row[c];
+    }
+    for (int k = 1; k < N; k++) {
+        float so[NX], g[NX];
+#pragma unroll
+        for (int m = 0; m < NX; m++) so[m] = So[k * blk + r * NX + m];
+        wave_sync();
+        if (lane < NX) {
+#pragma unroll
+            for (int c = 0; c < NX; c++) tile[lane * TS + c] = row[c];  // D_{k-1}^-1
+        }
+        wave_sync();
+#pragma unroll
+        for (int c = 0; c < NX; c++) g[c] = 0.f;
+#pragma unroll
+        for (int m = 0; m < NX; m++) {
+#pragma unroll
+            for (int c = 0; c < NX; c++) g[c] += so[m] * tile[m * TS + c];
+        }
+        if (lane < NX) {
+#pragma unroll
+            for (int c = 0; c < NX; c++) Gp[k * blk + r * NX + c] = g[c];
+        }
+        wave_sync();
+        if (lane < NX) {
+#pragma unroll
+            for (int m = 0; m < NX; m++) tile[lane * TS + m] = so[m];  // rows of S_{k,k-1}
+        }
+        wave_sync();
+#pragma unroll
+        for (int c = 0; c < NX; c++) {
+            float t = 0.f;
+#pragma unroll
+            for (int m = 0; m < NX; m++) t += g[m] * tile[c * TS + m];
+            row[c] = Sd[k * blk + r * NX + c] - t;
+        }
+        gj_inplace<NX>(lane, row, piv);
+        if (lane < NX) {
+#pragma unroll
+            for (int c = 0; c < NX; c++) Dp[k * blk + r * NX + c] = row[c];
+        }
+    }
+}
+
+// z = M^-1 r with M = L D L^T from k_btd_factor: forward sweep y_k = r_k - G_k y_{k-1}, w_k = D_k^-1 y_k, then
+// backward sweep z_k = w_k - G_{k+1}^T z_{k+1} (M^-1 = L^-T D^-1 L^-1). One 16-lane knot group runs both sweeps
+// (lane = row), the next knot's blocks loaded a step ahead; rv and zv are padded LDS vectors (knot k at
+// (k + 1) * 16), ys 16 floats of LDS for the group's current y_k. Gp, Dp: the problem's G_k and D_k^-1 blocks.
+template <int PL>
+__device__ __forceinline__ void btd_solve(int lane, int N, const float* Gp, const float* Dp, const float* rv,
+                                          float* zv, float* ys) {
+    constexpr int NX = Plant<PL>::NX, XP = 16;
+    const size_t blk = (size_t)NX * NX;
+    const int r = lane < NX ? lane : 0;
+    float gc[NX], dc[NX];
+#pragma unroll
+    for (int c = 0; c < NX; c++) {
+        gc[c] = 0.f;
+        dc[c] = Dp[r * NX + c];
+    }
+    for (int k = 0; k < N; k++) {
+        const size_t kn = (size_t)(k + 1 < N ? k + 1 : k) * blk;
+        float gn[NX], dn[NX];
+#pragma unroll
+        for (int c = 0; c < NX; c++) {
+            gn[c] = Gp[kn + r * NX + c];
+            dn[c] = Dp[kn + r * NX + c];
+        }
+        float y = rv[(k + 1) * XP + r];
+        if (k > 0) {
+            float t = 0.f;
+#pragma unroll
+            for (int c = 0; c < NX; c++) t += gc[c] * ys[c];
+            y -= t;
+        }
+        wave_sync();
+        if (lane < NX) ys[lane] = y;
+        wave_sync();
+        float w = 0.f;
+#pragma unroll
+        for (int c = 0; c < NX; c++) w += dc[c] * ys[c];
+        if (lane < NX) zv[(k + 1) * XP + lane] = w;
+#pragma unroll
+        for (int c = 0; c < NX; c++) {
+            gc[c] = gn[c];
+            dc[c] = dn[c];
+        }
+    }
+    wave_sync();
+    if (N < 2) return;
+#pragma unroll
+    for (int m = 0; m < NX; m++) gc[m] = Gp[(size_t)(N - 1) * blk + m * NX + r];  // column r of G_{N-1}
+    for (int k = N - 2; k >= 0; k--) {
+        const size_t kn = (size_t)(k > 0 ? k : 1) * blk;
+        float gn[NX];
+#pragma unroll
+        for (int m = 0; m < NX; m++) gn[m] = Gp[kn + m * NX + r];
+        float t = 0.f;
+#pragma unroll
+        for (int m = 0; m < NX; m++) t += gc[m] * zv[(k + 2) * XP + m];
+        const float x = zv[(k + 1) * XP + r] - t;
+        wave_sync();
+        if (lane < NX) zv[(k + 1) * XP + lane] = x;
+        wave_sync();
+#pragma unroll
+        for (int m = 0; m < NX; m++) gc[m] = gn[m];
+    }
+}
+
+// Two PCG solves on the same S in lockstep (the backward pass's step and adjoint chains), preconditioned with the
+// factorization of S (k_btd_factor, btd_solve): the rows of S are loaded once, every iteration does both mat-vecs,
+// each pair of dot products shares one barrier, and the two preconditioner sweeps run side by side on knot groups
+// 0 and 4 (different waves; group 1 when the work-group has only four groups). Solve i has its own right-hand side
+// a_i.gam, warm start and result a_i.lam, tolerance a_i.tol[b], absolute tolerance, iteration cap and count, and
+// the iteration of phase_pcg_reg; once it stops (its test or its cap) its vectors are left alone while the other
+// goes on. run_i = false skips solve i (count 0, lambda untouched). One knot per knot group (N <= 32).
 template <int PL>
 __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, int b, bool run1, bool run2,
-                                               float* sm, int* n1, int* n2) {
+                                               const float* Gm, const float* Di, float* sm, int* n1, int* n2) {
     constexpr int NX = Plant<PL>::NX, XP = 16;
     View<PL> v1(a1, b), v2(a2, b);
     const int N = a1.N, tid = threadIdx.x, k = tid / KG, lane = tid % KG;
@@ -388,27 +527,27 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
     float* sp2 = sr1 + (N + 2) * XP;
     float* sr2 = sp2 + (N + 2) * XP;
     float* red = sr2 + (N + 2) * XP;  // 2 x 64 floats: two double-buffered pairs of reduction slots
+    float* sz1 = red + 128;
+    float* sz2 = sz1 + (N + 2) * XP;
+    float* ys1 = sz2 + (N + 2) * XP;
+    float* ys2 = ys1 + XP;
     for (int i = tid; i < 4 * (N + 2) * XP; i += blockDim.x) sm[i] = 0.f;
     const bool val = (lane < NX) && (k < N);
     const int l = lane < NX ? lane : 0;
     const int kk = k < N ? k : 0;
-    float sd[NX], sl[NX], su[NX], pd[NX], pl[NX], pu[NX];
+    const int G = blockDim.x / KG, g2 = G > 4 ? 4 : 1;
+    const size_t pb = (size_t)b * N * NX * NX;
+    float sd[NX], sl[NX], su[NX];
     {
         const float* D = v1.Sd + (size_t)kk * NX * NX + l * NX;
         const float* L = v1.So + (size_t)kk * NX * NX + l * NX;
         const float* U = v1.So + (size_t)(kk + 1 < N ? kk + 1 : kk) * NX * NX + l;
-        const float* PD = v1.Pd + (size_t)kk * NX * NX + l * NX;
-        const float* PL_ = v1.Po + (size_t)kk * NX * NX + l * NX;
-        const float* PU = v1.Po + (size_t)(kk + 1 < N ? kk + 1 : kk) * NX * NX + l;
         const bool hasl = kk > 0, hasu = kk < N - 1;
 #pragma unroll
         for (int c = 0; c < NX; c++) {
             sd[c] = D[c];
-            pd[c] = PD[c];
             sl[c] = hasl ? L[c] : 0.f;
-            pl[c] = hasl ? PL_[c] : 0.f;
             su[c] = hasu ? U[c * NX] : 0.f;
-            pu[c] = hasu ? PU[c * NX] : 0.f;
         }
     }
     auto blk = [&](const float* M, const float4* xb, float s) {
@@ -443,7 +582,16 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
         sr2[o] = r2;
     }
     __syncthreads();
-    float z1 = (val && run1) ? mv(pd, pl, pu, sr1) : 0.f, z2 = (val && run2) ? mv(pd, pl, pu, sr2) : 0.f;
+    // z_i = M^-1 r_i on knot groups 0 and g2 while the others wait at the barrier
+    auto precond = [&](bool d1, bool d2, float& z1o, float& z2o) {
+        if (d1 && k == 0) btd_solve<PL>(lane, N, Gm + pb, Di + pb, sr1, sz1, ys1);
+        if (d2 && k == g2) btd_solve<PL>(lane, N, Gm + pb, Di + pb, sr2, sz2, ys2);
+        __syncthreads();
+        z1o = (val && d1) ? sz1[o] : 0.f;
+        z2o = (val && d2) ? sz2[o] : 0.f;
+    };
+    float z1, z2;
+    precond(run1, run2, z1, z2);
     float p1 = z1, p2 = z2;
     int slot = 0;
     const float2 rho = block_sum2(r1 * z1, r2 * z2, red + 64 * slot);
@@ -478,8 +626,7 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
             if (val) sr2[o] = r2;
         }
         __syncthreads();
-        z1 = (val && act1) ? mv(pd, pl, pu, sr1) : 0.f;
-        z2 = (val && act2) ? mv(pd, pl, pu, sr2) : 0.f;
+        precond(act1, act2, z1, z2);
         const float2 rz = block_sum2(r1 * z1, r2 * z2, red + 64 * slot);
         slot ^= 1;
         if (act1) {
@@ -760,16 +907,18 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg(Args a, int it) 
     if (threadIdx.x == 0) a.st_pcg[it * a.B + b] = n;
 }
 
-// Two PCG solves on one problem's S and P in lockstep (phase_pcg_reg2), each with the k_pcg skip rule (quirk Q9);
-// iteration counts to a1.st_pcg[b] and a2.st_pcg[b] when those are set.
+// Two PCG solves on one problem's S in lockstep (phase_pcg_reg2), preconditioned with the factorization of S in
+// Gm, Di (k_btd_factor), each with the k_pcg skip rule (quirk Q9); iteration counts to a1.st_pcg[b] and
+// a2.st_pcg[b] when those are set.
 template <int PL>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_dual(Args a1, Args a2) {
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_dual(Args a1, Args a2, const float* Gm,
+                                                                     const float* Di) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
     const bool run1 = !((a1.exit_rule == GATO_EXIT_KKT) ? !a1.active[b] : a1.conv[b]);
     const bool run2 = !((a2.exit_rule == GATO_EXIT_KKT) ? !a2.active[b] : a2.conv[b]);
     int n1 = 0, n2 = 0;
-    phase_pcg_reg2<PL>(a1, a2, b, run1, run2, sm, &n1, &n2);
+    phase_pcg_reg2<PL>(a1, a2, b, run1, run2, Gm, Di, sm, &n1, &n2);
     if (threadIdx.x == 0) {
         if (a1.st_pcg) a1.st_pcg[b] = n1;
         if (a2.st_pcg) a2.st_pcg[b] = n2;
@@ -1527,7 +1676,9 @@ struct gato_solver {
     hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
     // backward workspace: the step at the solution (lam_b, dz_b) and its correction solves (q_a, r_a, c_a,
     // gam_a, lam_a, dz_a); the adjoint solution (dz_v, lam_v) and its correction solves (q_w, r_w, c_w, gam_w,
-    // lam_w, dz_w); the backward PCG settings and the iteration counts of the two chains
+    // lam_w, dz_w); the factorization of S that preconditions the paired PCG (bwd_G, bwd_Di: N <= 32 only); the
+    // backward PCG settings and the iteration counts of the two chains
+    float *bwd_G = nullptr, *bwd_Di = nullptr;
     float *lam_b = nullptr, *dz_b = nullptr, *q_a = nullptr, *r_a = nullptr, *c_a = nullptr, *gam_a = nullptr,
           *lam_a = nullptr, *dz_a = nullptr, *dz_v = nullptr, *lam_v = nullptr, *d_bwd_tol = nullptr,
           *bst_kres = nullptr, *q_w = nullptr, *r_w = nullptr, *c_w = nullptr, *gam_w = nullptr, *lam_w = nullptr,
@@ -1627,6 +1778,9 @@ size_t layout(gato_solver* s, bool assign) {
     t = F(B * N * nx); if (assign) s->gam_w = t;
     t = F(B * N * nx); if (assign) s->lam_w = t;
     t = F(B * (s->traj + nu)); if (assign) s->dz_w = t;
+    const size_t nbt = N <= 32 ? (size_t)B * N * nx * nx : 0;  // used by the paired PCG, one knot per knot group
+    t = F(nbt); if (assign) s->bwd_G = t;
+    t = F(nbt); if (assign) s->bwd_Di = t;
     t = F(B); if (assign) a.st_init = t;
     t = F(B); if (assign) a.st_final = t;
     return off + 256;
@@ -1879,9 +2033,9 @@ template <int PL>
 int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, float* gQ, float* gR, hipStream_t st) {
     const int B = s->B, N = s->N, np = npass(s);
     const dim3 blk(grid_threads(s));
-    // (the dual PCG keeps four padded vectors and two pairs of reduction slots in LDS)
+    // (the dual PCG keeps six padded vectors, two pairs of reduction slots and two sweep vectors in LDS)
     const size_t lds = std::max({lds_bytes(s), (size_t)2 * N * gato::KG * sizeof(float),
-                                 (size_t)(4 * (N + 2) * 16 + 128) * sizeof(float)});
+                                 (size_t)(6 * (N + 2) * 16 + 128 + 32) * sizeof(float)});
     const int nlam = N * s->nx, ndz = s->traj + s->nu;
     gato::Args ab = s->a;  // the step at the solution, on its own multipliers and dz
     ab.exit_rule = GATO_EXIT_PCG0;
@@ -1898,6 +2052,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->lam_w, (size_t)B * nlam);
     hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, ab);
     launch_schur<PL>(B, blk, lds, st, ab);
+    if (np == 1) hipLaunchKernelGGL((gato::k_btd_factor<PL>), dim3((B + 3) / 4), dim3(64), 0, st, ab, s->bwd_G, s->bwd_Di);
     // correction solves of the step: same blocks, right-hand side (q_a, r_a, c_a) -> gam_a, solution (dz_a, lam_a)
     gato::Args ac = ab;
     ac.q = s->q_a;
@@ -1931,7 +2086,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
         if (i > 0) residual(nullptr, s->a.q, s->a.r, s->a.c, s->dz_b, s->lam_b, s->q_a, s->r_a, s->c_a, s->gam_a);
         residual(dL, nullptr, nullptr, nullptr, s->dz_v, s->lam_v, s->q_w, s->r_w, s->c_w, s->gam_w);
         if (np == 1) {
-            hipLaunchKernelGGL((gato::k_pcg_dual<PL>), dim3(B), blk, lds, st, as, aw);
+            hipLaunchKernelGGL((gato::k_pcg_dual<PL>), dim3(B), blk, lds, st, as, aw, s->bwd_G, s->bwd_Di);
         } else {  // S and P stream: a paired streamed PCG measured slower than two launches (more VALU, spills)
             launch_pcg<PL>(np, B, blk, lds, st, as, 0);
             launch_pcg<PL>(np, B, blk, lds, st, aw, 0);
