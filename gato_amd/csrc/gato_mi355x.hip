@@ -374,38 +374,60 @@ __device__ __forceinline__ float2 block_sum2(float v1, float v2, float* red) {
 // The backward pass's preconditioner (no reference counterpart): the block-tridiagonal factorization
 // S = L D L^T of the Schur complement, L unit lower block-bidiagonal with L_{k,k-1} = G_k, D = blkdiag(D_k):
 //   D_0 = S_00,  G_k = S_{k,k-1} D_{k-1}^-1,  D_k = S_kk - G_k S_{k,k-1}^T   (k = 1 .. N-1),
-// kept as G_k (k >= 1) and D_k^-1 (Gauss-Jordan, gj_inplace: S is definite and so are the D_k). M = L D L^T is S up
-// to fp32 rounding, so a PCG preconditioned with it (btd_solve) stops after one or two iterations where the
+// kept as G_k (k >= 1) and D_k^-1 (Gauss-Jordan without pivoting: S is definite and so are the D_k). M = L D L^T is
+// S up to fp32 rounding, so a PCG preconditioned with it (btd_solve) stops after one or two iterations where the
 // reference's preconditioner P takes 120 to 240 on the same systems at tolerance 1e-6, and reaches the same fp32
-// accuracy. Sequential over the knots: one 16-lane knot group per problem (lane = row), four problems per 64-thread
-// work-group. Gm, Di: B x N blocks of NX x NX floats (row-major), laid out like Sd.
+// accuracy. Sequential over the knots: one problem per 64-thread work-group, its first 16 lanes working (lane = row),
+// the next knot's blocks loaded a step ahead. Gm, Di: B x N blocks of NX x NX floats (row-major), laid out like Sd.
 template <int PL>
 __global__ void __launch_bounds__(64) k_btd_factor(Args a, float* Gm, float* Di) {
     constexpr int NX = Plant<PL>::NX, TS = 16;
-    __shared__ float tiles[4][NX * TS];
-    __shared__ float pivs[4][NX];
-    const int grp = threadIdx.x / KG, lane = threadIdx.x % KG, b = blockIdx.x * 4 + grp;
-    if (b >= a.B) return;
+    __shared__ float tile[NX * TS];
+    const int b = blockIdx.x, lane = threadIdx.x;
+    if (lane >= KG) return;  // one 16-lane group per problem: its own wave, so readlane broadcasts its rows
     const int N = a.N, r = lane < NX ? lane : 0;
     const size_t blk = (size_t)NX * NX, base = (size_t)b * N * blk;
     const float* Sd = a.Sd + base;
     const float* So = a.So + base;
     float* Gp = Gm + base;
     float* Dp = Di + base;
-    float* tile = tiles[grp];
-    float* piv = pivs[grp];
-    float row[NX];
+    // in-place Gauss-Jordan inverse, lane = row, the pivot row broadcast from lane p with readlane
+    auto invert = [&](float* row) {
+#pragma unroll
+        for (int p = 0; p < NX; p++) {
+            const float inv = __builtin_amdgcn_rcpf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(row[p]), p)));
+            float pr[NX];
+#pragma unroll
+            for (int c = 0; c < NX; c++)
+                pr[c] = c == p ? inv : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(row[c]), p)) * inv;
+            const float f = row[p];
+#pragma unroll
+            for (int c = 0; c < NX; c++) row[c] = lane == p ? pr[c] : (c == p ? 0.f : row[c]) - f * pr[c];
+        }
+    };
+    float row[NX], sdn[NX], son[NX];
 #pragma unroll
     for (int c = 0; c < NX; c++) row[c] = Sd[r * NX + c];
-    gj_inplace<NX>(lane, row, piv);
+#pragma unroll
+    for (int c = 0; c < NX; c++) {
+        sdn[c] = N > 1 ? Sd[blk + r * NX + c] : 0.f;
+        son[c] = N > 1 ? So[blk + r * NX + c] : 0.f;
+    }
+    invert(row);
     if (lane < NX) {
 #pragma unroll
         for (int c = 0; c < NX; c++) Dp[r * NX + c] = row[c];
     }
     for (int k = 1; k < N; k++) {
-        float so[NX], g[NX];
+        float so[NX], sd[NX], g[NX];
+        const size_t kn = (size_t)(k + 1 < N ? k + 1 : k) * blk;
 #pragma unroll
-        for (int m = 0; m < NX; m++) so[m] = So[k * blk + r * NX + m];
+        for (int c = 0; c < NX; c++) {
+            so[c] = son[c];
+            sd[c] = sdn[c];
+            sdn[c] = Sd[kn + r * NX + c];  // the next knot's rows, loaded a step ahead
+            son[c] = So[kn + r * NX + c];
+        }
         wave_sync();
         if (lane < NX) {
 #pragma unroll
@@ -434,9 +456,9 @@ __global__ void __launch_bounds__(64) k_btd_factor(Args a, float* Gm, float* Di)
             float t = 0.f;
 #pragma unroll
             for (int m = 0; m < NX; m++) t += g[m] * tile[c * TS + m];
-            row[c] = Sd[k * blk + r * NX + c] - t;
+            row[c] = sd[c] - t;
         }
-        gj_inplace<NX>(lane, row, piv);
+        invert(row);
         if (lane < NX) {
 #pragma unroll
             for (int c = 0; c < NX; c++) Dp[k * blk + r * NX + c] = row[c];
@@ -445,65 +467,61 @@ __global__ void __launch_bounds__(64) k_btd_factor(Args a, float* Gm, float* Di)
 }
 
 // z = M^-1 r with M = L D L^T from k_btd_factor: forward sweep y_k = r_k - G_k y_{k-1}, w_k = D_k^-1 y_k, then
-// backward sweep z_k = w_k - G_{k+1}^T z_{k+1} (M^-1 = L^-T D^-1 L^-1). One 16-lane knot group runs both sweeps
-// (lane = row), the next knot's blocks loaded a step ahead; rv and zv are padded LDS vectors (knot k at
-// (k + 1) * 16), ys 16 floats of LDS for the group's current y_k. Gp, Dp: the problem's G_k and D_k^-1 blocks.
+// backward sweep z_k = w_k - G_{k+1}^T z_{k+1} (M^-1 = L^-T D^-1 L^-1). One 16-lane knot group runs both sweeps,
+// lane = row, starting at lane `base` of its wave: each step's vector (y_{k-1}, y_k, z_{k+1}) is broadcast from the
+// group's lanes with readlane, so no LDS access sits on the sweep's dependency chain, and the next step's rows are
+// read a step ahead. rv, zv: padded LDS vectors (knot k at (k + 1) * 16); Gp, Dp: the problem's G_k and D_k^-1
+// blocks (in LDS in the paired PCG).
 template <int PL>
-__device__ __forceinline__ void btd_solve(int lane, int N, const float* Gp, const float* Dp, const float* rv,
-                                          float* zv, float* ys) {
+__device__ __forceinline__ void btd_solve(int lane, int base, int N, const float* Gp, const float* Dp,
+                                          const float* rv, float* zv) {
     constexpr int NX = Plant<PL>::NX, XP = 16;
-    const size_t blk = (size_t)NX * NX;
-    const int r = lane < NX ? lane : 0;
-    float gc[NX], dc[NX];
+    const int blk = NX * NX, r = lane < NX ? lane : 0;
+    base = __builtin_amdgcn_readfirstlane(base);
+    auto bcast = [&](float v, int c) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), base + c)); };
+    float gc[NX], dc[NX], gn[NX], dn[NX];
 #pragma unroll
     for (int c = 0; c < NX; c++) {
         gc[c] = 0.f;
         dc[c] = Dp[r * NX + c];
     }
+    float rc = rv[XP + r], y = 0.f;
     for (int k = 0; k < N; k++) {
-        const size_t kn = (size_t)(k + 1 < N ? k + 1 : k) * blk;
-        float gn[NX], dn[NX];
+        const int kn = (k + 1 < N ? k + 1 : k) * blk;
 #pragma unroll
         for (int c = 0; c < NX; c++) {
             gn[c] = Gp[kn + r * NX + c];
             dn[c] = Dp[kn + r * NX + c];
         }
-        float y = rv[(k + 1) * XP + r];
-        if (k > 0) {
-            float t = 0.f;
+        const float rn = rv[(k + 2 <= N ? k + 2 : N) * XP + r];
+        float t = 0.f;
 #pragma unroll
-            for (int c = 0; c < NX; c++) t += gc[c] * ys[c];
-            y -= t;
-        }
-        wave_sync();
-        if (lane < NX) ys[lane] = y;
-        wave_sync();
+        for (int c = 0; c < NX; c++) t += gc[c] * bcast(y, c);  // gc = 0 at k = 0
+        y = rc - t;
         float w = 0.f;
 #pragma unroll
-        for (int c = 0; c < NX; c++) w += dc[c] * ys[c];
+        for (int c = 0; c < NX; c++) w += dc[c] * bcast(y, c);
         if (lane < NX) zv[(k + 1) * XP + lane] = w;
 #pragma unroll
         for (int c = 0; c < NX; c++) {
             gc[c] = gn[c];
             dc[c] = dn[c];
         }
+        rc = rn;
     }
-    wave_sync();
     if (N < 2) return;
+    float x = zv[N * XP + r];  // z_{N-1} = w_{N-1}
 #pragma unroll
-    for (int m = 0; m < NX; m++) gc[m] = Gp[(size_t)(N - 1) * blk + m * NX + r];  // column r of G_{N-1}
+    for (int m = 0; m < NX; m++) gc[m] = Gp[(N - 1) * blk + m * NX + r];  // column r of G_{N-1}
     for (int k = N - 2; k >= 0; k--) {
-        const size_t kn = (size_t)(k > 0 ? k : 1) * blk;
-        float gn[NX];
+        const int kn = (k > 0 ? k : 1) * blk;
 #pragma unroll
         for (int m = 0; m < NX; m++) gn[m] = Gp[kn + m * NX + r];
         float t = 0.f;
 #pragma unroll
-        for (int m = 0; m < NX; m++) t += gc[m] * zv[(k + 2) * XP + m];
-        const float x = zv[(k + 1) * XP + r] - t;
-        wave_sync();
+        for (int m = 0; m < NX; m++) t += gc[m] * bcast(x, m);
+        x = zv[(k + 1) * XP + r] - t;
         if (lane < NX) zv[(k + 1) * XP + lane] = x;
-        wave_sync();
 #pragma unroll
         for (int m = 0; m < NX; m++) gc[m] = gn[m];
     }
@@ -512,7 +530,8 @@ __device__ __forceinline__ void btd_solve(int lane, int N, const float* Gp, cons
 // Two PCG solves on the same S in lockstep (the backward pass's step and adjoint chains), preconditioned with the
 // factorization of S (k_btd_factor, btd_solve): the rows of S are loaded once, every iteration does both mat-vecs,
 // each pair of dot products shares one barrier, and the two preconditioner sweeps run side by side on knot groups
-// 0 and 4 (different waves; group 1 when the work-group has only four groups). Solve i has its own right-hand side
+// 0 and 4 (different waves; group 1 when the work-group has only four groups) from a copy of the problem's factor
+// blocks in LDS (2 N NX^2 floats, 49 KB for iiwa14 at N = 32). Solve i has its own right-hand side
 // a_i.gam, warm start and result a_i.lam, tolerance a_i.tol[b], absolute tolerance, iteration cap and count, and
 // the iteration of phase_pcg_reg; once it stops (its test or its cap) its vectors are left alone while the other
 // goes on. run_i = false skips solve i (count 0, lambda untouched). One knot per knot group (N <= 32).
@@ -529,14 +548,24 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
     float* red = sr2 + (N + 2) * XP;  // 2 x 64 floats: two double-buffered pairs of reduction slots
     float* sz1 = red + 128;
     float* sz2 = sz1 + (N + 2) * XP;
-    float* ys1 = sz2 + (N + 2) * XP;
-    float* ys2 = ys1 + XP;
+    float* sG = sz2 + (N + 2) * XP;  // the problem's G_k and D_k^-1 blocks, copied to LDS for the sweeps
+    float* sD = sG + N * NX * NX;
     for (int i = tid; i < 4 * (N + 2) * XP; i += blockDim.x) sm[i] = 0.f;
+    {
+        const size_t pb = (size_t)b * N * NX * NX;
+        const float4* g4 = reinterpret_cast<const float4*>(Gm + pb);
+        const float4* d4 = reinterpret_cast<const float4*>(Di + pb);
+        float4* sg4 = reinterpret_cast<float4*>(sG);
+        float4* sd4 = reinterpret_cast<float4*>(sD);
+        for (int i = tid; i < N * NX * NX / 4; i += blockDim.x) {
+            sg4[i] = g4[i];
+            sd4[i] = d4[i];
+        }
+    }
     const bool val = (lane < NX) && (k < N);
     const int l = lane < NX ? lane : 0;
     const int kk = k < N ? k : 0;
     const int G = blockDim.x / KG, g2 = G > 4 ? 4 : 1;
-    const size_t pb = (size_t)b * N * NX * NX;
     float sd[NX], sl[NX], su[NX];
     {
         const float* D = v1.Sd + (size_t)kk * NX * NX + l * NX;
@@ -584,8 +613,8 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
     __syncthreads();
     // z_i = M^-1 r_i on knot groups 0 and g2 while the others wait at the barrier
     auto precond = [&](bool d1, bool d2, float& z1o, float& z2o) {
-        if (d1 && k == 0) btd_solve<PL>(lane, N, Gm + pb, Di + pb, sr1, sz1, ys1);
-        if (d2 && k == g2) btd_solve<PL>(lane, N, Gm + pb, Di + pb, sr2, sz2, ys2);
+        if (d1 && k == 0) btd_solve<PL>(lane, 0, N, sG, sD, sr1, sz1);
+        if (d2 && k == g2) btd_solve<PL>(lane, (g2 & 3) * KG, N, sG, sD, sr2, sz2);
         __syncthreads();
         z1o = (val && d1) ? sz1[o] : 0.f;
         z2o = (val && d2) ? sz2[o] : 0.f;
@@ -1685,7 +1714,7 @@ struct gato_solver {
           *dz_w = nullptr;
     int *bst_pcg = nullptr, *bst_pcg_s = nullptr;
     int bwd_max_pcg = 1000;
-    int bwd_refine = 2;
+    int bwd_refine = -1;  // -1: one round with the factorization preconditioner, two with P
     float bwd_tol = 1e-6f;
     // graph mode: the captured solve, valid for the launch arguments it was captured with
     hipStream_t cap_stream = nullptr;
@@ -2027,15 +2056,16 @@ int launch_solve(gato_solver* s, hipStream_t st) {
 
 
 // The backward pass: the step at the solution and the adjoint system, each solved with the step's Schur
-// complement and preconditioner and improved by `bwd_refine` rounds of iterative refinement (the residual
-// is evaluated with the KKT blocks themselves, so fp32 error in S and Q~^-1 does not limit the result).
+// complement, preconditioned with its block-tridiagonal factorization (N <= 32: k_btd_factor, k_pcg_dual) or
+// with the step's preconditioner P (otherwise), and improved by `bwd_refine` rounds of iterative refinement (the
+// residual is evaluated with the KKT blocks themselves, so fp32 error in S and Q~^-1 does not limit the result).
 template <int PL>
 int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, float* gQ, float* gR, hipStream_t st) {
     const int B = s->B, N = s->N, np = npass(s);
     const dim3 blk(grid_threads(s));
-    // (the dual PCG keeps six padded vectors, two pairs of reduction slots and two sweep vectors in LDS)
-    const size_t lds = std::max({lds_bytes(s), (size_t)2 * N * gato::KG * sizeof(float),
-                                 (size_t)(6 * (N + 2) * 16 + 128 + 32) * sizeof(float)});
+    const size_t lds = std::max(lds_bytes(s), (size_t)2 * N * gato::KG * sizeof(float));
+    // the dual PCG keeps six padded vectors, two pairs of reduction slots and the problem's factor blocks in LDS
+    const size_t lds_dual = std::max(lds, (size_t)(6 * (N + 2) * 16 + 128 + 2 * N * s->nx * s->nx) * sizeof(float));
     const int nlam = N * s->nx, ndz = s->traj + s->nu;
     gato::Args ab = s->a;  // the step at the solution, on its own multipliers and dz
     ab.exit_rule = GATO_EXIT_PCG0;
@@ -2052,7 +2082,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->lam_w, (size_t)B * nlam);
     hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, ab);
     launch_schur<PL>(B, blk, lds, st, ab);
-    if (np == 1) hipLaunchKernelGGL((gato::k_btd_factor<PL>), dim3((B + 3) / 4), dim3(64), 0, st, ab, s->bwd_G, s->bwd_Di);
+    if (np == 1) hipLaunchKernelGGL((gato::k_btd_factor<PL>), dim3(B), dim3(64), 0, st, ab, s->bwd_G, s->bwd_Di);
     // correction solves of the step: same blocks, right-hand side (q_a, r_a, c_a) -> gam_a, solution (dz_a, lam_a)
     gato::Args ac = ab;
     ac.q = s->q_a;
@@ -2081,12 +2111,13 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     };
     // the two chains are independent and share K's Schur complement and preconditioner: round i solves the
     // step's correction i (round 0: the step itself) and the adjoint's correction i side by side
-    for (int i = 0; i <= s->bwd_refine; i++) {
+    const int refine = s->bwd_refine >= 0 ? s->bwd_refine : (np == 1 ? 1 : 2);
+    for (int i = 0; i <= refine; i++) {
         const gato::Args& as = i == 0 ? ab : ac;
         if (i > 0) residual(nullptr, s->a.q, s->a.r, s->a.c, s->dz_b, s->lam_b, s->q_a, s->r_a, s->c_a, s->gam_a);
         residual(dL, nullptr, nullptr, nullptr, s->dz_v, s->lam_v, s->q_w, s->r_w, s->c_w, s->gam_w);
         if (np == 1) {
-            hipLaunchKernelGGL((gato::k_pcg_dual<PL>), dim3(B), blk, lds, st, as, aw, s->bwd_G, s->bwd_Di);
+            hipLaunchKernelGGL((gato::k_pcg_dual<PL>), dim3(B), blk, lds_dual, st, as, aw, s->bwd_G, s->bwd_Di);
         } else {  // S and P stream: a paired streamed PCG measured slower than two launches (more VALU, spills)
             launch_pcg<PL>(np, B, blk, lds, st, as, 0);
             launch_pcg<PL>(np, B, blk, lds, st, aw, 0);
@@ -2662,8 +2693,8 @@ int gato_debug_copy(gato_solver* s, const char* name, void* d_dst, int64_t cap, 
 
 int gato_set_backward_params(gato_solver* s, int32_t max_pcg_iters, float pcg_tol, int32_t refine) {
     if (!s) return fail(GATO_E_ARG, "null solver");
-    if (max_pcg_iters < 1 || !(pcg_tol >= 0.f) || refine < 0 || refine > 8)
-        return fail(GATO_E_ARG, "need max_pcg_iters >= 1, pcg_tol >= 0 and refine in [0, 8]");
+    if (max_pcg_iters < 1 || !(pcg_tol >= 0.f) || refine < -1 || refine > 8)
+        return fail(GATO_E_ARG, "need max_pcg_iters >= 1, pcg_tol >= 0 and refine in [-1, 8]");
     s->bwd_max_pcg = max_pcg_iters;
     s->bwd_tol = pcg_tol;
     s->bwd_refine = refine;
