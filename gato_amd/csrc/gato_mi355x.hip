@@ -655,6 +655,10 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
             if (val) sr2[o] = r2;
         }
         __syncthreads();
+        // a solve at its cap is done: its x is final, so its last preconditioner application and test are skipped
+        act1 = act1 && it1 < a1.max_pcg;
+        act2 = act2 && it2 < a2.max_pcg;
+        if (!act1 && !act2) break;  // the flags come from work-group sums: uniform
         precond(act1, act2, z1, z2);
         const float2 rz = block_sum2(r1 * z1, r2 * z2, red + 64 * slot);
         slot ^= 1;
@@ -1713,8 +1717,8 @@ struct gato_solver {
           *bst_kres = nullptr, *q_w = nullptr, *r_w = nullptr, *c_w = nullptr, *gam_w = nullptr, *lam_w = nullptr,
           *dz_w = nullptr;
     int *bst_pcg = nullptr, *bst_pcg_s = nullptr;
-    int bwd_max_pcg = 1000;
-    int bwd_refine = -1;  // -1: one round with the factorization preconditioner, two with P
+    int bwd_max_pcg = -1;  // -1: one iteration per round with the factorization preconditioner, 1000 with P
+    int bwd_refine = 2;
     float bwd_tol = 1e-6f;
     // graph mode: the captured solve, valid for the launch arguments it was captured with
     hipStream_t cap_stream = nullptr;
@@ -2090,7 +2094,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     ac.gam = s->gam_a;
     ac.lam = s->lam_a;
     ac.dz = s->dz_a;
-    ac.max_pcg = s->bwd_max_pcg;
+    ac.max_pcg = s->bwd_max_pcg > 0 ? s->bwd_max_pcg : (np == 1 ? 1 : 1000);
     ac.tol = s->d_bwd_tol;
     ac.pcg_abs_tol = 1e-30f;
     ac.st_pcg = s->bst_pcg_s;
@@ -2111,8 +2115,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     };
     // the two chains are independent and share K's Schur complement and preconditioner: round i solves the
     // step's correction i (round 0: the step itself) and the adjoint's correction i side by side
-    const int refine = s->bwd_refine >= 0 ? s->bwd_refine : (np == 1 ? 1 : 2);
-    for (int i = 0; i <= refine; i++) {
+    for (int i = 0; i <= s->bwd_refine; i++) {
         const gato::Args& as = i == 0 ? ab : ac;
         if (i > 0) residual(nullptr, s->a.q, s->a.r, s->a.c, s->dz_b, s->lam_b, s->q_a, s->r_a, s->c_a, s->gam_a);
         residual(dL, nullptr, nullptr, nullptr, s->dz_v, s->lam_v, s->q_w, s->r_w, s->c_w, s->gam_w);
@@ -2693,8 +2696,8 @@ int gato_debug_copy(gato_solver* s, const char* name, void* d_dst, int64_t cap, 
 
 int gato_set_backward_params(gato_solver* s, int32_t max_pcg_iters, float pcg_tol, int32_t refine) {
     if (!s) return fail(GATO_E_ARG, "null solver");
-    if (max_pcg_iters < 1 || !(pcg_tol >= 0.f) || refine < -1 || refine > 8)
-        return fail(GATO_E_ARG, "need max_pcg_iters >= 1, pcg_tol >= 0 and refine in [-1, 8]");
+    if (!(max_pcg_iters >= 1 || max_pcg_iters == -1) || !(pcg_tol >= 0.f) || refine < 0 || refine > 8)
+        return fail(GATO_E_ARG, "need max_pcg_iters >= 1 (or -1), pcg_tol >= 0 and refine in [0, 8]");
     s->bwd_max_pcg = max_pcg_iters;
     s->bwd_tol = pcg_tol;
     s->bwd_refine = refine;

@@ -265,7 +265,7 @@ class Solver:
         g["adj_pcg_iters"] = it
         return g
 
-    def set_backward_params(self, max_pcg_iters=1000, pcg_tol=1e-6, refine=-1):
+    def set_backward_params(self, max_pcg_iters=-1, pcg_tol=1e-6, refine=2):
         check(self.lib.gato_set_backward_params(self._h, int(max_pcg_iters), float(pcg_tol), int(refine)),
               "gato_set_backward_params")
 
