@@ -371,26 +371,32 @@ __device__ __forceinline__ float2 block_sum2(float v1, float v2, float* red) {
     return make_float2(lane_f(s1, 0) + lane_f(s1, 16), lane_f(s2, 0) + lane_f(s2, 16));
 }
 
-// The backward pass's preconditioner (no reference counterpart): the block-tridiagonal factorization
-// S = L D L^T of the Schur complement, L unit lower block-bidiagonal with L_{k,k-1} = G_k, D = blkdiag(D_k):
-//   D_0 = S_00,  G_k = S_{k,k-1} D_{k-1}^-1,  D_k = S_kk - G_k S_{k,k-1}^T   (k = 1 .. N-1),
-// kept as G_k (k >= 1) and D_k^-1 (Gauss-Jordan without pivoting: S is definite and so are the D_k). M = L D L^T is
-// S up to fp32 rounding, so a PCG preconditioned with it (btd_solve) stops after one or two iterations where the
-// reference's preconditioner P takes 120 to 240 on the same systems at tolerance 1e-6, and reaches the same fp32
-// accuracy. Sequential over the knots: one problem per 64-thread work-group, its first 16 lanes working (lane = row),
-// the next knot's blocks loaded a step ahead. Gm, Di: B x N blocks of NX x NX floats (row-major), laid out like Sd.
+// The backward pass's preconditioner (no reference counterpart): a twisted block-tridiagonal factorization of
+// the Schur complement, eliminating from both ends towards the middle knot m = N / 2:
+//   top     D_0 = S_00,          G_k = S_{k,k-1} D_{k-1}^-1,    D_k = S_kk - G_k S_{k,k-1}^T   (k = 1 .. m),
+//   bottom  E_{N-1} = S_{N-1,N-1}, H_k = S_{k,k+1} E_{k+1}^-1,  E_k = S_kk - H_k S_{k+1,k}     (k = N-2 .. m),
+//   middle  C_m = D_m + E_m - S_mm.
+// Kept as Gm[k] = G_k (1 <= k <= m), Gm[k + 1] = H_k (m <= k <= N - 2) and Di[k] = D_k^-1 (k < m), C_m^-1 (k = m),
+// E_k^-1 (k > m), the inverses by Gauss-Jordan without pivoting (S is definite, and so are D_k, E_k and C_m). The
+// factors give S exactly up to fp32 rounding, so a PCG preconditioned with them (btd_apply) stops after one or two
+// iterations where the reference's preconditioner P takes 120 to 240 on the same systems at tolerance 1e-6, and
+// reaches the same fp32 accuracy. The two eliminations run side by side, each sequential over its half of the
+// knots: one problem per 128-thread work-group, the first 16 lanes of wave 0 (top) and wave 1 (bottom) working
+// (lane = row), the next knot's rows loaded a step ahead. Gm, Di: B x N blocks of NX x NX floats (row-major).
 template <int PL>
-__global__ void __launch_bounds__(64) k_btd_factor(Args a, float* Gm, float* Di) {
+__global__ void __launch_bounds__(128) k_btd_factor(Args a, float* Gm, float* Di) {
     constexpr int NX = Plant<PL>::NX, TS = 16;
-    __shared__ float tile[NX * TS];
-    const int b = blockIdx.x, lane = threadIdx.x;
-    if (lane >= KG) return;  // one 16-lane group per problem: its own wave, so readlane broadcasts its rows
-    const int N = a.N, r = lane < NX ? lane : 0;
+    __shared__ float tiles[2][2][NX * TS];
+    __shared__ float pivot[2][NX * TS];  // D_m (top) and E_m (bottom)
+    const int b = blockIdx.x, side = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int N = a.N, m = N / 2, r = lane < NX ? lane : 0;
     const size_t blk = (size_t)NX * NX, base = (size_t)b * N * blk;
     const float* Sd = a.Sd + base;
     const float* So = a.So + base;
     float* Gp = Gm + base;
     float* Dp = Di + base;
+    float* t1 = tiles[side][0];
+    float* t2 = tiles[side][1];
     // in-place Gauss-Jordan inverse, lane = row, the pivot row broadcast from lane p with readlane
     auto invert = [&](float* row) {
 #pragma unroll
@@ -405,103 +411,145 @@ __global__ void __launch_bounds__(64) k_btd_factor(Args a, float* Gm, float* Di)
             for (int c = 0; c < NX; c++) row[c] = lane == p ? pr[c] : (c == p ? 0.f : row[c]) - f * pr[c];
         }
     };
-    float row[NX], sdn[NX], son[NX];
+    auto store = [&](float* dst, const float* v) {
+        if (lane < NX) {
 #pragma unroll
-    for (int c = 0; c < NX; c++) row[c] = Sd[r * NX + c];
+            for (int c = 0; c < NX; c++) dst[r * NX + c] = v[c];
+        }
+    };
+    float row[NX];
+    if (lane < KG) {
+        const int first = side == 0 ? 0 : N - 1, steps = side == 0 ? m : N - 1 - m;
 #pragma unroll
-    for (int c = 0; c < NX; c++) {
-        sdn[c] = N > 1 ? Sd[blk + r * NX + c] : 0.f;
-        son[c] = N > 1 ? So[blk + r * NX + c] : 0.f;
+        for (int c = 0; c < NX; c++) row[c] = Sd[first * blk + r * NX + c];
+        float son[NX], sdn[NX];  // the next step's block rows
+        {
+            const int k1 = side == 0 ? (steps > 0 ? 1 : 0) : (steps > 0 ? N - 1 : 0);  // top: S_{1,0}; bottom: S_{N-1,N-2}
+            const int d1 = side == 0 ? (steps > 0 ? 1 : 0) : (steps > 0 ? N - 2 : 0);
+#pragma unroll
+            for (int c = 0; c < NX; c++) {
+                son[c] = So[k1 * blk + r * NX + c];
+                sdn[c] = Sd[d1 * blk + r * NX + c];
+            }
+        }
+        for (int s = 0; s < steps; s++) {
+            // top: knot k = s + 1, off-diagonal block S_{k,k-1} = So_k; bottom: knot k = N - 2 - s, S_{k+1,k} = So_{k+1}
+            const int k = side == 0 ? s + 1 : N - 2 - s;
+            float so[NX], sd[NX], g[NX];
+            {
+                const int kn = side == 0 ? (k + 1 <= m ? k + 1 : k) : (k - 1 >= m ? k - 1 : k);
+                const int kon = side == 0 ? kn : kn + 1;
+#pragma unroll
+                for (int c = 0; c < NX; c++) {
+                    so[c] = son[c];
+                    sd[c] = sdn[c];
+                    son[c] = So[kon * blk + r * NX + c];
+                    sdn[c] = Sd[kn * blk + r * NX + c];
+                }
+            }
+            invert(row);  // D_{k-1}^-1 (top) or E_{k+1}^-1 (bottom)
+            store(Dp + (side == 0 ? k - 1 : k + 1) * blk, row);
+            wave_sync();
+            if (lane < NX) {
+#pragma unroll
+                for (int c = 0; c < NX; c++) {
+                    t1[lane * TS + c] = row[c];
+                    t2[lane * TS + c] = so[c];
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int c = 0; c < NX; c++) g[c] = 0.f;
+            if (side == 0) {  // G_k = So_k D_{k-1}^-1: row r = sum_j So_k[r][j] D^-1[j][.]
+#pragma unroll
+                for (int j = 0; j < NX; j++) {
+#pragma unroll
+                    for (int c = 0; c < NX; c++) g[c] += so[j] * t1[j * TS + c];
+                }
+            } else {  // H_k = So_{k+1}^T E_{k+1}^-1: row r = sum_j So_{k+1}[j][r] E^-1[j][.]
+#pragma unroll
+                for (int j = 0; j < NX; j++) {
+                    const float sj = t2[j * TS + r];
+#pragma unroll
+                    for (int c = 0; c < NX; c++) g[c] += sj * t1[j * TS + c];
+                }
+            }
+            store(Gp + (size_t)(side == 0 ? k : k + 1) * blk, g);
+#pragma unroll
+            for (int c = 0; c < NX; c++) {
+                float t = 0.f;
+                if (side == 0) {  // (G_k So_k^T)[r][c] = sum_j G_k[r][j] So_k[c][j]
+#pragma unroll
+                    for (int j = 0; j < NX; j++) t += g[j] * t2[c * TS + j];
+                } else {  // (H_k So_{k+1})[r][c] = sum_j H_k[r][j] So_{k+1}[j][c]
+#pragma unroll
+                    for (int j = 0; j < NX; j++) t += g[j] * t2[j * TS + c];
+                }
+                row[c] = sd[c] - t;
+            }
+            wave_sync();
+        }
+        if (lane < NX) {
+#pragma unroll
+            for (int c = 0; c < NX; c++) pivot[side][lane * TS + c] = row[c];
+        }
     }
-    invert(row);
-    if (lane < NX) {
+    __syncthreads();
+    if (side == 0 && lane < KG) {  // C_m = D_m + E_m - S_mm
 #pragma unroll
-        for (int c = 0; c < NX; c++) Dp[r * NX + c] = row[c];
-    }
-    for (int k = 1; k < N; k++) {
-        float so[NX], sd[NX], g[NX];
-        const size_t kn = (size_t)(k + 1 < N ? k + 1 : k) * blk;
-#pragma unroll
-        for (int c = 0; c < NX; c++) {
-            so[c] = son[c];
-            sd[c] = sdn[c];
-            sdn[c] = Sd[kn + r * NX + c];  // the next knot's rows, loaded a step ahead
-            son[c] = So[kn + r * NX + c];
-        }
-        wave_sync();
-        if (lane < NX) {
-#pragma unroll
-            for (int c = 0; c < NX; c++) tile[lane * TS + c] = row[c];  // D_{k-1}^-1
-        }
-        wave_sync();
-#pragma unroll
-        for (int c = 0; c < NX; c++) g[c] = 0.f;
-#pragma unroll
-        for (int m = 0; m < NX; m++) {
-#pragma unroll
-            for (int c = 0; c < NX; c++) g[c] += so[m] * tile[m * TS + c];
-        }
-        if (lane < NX) {
-#pragma unroll
-            for (int c = 0; c < NX; c++) Gp[k * blk + r * NX + c] = g[c];
-        }
-        wave_sync();
-        if (lane < NX) {
-#pragma unroll
-            for (int m = 0; m < NX; m++) tile[lane * TS + m] = so[m];  // rows of S_{k,k-1}
-        }
-        wave_sync();
-#pragma unroll
-        for (int c = 0; c < NX; c++) {
-            float t = 0.f;
-#pragma unroll
-            for (int m = 0; m < NX; m++) t += g[m] * tile[c * TS + m];
-            row[c] = sd[c] - t;
-        }
+        for (int c = 0; c < NX; c++) row[c] = pivot[0][r * TS + c] + pivot[1][r * TS + c] - Sd[m * blk + r * NX + c];
         invert(row);
-        if (lane < NX) {
-#pragma unroll
-            for (int c = 0; c < NX; c++) Dp[k * blk + r * NX + c] = row[c];
-        }
+        store(Dp + (size_t)m * blk, row);
     }
 }
 
-// z = M^-1 r with M = L D L^T from k_btd_factor: forward sweep y_k = r_k - G_k y_{k-1}, w_k = D_k^-1 y_k, then
-// backward sweep z_k = w_k - G_{k+1}^T z_{k+1} (M^-1 = L^-T D^-1 L^-1). One 16-lane knot group runs both sweeps,
-// lane = row, starting at lane `base` of its wave: each step's vector (y_{k-1}, y_k, z_{k+1}) is broadcast from the
-// group's lanes with readlane, so no LDS access sits on the sweep's dependency chain, and the next step's rows are
-// read a step ahead. rv, zv: padded LDS vectors (knot k at (k + 1) * 16); Gp, Dp: the problem's G_k and D_k^-1
-// blocks (in LDS in the paired PCG).
+// z = M^-1 r with the twisted factorization of k_btd_factor, in three phases separated by work-group barriers:
+// btd_ends (both ends towards m: top y_k = r_k - G_k y_{k-1}, w_k = D_k^-1 y_k for k < m; bottom u_k = r_k -
+// H_k u_{k+1}, v_k = E_k^-1 u_k for k > m; y_m and u_m to LDS), btd_middle (z_m = C_m^-1 (y_m + u_m - r_m)) and
+// btd_out (from m outwards: z_k = w_k - G_{k+1}^T z_{k+1} above, z_k = v_k - H_{k-1}^T z_{k-1} below). Each side
+// is one 16-lane knot group (lane = row) starting at lane `base` of its wave; each step's vector is broadcast
+// from the group's lanes with readlane and the next step's rows are read a step ahead. rv, zv: padded LDS
+// vectors (knot k at (k + 1) * 16); Gp, Dp: the problem's factor blocks (in LDS in the paired PCG); pv: 16
+// floats of LDS for the side's vector at m.
 template <int PL>
-__device__ __forceinline__ void btd_solve(int lane, int base, int N, const float* Gp, const float* Dp,
-                                          const float* rv, float* zv) {
+__device__ __forceinline__ void btd_ends(bool top, int lane, int base, int N, const float* Gp, const float* Dp,
+                                         const float* rv, float* zv, float* pv) {
     constexpr int NX = Plant<PL>::NX, XP = 16;
-    const int blk = NX * NX, r = lane < NX ? lane : 0;
+    const int blk = NX * NX, r = lane < NX ? lane : 0, m = N / 2;
     base = __builtin_amdgcn_readfirstlane(base);
     auto bcast = [&](float v, int c) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), base + c)); };
+    const int steps = top ? m + 1 : N - m;
+    auto knot = [&](int s) { return top ? s : N - 1 - s; };
+    auto mult = [&](int k) { const int g = top ? k : k + 1; return g < N ? g : N - 1; };  // G_k or H_k
     float gc[NX], dc[NX], gn[NX], dn[NX];
+    int k = knot(0);
 #pragma unroll
     for (int c = 0; c < NX; c++) {
         gc[c] = 0.f;
-        dc[c] = Dp[r * NX + c];
+        dc[c] = Dp[k * blk + r * NX + c];
     }
-    float rc = rv[XP + r], y = 0.f;
-    for (int k = 0; k < N; k++) {
-        const int kn = (k + 1 < N ? k + 1 : k) * blk;
+    float rc = rv[(k + 1) * XP + r], y = 0.f;
+    for (int s = 0; s < steps; s++) {
+        k = knot(s);
+        const int kn = knot(s + 1 < steps ? s + 1 : s);
 #pragma unroll
         for (int c = 0; c < NX; c++) {
-            gn[c] = Gp[kn + r * NX + c];
-            dn[c] = Dp[kn + r * NX + c];
+            gn[c] = Gp[mult(kn) * blk + r * NX + c];
+            dn[c] = Dp[kn * blk + r * NX + c];
         }
-        const float rn = rv[(k + 2 <= N ? k + 2 : N) * XP + r];
+        const float rn = rv[(kn + 1) * XP + r];
         float t = 0.f;
 #pragma unroll
-        for (int c = 0; c < NX; c++) t += gc[c] * bcast(y, c);  // gc = 0 at k = 0
+        for (int c = 0; c < NX; c++) t += gc[c] * bcast(y, c);  // gc = 0 at the first step
         y = rc - t;
-        float w = 0.f;
+        if (k != m) {
+            float w = 0.f;
 #pragma unroll
-        for (int c = 0; c < NX; c++) w += dc[c] * bcast(y, c);
-        if (lane < NX) zv[(k + 1) * XP + lane] = w;
+            for (int c = 0; c < NX; c++) w += dc[c] * bcast(y, c);
+            if (lane < NX) zv[(k + 1) * XP + lane] = w;
+        } else if (lane < NX) {
+            pv[lane] = y;
+        }
 #pragma unroll
         for (int c = 0; c < NX; c++) {
             gc[c] = gn[c];
@@ -509,29 +557,55 @@ __device__ __forceinline__ void btd_solve(int lane, int base, int N, const float
         }
         rc = rn;
     }
-    if (N < 2) return;
-    float x = zv[N * XP + r];  // z_{N-1} = w_{N-1}
+}
+
+template <int PL>
+__device__ __forceinline__ void btd_middle(int lane, int base, int N, const float* Dp, const float* rv, float* zv,
+                                           const float* pt, const float* pb) {
+    constexpr int NX = Plant<PL>::NX, XP = 16;
+    const int r = lane < NX ? lane : 0, m = N / 2;
+    base = __builtin_amdgcn_readfirstlane(base);
+    const float t = pt[r] + pb[r] - rv[(m + 1) * XP + r];
+    float z = 0.f;
 #pragma unroll
-    for (int m = 0; m < NX; m++) gc[m] = Gp[(N - 1) * blk + m * NX + r];  // column r of G_{N-1}
-    for (int k = N - 2; k >= 0; k--) {
-        const int kn = (k > 0 ? k : 1) * blk;
+    for (int c = 0; c < NX; c++)
+        z += Dp[m * NX * NX + r * NX + c] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t), base + c));
+    if (lane < NX) zv[(m + 1) * XP + lane] = z;
+}
+
+template <int PL>
+__device__ __forceinline__ void btd_out(bool top, int lane, int base, int N, const float* Gp, float* zv) {
+    constexpr int NX = Plant<PL>::NX, XP = 16;
+    const int blk = NX * NX, r = lane < NX ? lane : 0, m = N / 2;
+    base = __builtin_amdgcn_readfirstlane(base);
+    auto bcast = [&](float v, int c) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), base + c)); };
+    const int steps = top ? m : N - 1 - m;
+    if (steps <= 0) return;
+    auto knot = [&](int s) { return top ? m - 1 - s : m + 1 + s; };
+    auto col = [&](int k) { return top ? k + 1 : k; };  // G_{k+1} above, H_{k-1} = Gm[k] below
+    float x = zv[(m + 1) * XP + r];
+    float gc[NX], gn[NX];
 #pragma unroll
-        for (int m = 0; m < NX; m++) gn[m] = Gp[kn + m * NX + r];
+    for (int j = 0; j < NX; j++) gc[j] = Gp[col(knot(0)) * blk + j * NX + r];
+    for (int s = 0; s < steps; s++) {
+        const int k = knot(s), kn = knot(s + 1 < steps ? s + 1 : s);
+#pragma unroll
+        for (int j = 0; j < NX; j++) gn[j] = Gp[col(kn) * blk + j * NX + r];
         float t = 0.f;
 #pragma unroll
-        for (int m = 0; m < NX; m++) t += gc[m] * bcast(x, m);
+        for (int j = 0; j < NX; j++) t += gc[j] * bcast(x, j);
         x = zv[(k + 1) * XP + r] - t;
         if (lane < NX) zv[(k + 1) * XP + lane] = x;
 #pragma unroll
-        for (int m = 0; m < NX; m++) gc[m] = gn[m];
+        for (int j = 0; j < NX; j++) gc[j] = gn[j];
     }
 }
 
 // Two PCG solves on the same S in lockstep (the backward pass's step and adjoint chains), preconditioned with the
-// factorization of S (k_btd_factor, btd_solve): the rows of S are loaded once, every iteration does both mat-vecs,
-// each pair of dot products shares one barrier, and the two preconditioner sweeps run side by side on knot groups
-// 0 and 4 (different waves; group 1 when the work-group has only four groups) from a copy of the problem's factor
-// blocks in LDS (2 N NX^2 floats, 49 KB for iiwa14 at N = 32). Solve i has its own right-hand side
+// factorization of S (k_btd_factor; btd_ends, btd_middle, btd_out): the rows of S are loaded once, every iteration
+// does both mat-vecs, each pair of dot products shares one barrier, and the four sweep sides (top and bottom of
+// each chain) run side by side on knot groups 0, 4, 8, 12 (separate waves; fewer waves in smaller work-groups)
+// from a copy of the problem's factor blocks in LDS (2 N NX^2 floats, 49 KB for iiwa14 at N = 32). Solve i has its own right-hand side
 // a_i.gam, warm start and result a_i.lam, tolerance a_i.tol[b], absolute tolerance, iteration cap and count, and
 // the iteration of phase_pcg_reg; once it stops (its test or its cap) its vectors are left alone while the other
 // goes on. run_i = false skips solve i (count 0, lambda untouched). One knot per knot group (N <= 32).
@@ -550,6 +624,7 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
     float* sz2 = sz1 + (N + 2) * XP;
     float* sG = sz2 + (N + 2) * XP;  // the problem's G_k and D_k^-1 blocks, copied to LDS for the sweeps
     float* sD = sG + N * NX * NX;
+    float* pv = sD + N * NX * NX;  // 4 x 16 floats: each sweep side's vector at the middle knot
     for (int i = tid; i < 4 * (N + 2) * XP; i += blockDim.x) sm[i] = 0.f;
     {
         const size_t pb = (size_t)b * N * NX * NX;
@@ -565,7 +640,9 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
     const bool val = (lane < NX) && (k < N);
     const int l = lane < NX ? lane : 0;
     const int kk = k < N ? k : 0;
-    const int G = blockDim.x / KG, g2 = G > 4 ? 4 : 1;
+    const int G = blockDim.x / KG;
+    // knot groups of the four sweep sides (chain 1 top, bottom, chain 2 top, bottom): one per wave when possible
+    const int rg[4] = {0, G >= 16 ? 4 : 1, G >= 16 ? 8 : G >= 8 ? 4 : 2, G >= 16 ? 12 : G >= 8 ? 5 : 3};
     float sd[NX], sl[NX], su[NX];
     {
         const float* D = v1.Sd + (size_t)kk * NX * NX + l * NX;
@@ -611,10 +688,20 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
         sr2[o] = r2;
     }
     __syncthreads();
-    // z_i = M^-1 r_i on knot groups 0 and g2 while the others wait at the barrier
+    // z_i = M^-1 r_i: the top and bottom sides of both chains on four knot groups (four waves when there are
+    // 16 groups or more) while the others wait at the barriers
+    const int role = k == rg[0] ? 0 : k == rg[1] ? 1 : k == rg[2] ? 2 : k == rg[3] ? 3 : -1;
+    const bool rtop = (role & 1) == 0;
+    const int rbase = (k & 3) * KG;
+    const float* rrv = role < 2 ? sr1 : sr2;
+    float* rzv = role < 2 ? sz1 : sz2;
     auto precond = [&](bool d1, bool d2, float& z1o, float& z2o) {
-        if (d1 && k == 0) btd_solve<PL>(lane, 0, N, sG, sD, sr1, sz1);
-        if (d2 && k == g2) btd_solve<PL>(lane, (g2 & 3) * KG, N, sG, sD, sr2, sz2);
+        const bool mine = role >= 0 && (role < 2 ? d1 : d2);
+        if (mine) btd_ends<PL>(rtop, lane, rbase, N, sG, sD, rrv, rzv, pv + role * XP);
+        __syncthreads();
+        if (mine && rtop) btd_middle<PL>(lane, rbase, N, sD, rrv, rzv, pv + role * XP, pv + (role + 1) * XP);
+        __syncthreads();
+        if (mine) btd_out<PL>(rtop, lane, rbase, N, sG, rzv);
         __syncthreads();
         z1o = (val && d1) ? sz1[o] : 0.f;
         z2o = (val && d2) ? sz2[o] : 0.f;
@@ -2068,8 +2155,10 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     const int B = s->B, N = s->N, np = npass(s);
     const dim3 blk(grid_threads(s));
     const size_t lds = std::max(lds_bytes(s), (size_t)2 * N * gato::KG * sizeof(float));
-    // the dual PCG keeps six padded vectors, two pairs of reduction slots and the problem's factor blocks in LDS
-    const size_t lds_dual = std::max(lds, (size_t)(6 * (N + 2) * 16 + 128 + 2 * N * s->nx * s->nx) * sizeof(float));
+    // the dual PCG keeps six padded vectors, two pairs of reduction slots, the problem's factor blocks and the
+    // sweeps' middle vectors in LDS
+    const size_t lds_dual =
+        std::max(lds, (size_t)(6 * (N + 2) * 16 + 128 + 2 * N * s->nx * s->nx + 64) * sizeof(float));
     const int nlam = N * s->nx, ndz = s->traj + s->nu;
     gato::Args ab = s->a;  // the step at the solution, on its own multipliers and dz
     ab.exit_rule = GATO_EXIT_PCG0;
@@ -2086,7 +2175,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->lam_w, (size_t)B * nlam);
     hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, ab);
     launch_schur<PL>(B, blk, lds, st, ab);
-    if (np == 1) hipLaunchKernelGGL((gato::k_btd_factor<PL>), dim3(B), dim3(64), 0, st, ab, s->bwd_G, s->bwd_Di);
+    if (np == 1) hipLaunchKernelGGL((gato::k_btd_factor<PL>), dim3(B), dim3(128), 0, st, ab, s->bwd_G, s->bwd_Di);
     // correction solves of the step: same blocks, right-hand side (q_a, r_a, c_a) -> gam_a, solution (dz_a, lam_a)
     gato::Args ac = ab;
     ac.q = s->q_a;
