@@ -1029,10 +1029,13 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg(Args a, int it) 
 
 // Two PCG solves on one problem's S in lockstep (phase_pcg_reg2), preconditioned with the factorization of S in
 // Gm, Di (k_btd_factor), each with the k_pcg skip rule (quirk Q9); iteration counts to a1.st_pcg[b] and
-// a2.st_pcg[b] when those are set.
+// a2.st_pcg[b] when those are set. Then, per solve i, what k_dz and k_accumulate do for the problem: dz_i and
+// the KKT residual (to a_i.st_kres[b]) from the new multipliers, and, when zi is set, z_i += dz_i, l_i += lambda_i
+// and lambda_i cleared (the next correction's PCG starts from zero).
 template <int PL>
 __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_dual(Args a1, Args a2, const float* Gm,
-                                                                     const float* Di) {
+                                                                     const float* Di, float* z1, float* l1, float* z2,
+                                                                     float* l2) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
     const bool run1 = !((a1.exit_rule == GATO_EXIT_KKT) ? !a1.active[b] : a1.conv[b]);
@@ -1043,6 +1046,21 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_dual(Args a1, Ar
         if (a1.st_pcg) a1.st_pcg[b] = n1;
         if (a2.st_pcg) a2.st_pcg[b] = n2;
     }
+    auto finish = [&](const Args& a, float* z, float* l) {
+        __syncthreads();  // the multipliers are written and the LDS is free
+        const float kr = phase_dz<PL>(a, b, sm);  // ends with a barrier: dz is settled for the adds below
+        if (threadIdx.x == 0) a.st_kres[b] = kr;
+        if (z) {
+            const int nz = a.traj + Plant<PL>::NU, nl = a.N * Plant<PL>::NX;
+            for (int i = threadIdx.x; i < nz; i += blockDim.x) z[(size_t)b * nz + i] += a.dz[(size_t)b * nz + i];
+            for (int i = threadIdx.x; i < nl; i += blockDim.x) {
+                l[(size_t)b * nl + i] += a.lam[(size_t)b * nl + i];
+                a.lam[(size_t)b * nl + i] = 0.f;
+            }
+        }
+    };
+    finish(a1, z1, l1);
+    finish(a2, z2, l2);
 }
 
 template <int PL>
@@ -2208,12 +2226,14 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
         const gato::Args& as = i == 0 ? ab : ac;
         if (i > 0) residual(nullptr, s->a.q, s->a.r, s->a.c, s->dz_b, s->lam_b, s->q_a, s->r_a, s->c_a, s->gam_a);
         residual(dL, nullptr, nullptr, nullptr, s->dz_v, s->lam_v, s->q_w, s->r_w, s->c_w, s->gam_w);
-        if (np == 1) {
-            hipLaunchKernelGGL((gato::k_pcg_dual<PL>), dim3(B), blk, lds_dual, st, as, aw, s->bwd_G, s->bwd_Di);
-        } else {  // S and P stream: a paired streamed PCG measured slower than two launches (more VALU, spills)
-            launch_pcg<PL>(np, B, blk, lds, st, as, 0);
-            launch_pcg<PL>(np, B, blk, lds, st, aw, 0);
+        if (np == 1) {  // the solves, their dz and the accumulation in one launch
+            hipLaunchKernelGGL((gato::k_pcg_dual<PL>), dim3(B), blk, lds_dual, st, as, aw, s->bwd_G, s->bwd_Di,
+                               i > 0 ? s->dz_b : nullptr, i > 0 ? s->lam_b : nullptr, s->dz_v, s->lam_v);
+            continue;
         }
+        // S and P stream: a paired streamed PCG measured slower than two launches (more VALU, spills)
+        launch_pcg<PL>(np, B, blk, lds, st, as, 0);
+        launch_pcg<PL>(np, B, blk, lds, st, aw, 0);
         hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, as, 0);
         if (i > 0)
             hipLaunchKernelGGL(gato::k_accumulate, dim3(B), dim3(256), 0, st, s->dz_b, s->dz_a, s->lam_b, s->lam_a, ndz,
