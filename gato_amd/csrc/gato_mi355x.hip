@@ -1413,21 +1413,27 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_grad(Args a, const f
     }
 }
 
-// backward set-up: flags cleared, the step's multipliers warm-started from the solve's, adjoint buffers zeroed
+// backward set-up: flags cleared, the step's multipliers warm-started from the solve's, the correction and
+// adjoint buffers (lam_a, dz_a, dz_b; dz_v, lam_v, lam_w) zeroed, one work-group per problem
 __global__ void k_bwd_begin(Args a, const float* lam_keep, float* lam_b, float* lam_a, float* dz_b, float* dz_a,
-                            int nlam, int ndz) {
+                            float* dz_v, float* lam_v, float* lam_w, int nlam, int ndz) {
     const int b = blockIdx.x;
     if (threadIdx.x == 0) {
         a.conv[b] = 0;
         a.active[b] = 1;
     }
     for (int i = threadIdx.x; i < nlam; i += blockDim.x) {
-        lam_b[(size_t)b * nlam + i] = lam_keep[(size_t)b * nlam + i];
-        lam_a[(size_t)b * nlam + i] = 0.f;
+        const size_t j = (size_t)b * nlam + i;
+        lam_b[j] = lam_keep[j];
+        lam_a[j] = 0.f;
+        lam_v[j] = 0.f;
+        lam_w[j] = 0.f;
     }
     for (int i = threadIdx.x; i < ndz; i += blockDim.x) {
-        dz_b[(size_t)b * ndz + i] = 0.f;
-        dz_a[(size_t)b * ndz + i] = 0.f;
+        const size_t j = (size_t)b * ndz + i;
+        dz_b[j] = 0.f;
+        dz_a[j] = 0.f;
+        dz_v[j] = 0.f;
     }
 }
 
@@ -2187,10 +2193,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     ab.st_pcg = s->bst_pcg;
     ab.st_kres = s->bst_kres;
     hipLaunchKernelGGL(gato::k_bwd_begin, dim3(B), dim3(256), 0, st, ab, s->a.lam, s->lam_b, s->lam_a, s->dz_b,
-                       s->dz_a, nlam, ndz);
-    hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->dz_v, (size_t)B * ndz);
-    hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->lam_v, (size_t)B * nlam);
-    hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->lam_w, (size_t)B * nlam);
+                       s->dz_a, s->dz_v, s->lam_v, s->lam_w, nlam, ndz);
     hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, ab);
     launch_schur<PL>(B, blk, lds, st, ab);
     if (np == 1) hipLaunchKernelGGL((gato::k_btd_factor<PL>), dim3(B), dim3(128), 0, st, ab, s->bwd_G, s->bwd_Di);
