@@ -191,9 +191,8 @@ def time_scale(steps):
     """iiwa14 N=128 B=4096 figure-8s of seeded radius and phase. Shared log-weights theta = log diag(Q, R) are
     learned with Adam (lr 1e-2) so that the solutions reproduce those of hidden teacher weights; one step is a
     forward solve of all 4096 problems, the backward pass, and the sum over problems of the per-problem weight
-    gradients (the one reduction the path has). The backward runs one refinement round at PCG tolerance 1e-9:
-    the loss sums 4096 problems' gradients, so the two-round default's per-problem accuracy (worth 20% here,
-    where S and P stream and the step and adjoint chains cannot share a PCG launch) is not needed."""
+    gradients (the one reduction the path has). The backward runs with its defaults: two refinement rounds, one
+    PCG iteration per round with the factorization of S as preconditioner."""
     import numpy as np
     import torch
 
@@ -207,7 +206,6 @@ def time_scale(steps):
                             "q_cost", "qd_cost", "u_cost", "N_cost", "q_lim_cost", "vel_lim_cost", "ctrl_lim_cost",
                             "rho")}
     s = Solver(wl.plant, wl.N, wl.B, mode="graph", **kw)
-    s.set_backward_params(max_pcg_iters=1000, pcg_tol=1e-9, refine=1)
     dev = s.device
     t = lambda a: torch.tensor(a, dtype=torch.float32, device=dev).contiguous()
     xs, ref, xu0 = t(wl.xs), t(wl.ref), t(wl.xu)
@@ -243,7 +241,7 @@ def time_scale(steps):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     return {"workload": "iiwa14 N=128 B=4096 figure-8s (seeded radius/phase), shared diag Q/R learned by Adam",
-            "adam_steps": steps, "backward": "1 refinement round, adjoint PCG tolerance 1e-9",
+            "adam_steps": steps, "backward": "defaults: 2 refinement rounds, 1 PCG iteration per round (factorization preconditioner)",
             "fwd_bwd_solves_per_s": B * steps / el, "ms_per_step": 1e3 * el / steps,
             "loss_first": losses[0], "loss_last": losses[-1],
             "compulsory_hbm_GBs": 4 * (nx + 6 * wl.N + 2 * s.traj + 2 * nx * wl.N + 7) * B * steps / el / 1e9}

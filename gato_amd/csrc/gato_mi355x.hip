@@ -168,15 +168,23 @@ __device__ __forceinline__ float btd_row(const float* D, const float* L, const f
     return s;
 }
 
-// Warm-started PCG on S lambda = gamma (pcg.cuh:13-148). Returns the iteration count.
-template <int PL, int NPASS>
-__device__ __forceinline__ int phase_pcg(const Args& a, int b, float eps, float* sm) {
+template <int PL>
+__device__ void btd_apply(int N, const float* Gp, const float* Dp, const float* rv, float* zv, float* pv);
+
+// Warm-started PCG on S lambda = gamma (pcg.cuh:13-148). Returns the iteration count. BTD: precondition with the
+// twisted factorization of S (Gp, Dp: the problem's blocks from k_btd_factor, btd_apply) instead of P, for the
+// backward pass; the LDS then also holds z and the sweeps' middle vectors (3 (N + 2) 16 + 48 floats in all).
+template <int PL, int NPASS, bool BTD = false>
+__device__ __forceinline__ int phase_pcg(const Args& a, int b, float eps, float* sm, const float* Gp = nullptr,
+                                         const float* Dp = nullptr) {
     constexpr int NX = Plant<PL>::NX, XP = 16;
     View<PL> v(a, b);
     const int N = a.N, tid = threadIdx.x, g = tid / KG, lane = tid % KG, G = blockDim.x / KG;
     float* sp = sm;
     float* sr = sp + (N + 2) * XP;
     float* red = sr + (N + 2) * XP;
+    float* sz = red + 16;
+    float* pv = sz + (N + 2) * XP;
     for (int i = tid; i < 2 * (N + 2) * XP; i += blockDim.x) sm[i] = 0.f;
     __syncthreads();
     float x[NPASS], r[NPASS], z[NPASS], p[NPASS], Ap[NPASS];
@@ -196,10 +204,12 @@ __device__ __forceinline__ int phase_pcg(const Args& a, int b, float eps, float*
     for (int s = 0; s < NPASS; s++)
         if (val[s]) sr[(kk[s] + 1) * XP + lane] = r[s];
     __syncthreads();
+    if constexpr (BTD) btd_apply<PL>(N, Gp, Dp, sr, sz, pv);
     float loc = 0.f;
 #pragma unroll
     for (int s = 0; s < NPASS; s++) {
-        z[s] = val[s] ? btd_row<NX>(v.Pd, v.Po, sr, N, kk[s], lane) : 0.f;
+        if constexpr (BTD) z[s] = val[s] ? sz[(kk[s] + 1) * XP + lane] : 0.f;
+        else z[s] = val[s] ? btd_row<NX>(v.Pd, v.Po, sr, N, kk[s], lane) : 0.f;
         p[s] = z[s];
         loc += r[s] * z[s];
     }
@@ -227,10 +237,13 @@ __device__ __forceinline__ int phase_pcg(const Args& a, int b, float eps, float*
             if (val[s]) sr[(kk[s] + 1) * XP + lane] = r[s];
         }
         __syncthreads();
+        if (it >= a.max_pcg) break;  // x is final at the cap: its last preconditioner application is not needed
+        if constexpr (BTD) btd_apply<PL>(N, Gp, Dp, sr, sz, pv);
         loc = 0.f;
 #pragma unroll
         for (int s = 0; s < NPASS; s++) {
-            z[s] = val[s] ? btd_row<NX>(v.Pd, v.Po, sr, N, kk[s], lane) : 0.f;
+            if constexpr (BTD) z[s] = val[s] ? sz[(kk[s] + 1) * XP + lane] : 0.f;
+            else z[s] = val[s] ? btd_row<NX>(v.Pd, v.Po, sr, N, kk[s], lane) : 0.f;
             loc += r[s] * z[s];
         }
         const float rn = block_sum(loc, red);
@@ -341,6 +354,7 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
         r -= alpha * Ap;
         if (val) sr[(kk + 1) * XP + l] = r;
         __syncthreads();
+        if (it >= a.max_pcg) break;  // x is final at the cap: its last preconditioner application is not needed
         z = val ? mv(pd, pl, pu, sr) : 0.f;
         const float rn = block_sum1(r * z, red + 32 * slot);
         slot ^= 1;
@@ -599,6 +613,22 @@ __device__ __forceinline__ void btd_out(bool top, int lane, int base, int N, con
 #pragma unroll
         for (int j = 0; j < NX; j++) gc[j] = gn[j];
     }
+}
+
+// z = M^-1 r with the twisted factorization: btd_ends on knot groups 0 (top) and 4 (bottom; group 1 in a
+// work-group of fewer than eight groups), btd_middle, btd_out, separated by work-group barriers (every thread of the
+// work-group calls this). r in the padded LDS vector rv, z to zv, pv: 32 floats of LDS.
+template <int PL>
+__device__ void btd_apply(int N, const float* Gp, const float* Dp, const float* rv, float* zv, float* pv) {
+    const int g = threadIdx.x / KG, lane = threadIdx.x % KG, gb = blockDim.x / KG >= 8 ? 4 : 1;
+    if (g == 0) btd_ends<PL>(true, lane, 0, N, Gp, Dp, rv, zv, pv);
+    if (g == gb) btd_ends<PL>(false, lane, (gb & 3) * KG, N, Gp, Dp, rv, zv, pv + 16);
+    __syncthreads();
+    if (g == 0) btd_middle<PL>(lane, 0, N, Dp, rv, zv, pv, pv + 16);
+    __syncthreads();
+    if (g == 0) btd_out<PL>(true, lane, 0, N, Gp, zv);
+    if (g == gb) btd_out<PL>(false, lane, (gb & 3) * KG, N, Gp, zv);
+    __syncthreads();
 }
 
 // Two PCG solves on the same S in lockstep (the backward pass's step and adjoint chains), preconditioned with the
@@ -1061,6 +1091,20 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_dual(Args a1, Ar
     };
     finish(a1, z1, l1);
     finish(a2, z2, l2);
+}
+
+// One PCG solve with S and its rows streamed (several knots per knot group), preconditioned with the twisted
+// factorization of S in Gm, Di (k_btd_factor) read from global memory: the backward pass's solves when N > 32.
+// Skip rule and iteration count as k_pcg (st_pcg[b]).
+template <int PL, int NPASS>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_btd(Args a, const float* Gm, const float* Di) {
+    extern __shared__ __align__(16) float sm[];
+    const int b = blockIdx.x;
+    const bool skip = (a.exit_rule == GATO_EXIT_KKT) ? !a.active[b] : a.conv[b];  // quirk Q9 (pcg.cuh:30-33)
+    const size_t pb = (size_t)b * a.N * Plant<PL>::NX * Plant<PL>::NX;
+    int n = 0;
+    if (!skip) n = phase_pcg<PL, NPASS, true>(a, b, a.tol[b], sm, Gm + pb, Di + pb);
+    if (threadIdx.x == 0 && a.st_pcg) a.st_pcg[b] = n;
 }
 
 template <int PL>
@@ -1820,7 +1864,7 @@ struct gato_solver {
     hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
     // backward workspace: the step at the solution (lam_b, dz_b) and its correction solves (q_a, r_a, c_a,
     // gam_a, lam_a, dz_a); the adjoint solution (dz_v, lam_v) and its correction solves (q_w, r_w, c_w, gam_w,
-    // lam_w, dz_w); the factorization of S that preconditions the paired PCG (bwd_G, bwd_Di: N <= 32 only); the
+    // lam_w, dz_w); the factorization of S that preconditions the backward's PCG (bwd_G, bwd_Di); the
     // backward PCG settings and the iteration counts of the two chains
     float *bwd_G = nullptr, *bwd_Di = nullptr;
     float *lam_b = nullptr, *dz_b = nullptr, *q_a = nullptr, *r_a = nullptr, *c_a = nullptr, *gam_a = nullptr,
@@ -1922,9 +1966,8 @@ size_t layout(gato_solver* s, bool assign) {
     t = F(B * N * nx); if (assign) s->gam_w = t;
     t = F(B * N * nx); if (assign) s->lam_w = t;
     t = F(B * (s->traj + nu)); if (assign) s->dz_w = t;
-    const size_t nbt = N <= 32 ? (size_t)B * N * nx * nx : 0;  // used by the paired PCG, one knot per knot group
-    t = F(nbt); if (assign) s->bwd_G = t;
-    t = F(nbt); if (assign) s->bwd_Di = t;
+    t = F(B * N * nx * nx); if (assign) s->bwd_G = t;  // the backward's factorization of S
+    t = F(B * N * nx * nx); if (assign) s->bwd_Di = t;
     t = F(B); if (assign) a.st_init = t;
     t = F(B); if (assign) a.st_final = t;
     return off + 256;
@@ -1957,6 +2000,15 @@ void launch_merit(int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args&
         hipLaunchKernelGGL((gato::k_merit_n<PL>), dim3(B), dim3(256), lds, st, a, nal, which, it);
     else
         hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, nal, which, it);
+}
+
+// The backward's streamed PCG with the factorization preconditioner (k_pcg_btd), np knots per knot group.
+template <int PL>
+void launch_pcg_btd(int np, int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a, const float* Gm,
+                    const float* Di) {
+    if (np == 2) hipLaunchKernelGGL((gato::k_pcg_btd<PL, 2>), dim3(B), blk, lds, st, a, Gm, Di);
+    else if (np == 4) hipLaunchKernelGGL((gato::k_pcg_btd<PL, 4>), dim3(B), blk, lds, st, a, Gm, Di);
+    else hipLaunchKernelGGL((gato::k_pcg_btd<PL, 8>), dim3(B), blk, lds, st, a, Gm, Di);
 }
 
 template <int PL>
@@ -2171,9 +2223,10 @@ int launch_solve(gato_solver* s, hipStream_t st) {
 
 
 // The backward pass: the step at the solution and the adjoint system, each solved with the step's Schur
-// complement, preconditioned with its block-tridiagonal factorization (N <= 32: k_btd_factor, k_pcg_dual) or
-// with the step's preconditioner P (otherwise), and improved by `bwd_refine` rounds of iterative refinement (the
-// residual is evaluated with the KKT blocks themselves, so fp32 error in S and Q~^-1 does not limit the result).
+// complement, preconditioned with its twisted block-tridiagonal factorization (k_btd_factor; k_pcg_dual for both
+// chains with the rows of S in registers when N <= 32, k_pcg_btd per chain otherwise), and improved by
+// `bwd_refine` rounds of iterative refinement (the residual is evaluated with the KKT blocks themselves, so fp32
+// error in S and Q~^-1 does not limit the result).
 template <int PL>
 int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, float* gQ, float* gR, hipStream_t st) {
     const int B = s->B, N = s->N, np = npass(s);
@@ -2183,6 +2236,8 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     // sweeps' middle vectors in LDS
     const size_t lds_dual =
         std::max(lds, (size_t)(6 * (N + 2) * 16 + 128 + 2 * N * s->nx * s->nx + 64) * sizeof(float));
+    // the streamed PCG with the factorization keeps p, r, z, the reduction slots and the middle vectors in LDS
+    const size_t lds_btd = std::max(lds, (size_t)(3 * (N + 2) * 16 + 16 + 32) * sizeof(float));
     const int nlam = N * s->nx, ndz = s->traj + s->nu;
     gato::Args ab = s->a;  // the step at the solution, on its own multipliers and dz
     ab.exit_rule = GATO_EXIT_PCG0;
@@ -2196,7 +2251,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
                        s->dz_a, s->dz_v, s->lam_v, s->lam_w, nlam, ndz);
     hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, ab);
     launch_schur<PL>(B, blk, lds, st, ab);
-    if (np == 1) hipLaunchKernelGGL((gato::k_btd_factor<PL>), dim3(B), dim3(128), 0, st, ab, s->bwd_G, s->bwd_Di);
+    hipLaunchKernelGGL((gato::k_btd_factor<PL>), dim3(B), dim3(128), 0, st, ab, s->bwd_G, s->bwd_Di);
     // correction solves of the step: same blocks, right-hand side (q_a, r_a, c_a) -> gam_a, solution (dz_a, lam_a)
     gato::Args ac = ab;
     ac.q = s->q_a;
@@ -2204,7 +2259,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     ac.gam = s->gam_a;
     ac.lam = s->lam_a;
     ac.dz = s->dz_a;
-    ac.max_pcg = s->bwd_max_pcg > 0 ? s->bwd_max_pcg : (np == 1 ? 1 : 1000);
+    ac.max_pcg = s->bwd_max_pcg > 0 ? s->bwd_max_pcg : 1;
     ac.tol = s->d_bwd_tol;
     ac.pcg_abs_tol = 1e-30f;
     ac.st_pcg = s->bst_pcg_s;
@@ -2234,9 +2289,9 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
                                i > 0 ? s->dz_b : nullptr, i > 0 ? s->lam_b : nullptr, s->dz_v, s->lam_v);
             continue;
         }
-        // S and P stream: a paired streamed PCG measured slower than two launches (more VALU, spills)
-        launch_pcg<PL>(np, B, blk, lds, st, as, 0);
-        launch_pcg<PL>(np, B, blk, lds, st, aw, 0);
+        // S streams: the chains are separate launches (a paired streamed PCG measured slower than two launches)
+        launch_pcg_btd<PL>(np, B, blk, lds_btd, st, as, s->bwd_G, s->bwd_Di);
+        launch_pcg_btd<PL>(np, B, blk, lds_btd, st, aw, s->bwd_G, s->bwd_Di);
         hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, as, 0);
         if (i > 0)
             hipLaunchKernelGGL(gato::k_accumulate, dim3(B), dim3(256), 0, st, s->dz_b, s->dz_a, s->lam_b, s->lam_a, ndz,
