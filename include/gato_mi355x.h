@@ -135,11 +135,11 @@ int gato_sim_forward(gato_solver* s, float* d_xnext, const float* d_x, const flo
 int gato_backward(gato_solver* s, const float* d_xu, const float* d_xs, const float* d_ref, float dt,
                   const float* d_dL_dxu, float* d_grad_xs, float* d_grad_ref, float* d_grad_Qd, float* d_grad_Rd,
                   void* stream, int32_t* h_adj_pcg_iters);
-/* Backward solves: PCG iteration cap per round (-1, the default: 1 where M is the factorization, 1000 where it
- * is P), relative tolerance on the preconditioned residual r^T M^-1 r (default 1e-6) and rounds of iterative
- * refinement of the step and adjoint solutions (default 2, at most 8). With one knot per knot group (N <= 32)
- * M is the block-tridiagonal factorization of the Schur complement, so one PCG iteration per round is a solve
- * to the fp32 floor and the refinement rounds correct it; otherwise M^-1 is the step's preconditioner P. */
+/* Backward solves: PCG iteration cap per round (-1, the default: 1), relative tolerance on the preconditioned
+ * residual r^T M^-1 r (default 1e-6) and rounds of iterative refinement of the step and adjoint solutions
+ * (default 2, at most 8). M is the twisted block-tridiagonal factorization of the Schur complement, equal to it up
+ * to fp32 rounding, so one PCG iteration per round is a solve to the fp32 floor and the refinement rounds, whose
+ * residuals come from the KKT blocks, correct it. */
 int gato_set_backward_params(gato_solver* s, int32_t max_pcg_iters, float pcg_tol, int32_t refine);
 
 /* Extension: launch shape. `groups` 16-lane knot groups per work-group (a multiple of 4 in [4, 32], at most
