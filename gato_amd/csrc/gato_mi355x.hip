@@ -722,16 +722,26 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
     // 16 groups or more) while the others wait at the barriers
     const int role = k == rg[0] ? 0 : k == rg[1] ? 1 : k == rg[2] ? 2 : k == rg[3] ? 3 : -1;
     const bool rtop = (role & 1) == 0;
-    const int rbase = (k & 3) * KG;
     const float* rrv = role < 2 ? sr1 : sr2;
     float* rzv = role < 2 ? sz1 : sz2;
+    // With 16 groups or more each sweep side has a wave of its own (lanes 0..15). Groups that share a wave (fewer
+    // groups) take turns, one wave slot q at a time: the sweeps broadcast with readlane from lanes base + c, so
+    // only one group of a wave may run them at once.
+    const int nslot = G >= 16 ? 1 : 4;
     auto precond = [&](bool d1, bool d2, float& z1o, float& z2o) {
         const bool mine = role >= 0 && (role < 2 ? d1 : d2);
-        if (mine) btd_ends<PL>(rtop, lane, rbase, N, sG, sD, rrv, rzv, pv + role * XP);
+#pragma unroll 1
+        for (int q = 0; q < nslot; q++)
+            if (mine && (k & 3) == q) btd_ends<PL>(rtop, lane, q * KG, N, sG, sD, rrv, rzv, pv + role * XP);
         __syncthreads();
-        if (mine && rtop) btd_middle<PL>(lane, rbase, N, sD, rrv, rzv, pv + role * XP, pv + (role + 1) * XP);
+#pragma unroll 1
+        for (int q = 0; q < nslot; q++)
+            if (mine && rtop && (k & 3) == q)
+                btd_middle<PL>(lane, q * KG, N, sD, rrv, rzv, pv + role * XP, pv + (role + 1) * XP);
         __syncthreads();
-        if (mine) btd_out<PL>(rtop, lane, rbase, N, sG, rzv);
+#pragma unroll 1
+        for (int q = 0; q < nslot; q++)
+            if (mine && (k & 3) == q) btd_out<PL>(rtop, lane, q * KG, N, sG, rzv);
         __syncthreads();
         z1o = (val && d1) ? sz1[o] : 0.f;
         z2o = (val && d2) ? sz2[o] : 0.f;

@@ -34,12 +34,19 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
 
 
-@pytest.mark.parametrize("plant,hess,term,N", [("iiwa14", 0, 0, 16), ("iiwa14", 1, 1, 16), ("indy7", 0, 0, 16),
-                                                ("iiwa14", 0, 0, 32), ("iiwa14", 0, 0, 128)])
-def test_backward_matches_oracle_step_gradient(gpu, O, plant, hess, term, N):
-    # N = 32: the differentiation configuration's horizon; N = 128: the scale configuration's
+@pytest.mark.parametrize("plant,hess,term,N,groups", [("iiwa14", 0, 0, 16, None), ("iiwa14", 1, 1, 16, None),
+                                                       ("indy7", 0, 0, 16, None), ("iiwa14", 0, 0, 32, None),
+                                                       ("iiwa14", 0, 0, 128, None), ("iiwa14", 0, 0, 5, None),
+                                                       ("indy7", 0, 0, 3, None), ("iiwa14", 0, 0, 16, 4)])
+def test_backward_matches_oracle_step_gradient(gpu, O, plant, hess, term, N, groups):
+    # N = 32: the differentiation configuration's horizon; N = 128: the scale configuration's (streamed PCG);
+    # N = 5 and 3: odd horizons on small work-groups (eight and four knot groups), where the preconditioner's
+    # four sweep sides share waves and the factorization's middle knot sits off centre; N = 16 on four knot groups:
+    # the streamed PCG with both sweep sides in the work-group's one wave
     B = 4 if N <= 32 else 2
     s, wl, Qd, Rd, kw = _problem(plant, N, B, seed=21, hess_mode=hess, terminal_mode=term)
+    if groups is not None:
+        s.set_launch_shape(groups)
     dev = s.device
     xu = torch.tensor(wl.xu, dtype=torch.float32, device=dev)
     xs = torch.tensor(wl.xs, dtype=torch.float32, device=dev)
