@@ -1287,11 +1287,10 @@ __global__ void __launch_bounds__(512) k_fused(Args a) {
 //   gamma_0 = c_0 - Q~_0^-1 q'_0,  gamma_{k+1} = c_{k+1} - Q~_{k+1}^-1 q'_{k+1} + phi_k q'_k + B_k R_k^-1 r'_k,
 // the same expression as the Schur phase's gamma (schur_linsys.cuh:60-98), for arbitrary (q', r', c').
 template <int PL>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_gamma_rhs(Args a, const float* qv, const float* rv,
-                                                                      const float* cv, float* gam) {
+__device__ __forceinline__ void gamma_rhs_body(const Args& a, int b, const float* qv, const float* rv,
+                                               const float* cv, float* gam) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
-    const int b = blockIdx.x;
     View<PL> v(a, b);
     const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
     const float* qb = qv + (size_t)b * N * NX;
@@ -1324,14 +1323,11 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_gamma_rhs(Args a, co
 //   r_c0 = rhs_c0 + z_x0,  r_c{k+1} = rhs_c{k+1} - A_k z_xk - B_k z_uk + z_x{k+1}.
 // rhs_z is rz (trajectory layout) when given, else -(gq, gr); rhs_c is cc when given, else 0.
 template <int PL>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_kkt_resid(Args a, const float* rz,
-                                                                      const float* gq, const float* gr,
-                                                                      const float* cc, const float* zall,
-                                                                      const float* lall, float* qo, float* ro,
-                                                                      float* co) {
+__device__ __forceinline__ void kkt_resid_body(const Args& a, int b, const float* rz, const float* gq,
+                                               const float* gr, const float* cc, const float* zall,
+                                               const float* lall, float* qo, float* ro, float* co) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU, ST = NX + NU;
-    const int b = blockIdx.x;
     View<PL> v(a, b);
     const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
     const float rho = a.rho[b];
@@ -1391,6 +1387,24 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_kkt_resid(Args a, co
             ro[(size_t)b * N * NU + k * NU + i] = ru;
         }
     }
+}
+
+// One backward chain's residual inputs and outputs (kkt_resid_body's and gamma_rhs_body's arguments).
+struct ResidChain {
+    const float *rz, *gq, *gr, *cc, *z, *l;
+    float *q, *r, *c, *gam;
+};
+
+// A refinement round's residuals and Schur right-hand sides for both backward chains, problem by problem:
+// kkt_resid_body and then gamma_rhs_body for chain 1 (when do1) and chain 2, in one launch.
+template <int PL>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_residual2(Args a, ResidChain c1, ResidChain c2, int do1) {
+    const int b = blockIdx.x;
+    if (do1) kkt_resid_body<PL>(a, b, c1.rz, c1.gq, c1.gr, c1.cc, c1.z, c1.l, c1.q, c1.r, c1.c);
+    kkt_resid_body<PL>(a, b, c2.rz, c2.gq, c2.gr, c2.cc, c2.z, c2.l, c2.q, c2.r, c2.c);
+    __syncthreads();  // the problem's residual rows are written before the right-hand sides read their neighbours
+    if (do1) gamma_rhs_body<PL>(a, b, c1.q, c1.r, c1.c, c1.gam);
+    gamma_rhs_body<PL>(a, b, c2.q, c2.r, c2.c, c2.gam);
 }
 
 // z += dz, l += dl; dl is cleared (the next correction's PCG starts from zero)
@@ -2283,17 +2297,14 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     aw.lam = s->lam_w;
     aw.dz = s->dz_w;
     aw.st_pcg = s->bst_pcg;
-    auto residual = [&](const float* rz, const float* gq, const float* gr, const float* cc, const float* z,
-                        const float* l, float* q, float* r, float* c, float* gam) {
-        hipLaunchKernelGGL((gato::k_kkt_resid<PL>), dim3(B), blk, lds, st, ab, rz, gq, gr, cc, z, l, q, r, c);
-        hipLaunchKernelGGL((gato::k_gamma_rhs<PL>), dim3(B), blk, lds, st, ab, q, r, c, gam);
-    };
+    // residuals of the step's correction (from round 1) and of the adjoint: (q, r, c) and gamma for each chain
+    const gato::ResidChain rs{nullptr, s->a.q, s->a.r, s->a.c, s->dz_b, s->lam_b, s->q_a, s->r_a, s->c_a, s->gam_a};
+    const gato::ResidChain rw{dL, nullptr, nullptr, nullptr, s->dz_v, s->lam_v, s->q_w, s->r_w, s->c_w, s->gam_w};
     // the two chains are independent and share K's Schur complement and preconditioner: round i solves the
     // step's correction i (round 0: the step itself) and the adjoint's correction i side by side
     for (int i = 0; i <= s->bwd_refine; i++) {
         const gato::Args& as = i == 0 ? ab : ac;
-        if (i > 0) residual(nullptr, s->a.q, s->a.r, s->a.c, s->dz_b, s->lam_b, s->q_a, s->r_a, s->c_a, s->gam_a);
-        residual(dL, nullptr, nullptr, nullptr, s->dz_v, s->lam_v, s->q_w, s->r_w, s->c_w, s->gam_w);
+        hipLaunchKernelGGL((gato::k_residual2<PL>), dim3(B), blk, lds, st, ab, rs, rw, i > 0 ? 1 : 0);
         if (np == 1) {  // the solves, their dz and the accumulation in one launch
             hipLaunchKernelGGL((gato::k_pcg_dual<PL>), dim3(B), blk, lds_dual, st, as, aw, s->bwd_G, s->bwd_Di,
                                i > 0 ? s->dz_b : nullptr, i > 0 ? s->lam_b : nullptr, s->dz_v, s->lam_v);
