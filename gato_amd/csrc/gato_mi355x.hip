@@ -2297,11 +2297,11 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     aw.lam = s->lam_w;
     aw.dz = s->dz_w;
     aw.st_pcg = s->bst_pcg;
-    // residuals of the step's correction (from round 1) and of the adjoint: (q, r, c) and gamma for each chain
+    // residuals of the step's correction (from refinement round 1 on) and of the adjoint: (q, r, c) and gamma
     const gato::ResidChain rs{nullptr, s->a.q, s->a.r, s->a.c, s->dz_b, s->lam_b, s->q_a, s->r_a, s->c_a, s->gam_a};
     const gato::ResidChain rw{dL, nullptr, nullptr, nullptr, s->dz_v, s->lam_v, s->q_w, s->r_w, s->c_w, s->gam_w};
     // the two chains are independent and share K's Schur complement and preconditioner: round i solves the
-    // step's correction i (round 0: the step itself) and the adjoint's correction i side by side
+    // step's correction i (refinement round 0: the step itself) and the adjoint's correction i side by side
     for (int i = 0; i <= s->bwd_refine; i++) {
         const gato::Args& as = i == 0 ? ab : ac;
         hipLaunchKernelGGL((gato::k_residual2<PL>), dim3(B), blk, lds, st, ab, rs, rw, i > 0 ? 1 : 0);
