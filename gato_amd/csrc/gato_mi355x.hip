@@ -2284,6 +2284,7 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
     ac.lam = s->lam_a;
     ac.dz = s->dz_a;
     ac.max_pcg = s->bwd_max_pcg > 0 ? s->bwd_max_pcg : 1;
+    ab.max_pcg = ac.max_pcg;  // the step's own solve too: the refinement rounds correct it like its corrections
     ac.tol = s->d_bwd_tol;
     ac.pcg_abs_tol = 1e-30f;
     ac.st_pcg = s->bst_pcg_s;
