@@ -1155,7 +1155,7 @@ __device__ __forceinline__ int check_problem(const Args& a, int b, int it, int n
 // convergence bookkeeping, adding it to the converged count solved[it] (zero at the start of the solve). The
 // batch-level exit test on that count runs at the start of the line search (merit_body, which == 3).
 template <int PL, int NPASS>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_dz(Args a, int it) {
+__device__ __forceinline__ void pcg_dz_body(const Args& a, int it) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
     if (stopped(a)) return;
@@ -1173,6 +1173,20 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_dz(Args a, int i
         a.st_kres[it * a.B + b] = kr;
         if (check_problem(a, b, it, n, kr)) atomicAdd(&a.solved[it], 1);
     }
+}
+
+template <int PL, int NPASS>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_dz(Args a, int it) {
+    pcg_dz_body<PL, NPASS>(a, it);
+}
+
+// The same in 1024-thread work-groups (64 knot groups) for the streamed PCG (NPASS >= 2, 64 < N <= 128): half the
+// knots per lane and twice the waves per problem of the 512-thread launch, at the same 128-VGPR budget (one
+// work-group per CU). For batches too small to fill the GPU with 512-thread work-groups.
+template <int PL, int NPASS>
+__global__ void __launch_bounds__(1024) GATO_TWO_WG_PER_CU k_pcg_dz_wide(Args a, int it) {
+    static_assert(NPASS >= 2, "the register-resident PCG's reductions hold at most 32 knot groups");
+    pcg_dz_body<PL, NPASS>(a, it);
 }
 
 // The convergence bookkeeping (check_problem) for every problem of the batch in one work-group, the converged
@@ -2131,10 +2145,16 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     hipLaunchKernelGGL(gato::k_begin, dim3(sb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
     launch_merit<PL>(B, blk, lds, st, a, 1, 0, 0);
+    // 64 < N <= 128: the PCG in 1024-thread work-groups, two knots per knot group (k_pcg_dz_wide)
+    const bool wide = N > 64 && N <= 128;
+    const size_t lds_wide = std::max(lds, (size_t)(64 * gato::KG + 16) * sizeof(float));
     for (int it = 0; it < iters; it++) {
         hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
         launch_schur<PL>(B, blk, lds, st, a);
-        launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
+        if (wide)
+            hipLaunchKernelGGL((gato::k_pcg_dz_wide<PL, 2>), dim3(B), dim3(1024), lds_wide, st, a, it);
+        else
+            launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
         launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 3, it);
     }
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
