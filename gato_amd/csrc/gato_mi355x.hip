@@ -1560,8 +1560,14 @@ __global__ void __launch_bounds__(1024) k_select(const float* x, const float* u,
     se[threadIdx.x] = emin;
     si[threadIdx.x] = imin;
     __syncthreads();
-    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
+    // Tree over n = blockDim.x slots, which need not be a power of two: the first step starts at half the next
+    // power of two and folds only the slots that exist (i + h < n) into [0, h); every later step halves a power
+    // of two. `better` is a total order, so the fold order does not change the winner.
+    const int n = blockDim.x;
+    int h = 1;
+    while (2 * h < n) h *= 2;
+    for (; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h && (int)threadIdx.x + h < n) {
             const float e2 = se[threadIdx.x + h];
             const int i2 = si[threadIdx.x + h];
             if (better(e2, i2, se[threadIdx.x], si[threadIdx.x])) {
@@ -2892,7 +2898,8 @@ int gato_debug_copy(gato_solver* s, const char* name, void* d_dst, int64_t cap, 
                      {"So", a.So, B * N * nx * nx},       {"Pd", a.Pd, B * N * nx * nx},  {"Po", a.Po, B * N * nx * nx},
                      {"gamma", a.gam, B * N * nx},        {"lambda", a.lam, B * N * nx},  {"dz", a.dz, B * (s->traj + nu)},
                      {"merit_parts", a.mpart, B * gato::NUM_ALPHAS * N}, {"pcg_iters", a.st_pcg, it * B},
-                     {"kkt_residual", a.st_kres, it * B}, {"f_ext", s->d_fext, B * 6}};
+                     {"kkt_residual", a.st_kres, it * B}, {"f_ext", s->d_fext, B * 6},
+                     {"Qd", s->d_Qd, B * nx},             {"Rd", s->d_Rd, B * nu}};
     for (const E& e : tab) {
         if (std::strcmp(e.n, name) != 0) continue;
         if (count) *count = (int64_t)e.c;

@@ -76,6 +76,8 @@ class MixedSolver:
         pref = P(*[t.data_ptr() for t in ref])
         st = (_lib.GatoStats * n)(*self._stats) if stats else None
         dt = self.cfg["dt"] if dt is None else dt
+        for v in self._views:
+            v._touch()
         check(self.lib.gato_mixed_solve(self._h, pxu, pxs, pref, float(dt),
                                         C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream), st),
               "gato_mixed_solve")

@@ -86,6 +86,7 @@ class ForceEstimator:
     def generate_batch(self, q):
         """Hypotheses at joint positions q: solver wrenches set on the device; returns the world batch [B, 6]."""
         qd = _dev_f32(q, self.solver.nq, "q", self.device)
+        self.solver._touch()  # the solver's wrenches change
         check(self.lib.gato_force_est_generate(self._h, self.solver._h, _ptr(qd), _ptr(self._world), self._stream()),
               "gato_force_est_generate")
         return self._world.view(self.batch_size, 6)

@@ -80,6 +80,7 @@ class Solver:
         else:
             self._h = _handle  # a class of a MixedSolver: owned by it
         self.mode, self.exit_rule = mode, exit_rule
+        self._pending = []  # backward passes waiting for the state their forward solve left (see _touch)
         f = dict(dtype=torch.float32, device=self.device)
         self._xu = torch.zeros(self.B, self.traj, **f)
         self._xs = torch.zeros(self.B, self.nx, **f)
@@ -110,6 +111,16 @@ class Solver:
             self._h = None
 
     # ---------------------------------------------------------------- helpers
+    def _touch(self):
+        """Called before anything changes the solver's state (multipliers, rho, wrenches, weights): a backward
+        pass still waiting for the state its forward solve left receives a snapshot of it first."""
+        waiting = [p for p in self._pending if p.snap is None and not p.done]
+        self._pending = []
+        if waiting:
+            snap = self.snapshot()
+            for p in waiting:
+                p.snap = snap
+
     def _stream(self):
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
@@ -146,6 +157,7 @@ class Solver:
                                     for k in ("sqp_iters", "kkt_converged", "initial_merit", "final_merit", "pcg_iters",
                                               "ls_min_merit", "ls_step_size", "kkt_residual")])
         with torch.cuda.device(self.device):
+            self._touch()
             check(self.lib.gato_solve(self._h, _ptr(xu), _ptr(xs), _ptr(ref), float(timestep), self._stream(),
                                       C.byref(gs)), "gato_solve")
         n_ls = gs.ls_num_iters
@@ -181,6 +193,7 @@ class Solver:
             gs = GatoStats(0.0, 0, 0, *[st[k].ctypes.data_as(C.POINTER(C.c_int32 if st[k].dtype == np.int32 else C.c_float))
                                         for k in ("sqp_iters", "kkt_converged", "initial_merit", "final_merit",
                                                   "pcg_iters", "ls_min_merit", "ls_step_size", "kkt_residual")])
+        self._touch()
         check(self.lib.gato_solve(self._h, _ptr(xu), _ptr(xs), _ptr(ref), float(dt), self._stream(),
                                   C.byref(gs) if gs is not None else None), "gato_solve")
         if stats:
@@ -190,23 +203,28 @@ class Solver:
     def set_f_ext_batch(self, f_ext_batch):
         f = torch.zeros(self.B * 6, dtype=torch.float32, device=self.device)
         self._to_dev(f_ext_batch, f, "f_ext_batch")
+        self._touch()
         check(self.lib.gato_set_f_ext_batch(self._h, _ptr(f), self._stream()), "gato_set_f_ext_batch")
 
     def set_rho_penalty_batch(self, rho_batch, set_as_reset_default=True):
         a = self._host_f32(rho_batch, self.B, "rho_batch")
+        self._touch()
         check(self.lib.gato_set_rho_penalty_batch(self._h, _fptr(a), int(bool(set_as_reset_default))),
               "gato_set_rho_penalty_batch")
 
     def set_drho_batch(self, drho_batch, set_as_reset_default=True):
         a = self._host_f32(drho_batch, self.B, "drho_batch")
+        self._touch()
         check(self.lib.gato_set_drho_batch(self._h, _fptr(a), int(bool(set_as_reset_default))), "gato_set_drho_batch")
 
     def set_mu_batch(self, mu_batch):
         a = self._host_f32(mu_batch, self.B, "mu_batch")
+        self._touch()
         check(self.lib.gato_set_mu_batch(self._h, _fptr(a)), "gato_set_mu_batch")
 
     def set_pcg_tol_batch(self, eps_batch):
         a = self._host_f32(eps_batch, self.B, "eps_batch")
+        self._touch()
         check(self.lib.gato_set_pcg_tol_batch(self._h, _fptr(a)), "gato_set_pcg_tol_batch")
 
     def sim_forward(self, xk, uk, dt):
@@ -218,18 +236,22 @@ class Solver:
         return self._xn.cpu().numpy()
 
     def reset_dual(self):
+        self._touch()
         check(self.lib.gato_reset_dual(self._h), "gato_reset_dual")
 
     def reset_rho(self):
+        self._touch()
         check(self.lib.gato_reset_rho(self._h), "gato_reset_rho")
 
     def set_rho_adaptation(self, enabled):
+        self._touch()
         check(self.lib.gato_set_rho_adaptation(self._h, int(bool(enabled))), "gato_set_rho_adaptation")
 
     # ---------------------------------------------------------------- extensions
     def set_mode(self, mode=None, exit_rule=None):
         self.mode = mode or self.mode
         self.exit_rule = exit_rule or self.exit_rule
+        self._touch()
         check(self.lib.gato_set_mode(self._h, MODES[self.mode], EXITS[self.exit_rule]), "gato_set_mode")
 
     def set_joint_cost(self, Qd=None, Rd=None, xref=None):
@@ -242,6 +264,7 @@ class Solver:
             else:
                 t = torch.zeros(n, dtype=torch.float32, device=self.device)
                 bufs.append(self._to_dev(x, t, name))
+        self._touch()
         check(self.lib.gato_set_joint_cost(self._h, *[None if b is None else _ptr(b) for b in bufs], self._stream()),
               "gato_set_joint_cost")
 
@@ -321,9 +344,35 @@ class Solver:
         xs = self._to_dev(xs, self._xs, "xs")
         ref = self._to_dev(ref, self._ref, "ref")
         g = C.c_int32()
+        self._touch()
         check(self.lib.gato_autotune(self._h, _ptr(xu), _ptr(xs), _ptr(ref), float(self.cfg["dt"] if dt is None else dt),
                                      self._stream(), C.byref(g)), "gato_autotune")
         return g.value
+
+    def snapshot(self):
+        """Everything the backward pass reads from the solver besides its arguments: multipliers (device copy),
+        rho and drho, the per-problem wrenches and the diagonal joint weights (device copies)."""
+        lam = torch.empty(self.B, self.N * self.nx, dtype=torch.float32, device=self.device)
+        rho = np.zeros(self.B, np.float32)
+        drho = np.zeros(self.B, np.float32)
+        check(self.lib.gato_get_state(self._h, _ptr(lam), _fptr(rho), _fptr(drho)), "gato_get_state")
+        snap = dict(lam=lam, rho=rho, drho=drho)
+        for name, n in (("f_ext", 6), ("Qd", self.nx), ("Rd", self.nu)):
+            t = torch.empty(self.B * n, dtype=torch.float32, device=self.device)
+            cnt = C.c_int64()
+            check(self.lib.gato_debug_copy(self._h, name.encode(), _ptr(t), t.numel(), C.byref(cnt)),
+                  "gato_debug_copy")
+            snap[name] = t
+        return snap
+
+    def restore(self, snap):
+        """Put back a snapshot() (the joint reference and cost mode are left as they are)."""
+        self._touch()
+        check(self.lib.gato_set_state(self._h, _ptr(snap["lam"]), _fptr(snap["rho"]), _fptr(snap["drho"])),
+              "gato_set_state")
+        check(self.lib.gato_set_f_ext_batch(self._h, _ptr(snap["f_ext"]), self._stream()), "gato_set_f_ext_batch")
+        check(self.lib.gato_set_joint_cost(self._h, _ptr(snap["Qd"]), _ptr(snap["Rd"]), None, self._stream()),
+              "gato_set_joint_cost")
 
     def get_state(self):
         lam = torch.zeros(self.B, self.N * self.nx, dtype=torch.float32, device=self.device)
@@ -339,6 +388,7 @@ class Solver:
             self._to_dev(lam, lt, "lambda")
         r = self._host_f32(rho, self.B, "rho") if rho is not None else None
         d = self._host_f32(drho, self.B, "drho") if drho is not None else None
+        self._touch()
         check(self.lib.gato_set_state(self._h, None if lt is None else _ptr(lt), None if r is None else _fptr(r),
                                       None if d is None else _fptr(d)), "gato_set_state")
 
@@ -348,6 +398,7 @@ class Solver:
         self._to_dev(xs, self._xs, "xs")
         self._to_dev(ref, self._ref, "ref")
         dt = self.cfg["dt"] if dt is None else dt
+        self._touch()
         check(self.lib.gato_debug_phase(self._h, int(phase), _ptr(self._xu), _ptr(self._xs), _ptr(self._ref),
                                         float(dt), self._stream()), "gato_debug_phase")
         return self._xu

@@ -237,7 +237,7 @@ int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_
                      void* stream);
 /* Extension (phase-level tests): copy workspace array `name` ("A", "B", "c", "Qp", "Qv", "Rg", "q", "r",
  * "Qip", "Qiv", "Ri", "Sd", "So", "Pd", "Po", "gamma", "lambda", "dz", "merit_parts", "pcg_iters",
- * "kkt_residual", "f_ext") into the device buffer d_dst of capacity `cap` elements; *count = its length. */
+ * "kkt_residual", "f_ext", "Qd", "Rd") into the device buffer d_dst of capacity `cap` elements; *count = its length. */
 int gato_debug_copy(gato_solver* s, const char* name, void* d_dst, int64_t cap, int64_t* count);
 
 /* Description of the last failure on the calling thread ("" if none). */

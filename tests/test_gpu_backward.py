@@ -106,3 +106,41 @@ def test_autograd_function_routes_through_the_backward(gpu):
     for t, k in ((xs, "xs"), (ref, "ref"), (Qt, "Qd"), (Rt, "Rd")):
         assert t.grad is not None and torch.isfinite(t.grad).all()
         torch.testing.assert_close(t.grad, g[k], rtol=1e-5, atol=1e-6)
+
+
+def test_autograd_backward_uses_its_own_forward_state(gpu):
+    # two forward solves on one solver, then the backward of the first: it must see the first forward's weights,
+    # rho and multipliers (saved when the second forward changed them), give the gradients of a backward run right
+    # after the first forward, and leave the solver in the second forward's state
+    from gato_amd.autograd import solve
+
+    s, wl, Qd, Rd, _ = _problem("iiwa14", 8, 2, seed=25)
+    dev = s.device
+    t = lambda a, g=False: torch.tensor(a, dtype=torch.float32, device=dev, requires_grad=g)
+    xu0 = t(wl.xu)
+
+    def fwd(scale):
+        xs, ref, Qt, Rt = t(wl.xs, True), t(wl.ref, True), t(Qd * scale, True), t(Rd * scale, True)
+        XU = solve(s, xs, ref, Qt, Rt, xu0)
+        return XU, (xs, ref, Qt, Rt)
+
+    s.reset_dual()
+    s.reset_rho()
+    XU1, in1 = fwd(1.0)
+    (XU1 ** 2).sum().backward()
+    expect = [x.grad.clone() for x in in1]
+    s.reset_dual()
+    s.reset_rho()
+    XU1b, in1b = fwd(1.0)
+    torch.testing.assert_close(XU1b, XU1, rtol=0, atol=0)
+    XU2, in2 = fwd(7.0)  # different weights, warm-started multipliers, adapted rho
+    after2 = s.snapshot()
+    (XU1b ** 2).sum().backward()
+    for x, e in zip(in1b, expect):
+        torch.testing.assert_close(x.grad, e, rtol=0, atol=0)
+    now = s.snapshot()
+    for k in ("lam", "f_ext", "Qd", "Rd"):
+        assert torch.equal(now[k], after2[k]), k
+    np.testing.assert_array_equal(now["rho"], after2["rho"])
+    (XU2 ** 2).sum().backward()
+    assert all(torch.isfinite(x.grad).all() for x in in2)

@@ -239,6 +239,36 @@ def test_graph_equals_plain_when_problems_freeze_and_the_batch_stops(gpu, N):
             assert conv == wl.B and rp["n_iters"] == 1
 
 
+def test_graph_equals_plain_to_rounding_at_long_horizons(gpu):
+    # For 64 < N <= 128 the graph path runs its PCG in 1024-thread work-groups (two knots per knot group), the plain
+    # path in 512-thread ones (four), so the PCG's dot products are summed in a different order and the results
+    # agree to rounding, not bit for bit. Joint-space tracking (well conditioned, the PCG converges before its cap)
+    # keeps the step decisions away from ties: the exit decisions, SQP counts and line-search steps must be equal,
+    # PCG counts within one, trajectories and merits within 1e-5. The KKT tolerance sits in the widest gap of the
+    # second iteration's residuals, so rounding cannot move a problem across it.
+    wl = W.joint_sinusoid("iiwa14", N=128, B=16, seed=5)
+    wl.params["pcg_tol"] = 1e-7
+
+    def run(mode, **over):
+        s = make(wl, mode, max_sqp_iters=4, **over)
+        s.set_joint_cost(wl.Qd, wl.Rd, wl.xref)
+        return s.solve(wl.xu, wl.dt, wl.xs, wl.ref)
+
+    probe = run("plain", kkt_tol=0.0, exit_rule="kkt")
+    kr = np.sort(probe["kkt_residual"][1])
+    lo = len(kr) // 4
+    gap = lo + int(np.argmax(np.log(kr[lo + 1:3 * len(kr) // 4 + 1]) - np.log(kr[lo:3 * len(kr) // 4])))
+    tol = float(np.sqrt(kr[gap] * kr[gap + 1]))
+    for over in (dict(), dict(kkt_tol=tol, exit_rule="kkt")):
+        rp, rg = run("plain", **over), run("graph", **over)
+        for key in ("sqp_iters", "kkt_converged", "n_iters", "ls_step_size"):
+            np.testing.assert_array_equal(rg[key], rp[key], err_msg=f"{key} {over}")
+        assert np.abs(rg["all_pcg_iters"] - rp["all_pcg_iters"]).max() <= 1
+        assert xu_rel(rg["XU"], rp["XU"]).max() < 1e-5
+        assert rel_err(rg["final_merit"], rp["final_merit"]) < 1e-5
+    assert len(np.unique(rg["sqp_iters"])) > 1  # problems froze at different iterations
+
+
 def test_state_persists_and_resets(gpu, O):
     wl = W.drop_in_figure8("iiwa14", N=16, B=4, seed=13)
     s = make(wl, "fused", max_sqp_iters=2)
@@ -329,10 +359,11 @@ def test_kkt_exit_rule(gpu, mode):
     np.testing.assert_array_equal(r["XU"], wl.xu.astype(np.float32))
 
 
-def test_full_size_invariants(gpu):
-    # the bench workload at full size: iiwa14, N=32, B=512, <= 10 SQP iterations
+@pytest.mark.parametrize("mode", ["graph", "fused"])
+def test_full_size_invariants(gpu, mode):
+    # the bench workload at full size: iiwa14, N=32, B=512, <= 10 SQP iterations (graph: the mode bench.py times)
     wl = W.drop_in_figure8("iiwa14", N=32, B=512, seed=1)
-    s = make(wl, "fused")
+    s = make(wl, mode)
     r = s.solve(wl.xu, wl.dt, wl.xs, wl.ref)
     assert np.isfinite(r["XU"]).all()
     assert np.all(r["final_merit"] <= r["initial_merit"])  # only strictly decreasing steps are taken
@@ -399,6 +430,29 @@ def test_select_hypothesis_matches_host_selection(gpu, plant):
     assert torch.equal(xu, row9.expand(B, -1))
     best, err = s.select_hypothesis(x, u, np.full(2 * n, np.nan, np.float32), 0.01, xu)
     assert np.isnan(err).all() and best == 0 == int(np.argmin(err))
+
+
+@pytest.mark.parametrize("B", [170, 192, 320, 384, 640, 768, 1000])
+def test_select_hypothesis_at_batches_that_are_not_powers_of_two(gpu, B):
+    # the selection's work-group has 64 * ceil(B / 64) lanes (capped at 1024): its reduction must reach every
+    # slot when that is not a power of two; the winner is placed in each part of the slot range in turn
+    import torch
+
+    from gato_amd.solver import Solver
+
+    s = Solver("iiwa14", 8, B)
+    rng = np.random.default_rng(B)
+    s.set_f_ext_batch(rng.normal(0, 3, (B, 6)).astype(np.float32))
+    x = rng.uniform(-1, 1, 14).astype(np.float32)
+    u = rng.normal(0, 5, 7).astype(np.float32)
+    nxt = s.sim_forward(x, u, 0.01)
+    threads = min(1024, 64 * ((B + 63) // 64))
+    for target in sorted({0, 63, 64, threads // 2 - 1, threads // 2, B // 3, B - 65, B - 2, B - 1}):
+        xu = torch.zeros(B, s.traj, dtype=torch.float32, device=s.device)
+        xu[:, 0] = torch.arange(B, dtype=torch.float32, device=s.device)
+        best, err = s.select_hypothesis(x, u, nxt[target], 0.01, xu)
+        assert best == int(np.argmin(err)) == target, (B, target, best)
+        assert torch.equal(xu[:, 0], torch.full((B,), float(target), device=s.device))
 
 
 @pytest.mark.parametrize("plant", ["iiwa14", "indy7"])
