@@ -40,6 +40,8 @@ SIGNATURES = {
     "gato_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32, C.POINTER(GatoParams)]),
     "gato_destroy": (C.c_int, [_VP]),
     "gato_solve": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, C.POINTER(GatoStats)]),
+    "gato_solve_noalloc": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(GatoParams), _VP, _VP, _VP, C.c_float,
+                                     _VP, C.POINTER(GatoStats)]),
     "gato_set_joint_cost": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
     "gato_set_f_ext_batch": (C.c_int, [_VP, _VP, _VP]),
     "gato_set_rho_penalty_batch": (C.c_int, [_VP, _fp, C.c_int32]),

@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1223,6 +1224,46 @@ __global__ void k_end(Args a) {
     a.drho[b] = a.drho_init[b];  // bsqp.cuh:189
 }
 
+// The per-problem state a freshly constructed solver starts from (bsqp.cuh:43-59): rho = rho0, drho = 1 (also the
+// value it resets to), mu, PCG tolerance, and the joint weights the reference cost implies (Qd = [0, qd_cost],
+// Rd = u_cost). Multipliers and wrenches start at zero with the rest of the cleared workspace.
+__global__ void k_fresh_state(int B, float* rho, float* drho, float* drho_init, float* mu, float* tol, float* Qd,
+                              float* Rd, int nq, float rho0, float mu0, float tol0, float qd_cost, float u_cost) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    rho[b] = rho0;
+    drho[b] = 1.f;
+    drho_init[b] = 1.f;
+    mu[b] = mu0;
+    tol[b] = tol0;
+    for (int i = 0; i < 2 * nq; i++) Qd[(size_t)b * 2 * nq + i] = i < nq ? 0.f : qd_cost;
+    for (int i = 0; i < nq; i++) Rd[(size_t)b * nq + i] = u_cost;
+}
+
+// SQP loop iterations run (the largest per-problem count) and iterations that reached the line search (the last one
+// with a step recorded for any problem): out[0], out[1], as the host statistics report them.
+__global__ void __launch_bounds__(256) k_solve_summary(Args a, int* out) {
+    __shared__ int m[2];
+    if (threadIdx.x == 0) m[0] = m[1] = 0;
+    __syncthreads();
+    int n = 0, l = 0;
+    for (int b = threadIdx.x; b < a.B; b += blockDim.x) {
+        n = max(n, a.st_sqp[b]);
+        for (int i = a.max_sqp - 1; i >= 0; i--)
+            if (a.st_step[(size_t)i * a.B + b] != 0.f) {
+                l = max(l, i + 1);
+                break;
+            }
+    }
+    atomicMax(&m[0], n);
+    atomicMax(&m[1], l);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = m[0];
+        out[1] = m[1];
+    }
+}
+
 // ------------------------------------------------------------------ fused persistent kernel
 // One work-group per problem; problems are independent (a converged problem stops iterating;
 // the reference instead keeps stepping converged problems until the whole batch has converged).
@@ -1869,6 +1910,11 @@ __global__ void k_sim(float* xn, const float* x, const float* u, const float* fe
 
 }  // namespace gato
 
+// Workspace of the allocation-free solve (gato_solve_noalloc): one static array per GPU, resident from the library's
+// load until the process ends. Host code never passes its name to a kernel or a copy (in host code the name is not a
+// device address): it uses the address hipGetSymbolAddress returns for the current device (scratch_address).
+static __device__ __align__(256) unsigned char scratch[GATO_NOALLOC_SCRATCH_BYTES];
+
 // =================================================================== host side
 namespace {
 
@@ -1886,6 +1932,25 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 int plant_nq(int plant) { return plant == GATO_PLANT_IIWA14 ? 7 : plant == GATO_PLANT_INDY7 ? 6 : -1; }
+
+// Device address of `scratch` on the current device, looked up once per device (each GPU holds its own copy).
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[MAX_DEVICES] = {};
+
+int scratch_address(void** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(GATO_E_ARG, "device index beyond the allocation-free solve's table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
 
 }  // namespace
 
@@ -1941,7 +2006,9 @@ T* carve(gato_solver* s, size_t& off, size_t count) {
     return p;
 }
 
-size_t layout(gato_solver* s, bool assign) {
+// The solver's workspace, carved from s->pool in 256-byte-aligned arrays. The forward solve's arrays come first;
+// with forward_only the layout stops after them (the allocation-free solve carves only those).
+size_t layout(gato_solver* s, bool assign, bool forward_only = false) {
     const size_t N = s->N, B = s->B, nx = s->nx, nu = s->nu, nq = s->nq;
     size_t off = 0;
     auto F = [&](size_t n) -> float* { float* p = carve<float>(s, off, n); return assign ? p : nullptr; };
@@ -1987,7 +2054,11 @@ size_t layout(gato_solver* s, bool assign) {
     t = F(it * B); if (assign) a.st_lsm = t;
     t = F(it * B); if (assign) a.st_step = t;
     ip = I(B); if (assign) a.st_sqp = ip;
-    ip = I(2); if (assign) s->d_stop = ip;  // [0] graph-mode stop flag, [1] selected hypothesis
+    // [0] graph-mode stop flag, [1] selected hypothesis, [2] SQP loop iterations run, [3] line-search iterations
+    ip = I(4); if (assign) s->d_stop = ip;
+    t = F(B); if (assign) a.st_init = t;
+    t = F(B); if (assign) a.st_final = t;
+    if (forward_only) return off + 256;
     t = F(B * N * nx); if (assign) s->lam_save = t;
     t = F(B * s->traj); if (assign) s->xu_tune = t;
     t = F(B * N * nx); if (assign) s->lam_b = t;
@@ -2012,8 +2083,6 @@ size_t layout(gato_solver* s, bool assign) {
     t = F(B * (s->traj + nu)); if (assign) s->dz_w = t;
     t = F(B * N * nx * nx); if (assign) s->bwd_G = t;  // the backward's factorization of S
     t = F(B * N * nx * nx); if (assign) s->bwd_Di = t;
-    t = F(B); if (assign) a.st_init = t;
-    t = F(B); if (assign) a.st_final = t;
     return off + 256;
 }
 
@@ -3109,6 +3178,73 @@ int gato_mixed_solve(gato_mixed* m, float* const* d_xu, const float* const* d_xs
 int gato_mixed_last_time(gato_mixed* m, double* us) {
     if (!m || !us) return fail(GATO_E_ARG, "null argument");
     *us = m->last_us;
+    return 0;
+}
+
+int gato_solve_noalloc(int32_t plant, int32_t knots, int32_t batch, const gato_params* params, float* d_xu,
+                       const float* d_xs, const float* d_ref, float dt, void* stream, gato_stats* stats) {
+    if (!params || !d_xu || !d_xs || !d_ref) return fail(GATO_E_ARG, "null argument");
+    const int nq = plant_nq(plant);
+    if (nq < 0) return fail(GATO_E_ARG, "unknown plant");
+    if (knots < 2 || knots > 128) return fail(GATO_E_ARG, "knots must be in [2, 128]");
+    if (batch < 1) return fail(GATO_E_ARG, "batch must be >= 1");
+    if (params->max_sqp_iters < 1 || params->max_sqp_iters > 1000) return fail(GATO_E_ARG, "max_sqp_iters must be in [1, 1000]");
+    if (params->exit_rule != GATO_EXIT_PCG0 && params->exit_rule != GATO_EXIT_KKT) return fail(GATO_E_ARG, "unknown exit rule");
+    gato_solver s;  // a fresh solver whose workspace is the static buffer: no pool of its own, nothing to free
+    s.plant = plant;
+    s.N = knots;
+    s.B = batch;
+    s.nq = nq;
+    s.nx = 2 * nq;
+    s.nu = nq;
+    s.traj = (s.nx + s.nu) * knots - s.nu;
+    s.prm = *params;
+    s.prm.mode = GATO_MODE_GRAPH;
+    s.G = default_groups(knots);
+    const size_t need = layout(&s, false, true);
+    if (need > (size_t)GATO_NOALLOC_SCRATCH_BYTES)
+        return fail(GATO_E_ARG, "gato_solve_noalloc: " + std::to_string(batch) + " problems of " + std::to_string(knots) +
+                                    " knots need " + std::to_string(need) + " bytes of workspace, more than the static " +
+                                    std::to_string((size_t)GATO_NOALLOC_SCRATCH_BYTES) + "-byte buffer holds");
+    void* base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc) return rc;
+    s.pool = static_cast<char*>(base);
+    s.pool_bytes = need;
+    layout(&s, true, true);
+    hipStream_t st = (hipStream_t)stream;
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    // the previous call's contents: cleared, then the fresh solver's state (gato_create clears its pool the same way)
+    hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, reinterpret_cast<float*>(s.pool), need / sizeof(float));
+    hipLaunchKernelGGL(gato::k_fresh_state, dim3((batch + 255) / 256), dim3(256), 0, st, batch, s.d_rho, s.d_drho,
+                       s.d_drho_init, s.d_mu, s.d_tol, s.d_Qd, s.d_Rd, nq, params->rho, params->mu, params->pcg_tol,
+                       params->qd_cost, params->u_cost);
+    fill_args(&s, d_xu, d_xs, d_ref, dt);
+    s.a.stop = s.d_stop;
+    s.a.stop_w = s.d_stop;
+    if (plant == GATO_PLANT_IIWA14) enqueue_graph_solve<0>(&s, st);
+    else enqueue_graph_solve<1>(&s, st);
+    HIP_TRY(hipGetLastError());
+    if (!stats) return 0;
+    hipLaunchKernelGGL(gato::k_solve_summary, dim3(1), dim3(256), 0, st, s.a, s.d_stop + 2);
+    HIP_TRY(hipGetLastError());
+    const size_t B = batch, it = params->max_sqp_iters;
+    int32_t counts[2] = {0, 0};
+    auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
+        if (dst) HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        return 0;
+    };
+    if ((rc = cp(counts, s.d_stop + 2, sizeof(counts))) || (rc = cp(stats->sqp_iters, s.a.st_sqp, B * 4)) ||
+        (rc = cp(stats->kkt_converged, s.a.conv, B * 4)) || (rc = cp(stats->initial_merit, s.a.st_init, B * 4)) ||
+        (rc = cp(stats->final_merit, s.a.st_final, B * 4)) || (rc = cp(stats->pcg_iters, s.a.st_pcg, it * B * 4)) ||
+        (rc = cp(stats->ls_min_merit, s.a.st_lsm, it * B * 4)) || (rc = cp(stats->ls_step_size, s.a.st_step, it * B * 4)) ||
+        (rc = cp(stats->kkt_residual, s.a.st_kres, it * B * 4)))
+        return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    stats->solve_time_us = std::chrono::duration<double, std::micro>(t1 - t0).count();
+    stats->n_iters = counts[0];
+    stats->ls_num_iters = counts[1];
     return 0;
 }
 

@@ -88,6 +88,21 @@ int gato_destroy(gato_solver* s);
 int gato_solve(gato_solver* s, float* d_xu, const float* d_xs, const float* d_ref, float dt, void* stream,
                gato_stats* stats);
 
+/* Allocation-free solve. No counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It runs gato_solve's
+ * GATO_MODE_GRAPH kernels (params->mode is ignored), enqueued on `stream` directly, for a solver fresh from
+ * gato_create: lambda = 0, rho = params->rho, drho = 1, mu = params->mu, PCG tolerance = params->pcg_tol, no
+ * wrenches, the reference cost. Its workspace is carved from a static device buffer of GATO_NOALLOC_SCRATCH_BYTES
+ * that the library holds on every GPU it runs on until the process ends; the call makes no hipMalloc, hipFree or
+ * other allocating call. A problem set whose workspace exceeds the buffer is refused with GATO_E_ARG before anything
+ * runs (iiwa14 at N = 32 and 10 SQP iterations: about 190 problems). Every call on one GPU uses that GPU's one
+ * buffer and clears it first: nothing carries over from the previous call, and calls that overlap on one GPU (other
+ * streams, other threads) race. stats: NULL for an asynchronous call; otherwise filled as by gato_solve, and the
+ * call returns after the solve completed. Same arguments and layouts as gato_create + gato_solve. */
+#define GATO_NOALLOC_SCRATCH_BYTES (32u << 20)
+int gato_solve_noalloc(int32_t plant, int32_t knots, int32_t batch, const gato_params* params, float* d_xu,
+                       const float* d_xs, const float* d_ref, float dt, void* stream, gato_stats* stats);
+
 /* Extension: joint-space tracking terms 1/2 sum_i Qd_i (x_i - xref_{k,i})^2 + 1/2 sum_i Rd_i u_i^2
  * replacing qd_cost / u_cost (NULL pointers restore the reference cost). Device pointers
  * d_Qd [B * nx], d_Rd [B * nu], d_xref [B * N * nx]; copied, so they may be freed afterwards. */

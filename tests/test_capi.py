@@ -74,6 +74,15 @@ def test_argument_validation_without_device():
     assert lib.gato_force_est_generate(None, None, None, None, None) == 1001
     assert lib.gato_force_est_update(None, None, None, None, 0.5, 0.8, None, None) == 1001
     assert lib.gato_force_est_destroy(None) == 1001
+    # the allocation-free solve refuses bad arguments and a workspace beyond its static buffer before any device call
+    d = C.c_void_p(256)  # never dereferenced: every case below is refused first
+    p.max_sqp_iters = 10
+    assert lib.gato_solve_noalloc(0, 32, 4, C.byref(p), None, d, d, 0.01, None, None) == 1001
+    assert lib.gato_solve_noalloc(7, 32, 4, C.byref(p), d, d, d, 0.01, None, None) == 1001
+    assert lib.gato_solve_noalloc(0, 129, 4, C.byref(p), d, d, d, 0.01, None, None) == 1001
+    assert lib.gato_solve_noalloc(0, 32, 4096, C.byref(p), d, d, d, 0.01, None, None) == 1001
+    msg = lib.gato_last_error().decode()
+    assert "static" in msg and str(32 << 20) in msg
 
 
 def test_drop_in_module_surface():
