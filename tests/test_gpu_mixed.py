@@ -95,3 +95,75 @@ def test_autotune_keeps_state_and_inputs(gpu):
     np.testing.assert_array_equal(lam0, lam1)
     np.testing.assert_array_equal(rho0, rho1)
     np.testing.assert_array_equal(drho0, drho1)
+
+
+def _oracle_first_iteration(O, wl, idx):
+    p = wl.params
+    cfg = O.make_cfg(wl.plant, N=wl.N, dt=p["dt"], max_sqp_iters=1, kkt_tol=p["kkt_tol"],
+                     max_pcg_iters=p["max_pcg_iters"], solve_ratio=p["solve_ratio"], q_cost=p["q_cost"],
+                     qd_cost=p["qd_cost"], u_cost=p["u_cost"], N_cost=p["N_cost"], q_lim_cost=p["q_lim_cost"],
+                     vel_lim_cost=p["vel_lim_cost"], ctrl_lim_cost=p["ctrl_lim_cost"])
+    f = lambda a: a[idx].astype(np.float32).astype(np.float64)
+    o = O.solve(cfg, f(wl.xu), f(wl.xs), f(wl.ref), mu=p["mu"], pcg_tol=p["pcg_tol"], rho0=p["rho"])
+    return cfg, o
+
+
+def test_uneven_batch_workload_full_size_and_first_iteration_against_oracle(gpu, O):
+    # The uneven-batch configuration as bench.py runs it: 1024 figure-8 problems, half iiwa14 and half indy7, N drawn
+    # from {8, 32, 128}, radius ~U(0.1, 0.25) m and phase ~U(0, 2 pi), one graph launch with launch shapes tuned on
+    # the first solve, <= 10 SQP iterations. At full size: finite trajectories, merit non-increasing over the
+    # recorded iterations, steps in the step-length set, batch-uniform SQP counts within each class (the reference
+    # rule), PCG counts within the cap. Then a one-iteration mixed solve of the same inputs, and for three problems of
+    # every (plant, N) class the oracle's first iteration: initial merit and PCG count, and the line search's
+    # consistency as in the single-class figure-8 test (the accepted merit is the oracle's merit of the GPU's new
+    # iterate; a rejected step leaves the trajectory unchanged bit for bit). Merit tolerance 1e-5 up to N = 32; at
+    # N = 128 the fp32 sum over 127 defects of 12-14 components (mu |.|_1 of differences of nearly equal states)
+    # carries ~4e-5 of the merit, so 1e-4 there.
+    from gato_amd.mixed import MixedSolver
+
+    wls = W.mixed_figure8(B=1024, seed=2)
+    keys = sorted(wls)
+    assert sum(wls[k].B for k in keys) == 1024 and len(keys) == 6
+    kw = {k: wls[keys[0]].params[k] for k in SCALARS}
+    allowed = {-1.0} | {1.0 / (1 << j) for j in range(8)}
+    m = MixedSolver([(k[0], k[1], wls[k].B) for k in keys], **kw)
+    dev = m.device
+    ins = [_dev(wls[k], dev) for k in keys]
+    st = m.solve_device([x[0] for x in ins], [x[1] for x in ins], [x[2] for x in ins], stats=True)
+    for i, k in enumerate(keys):
+        r = st[i]
+        n = r["n_iters"]
+        assert torch.isfinite(ins[i][0]).all(), k
+        assert np.all(r["sqp_iters"] == r["sqp_iters"][0]) and 1 <= n <= kw["max_sqp_iters"], k
+        steps = r["ls_step_size"][:n]
+        assert set(np.unique(steps)).issubset(allowed | {0.0}), k
+        lsm = np.concatenate([r["initial_merit"][None], r["ls_min_merit"][:n]])
+        assert np.all(np.diff(lsm, axis=0) <= 0), k
+        assert np.all(r["final_merit"] <= r["initial_merit"]), k
+        assert np.all((r["pcg_iters"][:n] >= 0) & (r["pcg_iters"][:n] <= kw["max_pcg_iters"])), k
+    kw1 = dict(kw, max_sqp_iters=1)
+    m1 = MixedSolver([(k[0], k[1], wls[k].B) for k in keys], **kw1)
+    ins1 = [_dev(wls[k], dev) for k in keys]
+    st1 = m1.solve_device([x[0] for x in ins1], [x[1] for x in ins1], [x[2] for x in ins1], stats=True)
+    for i, k in enumerate(keys):
+        wl = wls[k]
+        tol = 1e-5 if wl.N <= 32 else 1e-4
+        idx = np.array([0, wl.B // 2, wl.B - 1])
+        cfg, o = _oracle_first_iteration(O, wl, idx)
+        r = st1[i]
+        XU = ins1[i][0].cpu().numpy()[idx]
+        xu0 = wl.xu[idx].astype(np.float32)
+        g_init = r["initial_merit"][idx].astype(np.float64)
+        assert np.abs(g_init - o["initial_merit"]).max() <= tol * np.abs(o["initial_merit"]).max(), k
+        assert np.all(np.abs(r["pcg_iters"][0][idx] - o["pcg_iters"][0]) <= 2), k
+        f = lambda a: a.astype(np.float32).astype(np.float64)
+        for j, b in enumerate(idx):
+            step = r["ls_step_size"][0][b]
+            assert step in allowed, (k, b)
+            m_new = O.merit(cfg, XU[j].astype(np.float64), np.zeros(wl.xu.shape[1]), 0.0, f(wl.xs[b]), f(wl.ref[b]),
+                            wl.params["mu"])
+            assert abs(r["ls_min_merit"][0][b] - m_new) <= tol * abs(m_new), (k, b)
+            if step < 0:
+                np.testing.assert_array_equal(XU[j], xu0[j])
+            else:
+                assert m_new < r["initial_merit"][b], (k, b)
