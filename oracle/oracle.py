@@ -330,3 +330,71 @@ def step_grad(cfg, xu, xs, ref, rho, dL_dz, fext=None, Qd=None, Rd=None, xref=No
                 val += w * ((vq @ a) * (gvec @ dq) + (vq @ gvec) * (a @ dq))
             gref[k, j] = val
     return dict(xs=gxs, ref=gref, Qd=gQ, Rd=gR, dz=dz_, lam=lam, v=vz, v_lam=vl)
+
+
+# ----------------------------------------------------------------------------- derivative of the solution
+# At a solution the step vanishes: F(z, lambda; theta) = [g(z) + C(z)^T lambda; c(z)] = 0, with g the cost gradient
+# of the KKT set-up (including its terminal-block convention) and C the constraint Jacobian. The solution's
+# derivative is then the implicit one, d[z; lambda]/dtheta = -J^-1 dF/dtheta with J = dF/d(z, lambda), the exact
+# Jacobian: its upper-left block is the Hessian of the Lagrangian (cost curvature and the multipliers times the
+# dynamics' second derivatives), not the SQP model G~ the step uses. (No counterpart in the reference.)
+
+
+def kkt_function(cfg, xu, lam, xs, ref, fext=None, Qd=None, Rd=None, xref=None):
+    """F(z, lambda) = [g + C^T lambda; c] of one problem (dense FP64), and C."""
+    nq_, nx, nu = dims(cfg)
+    Qw, Rw = _joint_weights(cfg, Qd, Rd)
+    kkt = setup_kkt(cfg, _f64(xu), xs, ref, fext, Qw, Rw, xref)
+    _, g, Cm, c = dense_kkt(kkt, 0.0, nq_)
+    return np.concatenate([g + Cm.T @ _f64(lam), c]), Cm
+
+
+def fixed_point(cfg, xu, xs, ref, rho=1e-3, fext=None, Qd=None, Rd=None, xref=None, iters=60, tol=1e-12):
+    """Full SQP steps z <- z + dz from xu (no line search) until |dz|_inf <= tol; returns (z, lambda, |dz|_inf)."""
+    z = _f64(xu).copy()
+    for _ in range(iters):
+        dz_, lam, _ = step_map(cfg, z, xs, ref, rho, fext, Qd, Rd, xref)
+        z += dz_
+        if np.abs(dz_).max() <= tol:
+            break
+    dz_, lam, _ = step_map(cfg, z, xs, ref, rho, fext, Qd, Rd, xref)
+    return z, lam, float(np.abs(dz_).max())
+
+
+def solution_grad(cfg, xu, lam, xs, ref, dL_dz, fext=None, Qd=None, Rd=None, xref=None, h=1e-6):
+    """Gradient of L(z*(theta)) w.r.t. x_s [nx], reference [N, 6], diag Q [nx] and diag R [nu] at a solution (xu,
+    lam) of one problem, given dL/dz [traj]: w = J^-1 [dL/dz; 0] with J built column by column from central
+    differences (step h) of the analytic F along z and exactly in lambda; then dL/dtheta = -w^T dF/dtheta, the
+    same parameter terms as step_grad with the step dz = 0."""
+    nq_, nx, nu = dims(cfg)
+    N = cfg.N
+    Qw, Rw = _joint_weights(cfg, Qd, Rd)
+    xu = _f64(xu)
+    nz = xu.shape[0]
+    F0, Cm = kkt_function(cfg, xu, lam, xs, ref, fext, Qw, Rw, xref)
+    nc = Cm.shape[0]
+    J = np.zeros((nz + nc, nz + nc))
+    for j in range(nz):
+        e = np.zeros(nz)
+        e[j] = h
+        Fp, _ = kkt_function(cfg, xu + e, lam, xs, ref, fext, Qw, Rw, xref)
+        Fm, _ = kkt_function(cfg, xu - e, lam, xs, ref, fext, Qw, Rw, xref)
+        J[:nz, j] = (Fp[:nz] - Fm[:nz]) / (2 * h)
+    J[:nz, nz:] = Cm.T
+    J[nz:, :nz] = Cm
+    w = np.linalg.solve(J, np.concatenate([_f64(dL_dz), np.zeros(nc)]))
+    vz, vl = w[:nz], w[nz:]
+    xr = np.zeros((N, nx)) if xref is None else _f64(xref).reshape(N, nx)
+    st = nx + nu
+    gQ, gR, gref = np.zeros(nx), np.zeros(nu), np.zeros((N, 6))
+    for k in range(N):
+        term = k == N - 1
+        src = (N - 1 if cfg.terminal_mode else N - 2) if term else k
+        xsrc = xu[src * st:src * st + nx]
+        gQ += -vz[k * st:k * st + nx] * (xsrc - xr[k])
+        if not term:
+            gR += -vz[k * st + nx:k * st + nx + nu] * xu[k * st + nx:k * st + nx + nu]
+        w_k = (cfg.N_cost if cfg.terminal_mode else cfg.q_cost) if term else cfg.q_cost
+        _, Jp = fk(_plant_name(cfg.plant), xsrc[:nq_])
+        gref[k, :3] = w_k * (Jp @ vz[k * st:k * st + nq_])
+    return dict(xs=-vl[:nx], ref=gref, Qd=gQ, Rd=gR, v=vz, v_lam=vl, F_inf=float(np.abs(F0).max()))

@@ -94,3 +94,85 @@ def test_gradient_at_a_fixed_point_is_the_implicit_derivative(O):
         rhs[nz + i] = -1.0
         expect[i] = wvec @ np.linalg.solve(K, rhs)[:nz]
     np.testing.assert_allclose(gr["xs"], expect, rtol=1e-9, atol=1e-12)
+
+
+def _config4_loss(O, plant, N, nx, nu, xu, ref):
+    """||u_0||^2 + sum_k ||p_k - ref_k||^2 (the differentiation configuration's loss) and its gradient in z."""
+    nq = nx // 2
+    st = nx + nu
+    u0 = xu[nx:nx + nu]
+    L = u0 @ u0
+    g = np.zeros_like(xu)
+    g[nx:nx + nu] = 2 * u0
+    for k in range(N):
+        p, J = O.fk(plant, xu[k * st:k * st + nq])
+        d = p - ref[k, :3]
+        L += d @ d
+        g[k * st:k * st + nq] += 2 * J.T @ d
+    return L, g
+
+
+@pytest.mark.parametrize("plant,hess,seed", [("iiwa14", 0, 3), ("iiwa14", 1, 3), ("indy7", 0, 5)])
+def test_solution_gradient_matches_central_differences_of_resolves(O, plant, hess, seed):
+    # The derivative of the solution itself: at a fixed point of the step (|dz| <= 1e-12, full steps from the
+    # figure-8 start, the consistent terminal block), solution_grad must equal central differences with eps = 1e-3
+    # (FP64) of the loss of the RE-SOLVED problem, perturbing x_s, the reference xyz and the log-weights (weights
+    # log-uniform on [1e-3, 1e2]). The explicit dependence of the loss on the reference is added to both sides'
+    # reference gradient. The one-step gradient (step_grad) at the same point misses the solution's curvature and is
+    # off by more than 5%, so this check tells the two apart.
+    N = 8
+    wl = W.drop_in_figure8(plant, N=N, B=1, seed=seed)
+    rng = np.random.default_rng(seed)
+    nq = W.NQ[plant]
+    nx, nu = 2 * nq, nq
+    Qd = np.exp(rng.uniform(np.log(1e-3), np.log(1e2), nx))
+    Rd = np.exp(rng.uniform(np.log(1e-3), np.log(1e2), nu))
+    cfg = O.make_cfg(plant, N=N, dt=0.01, q_cost=10.0, N_cost=10.0, qd_cost=1e-3, u_cost=1e-3, q_lim_cost=0.0,
+                     terminal_mode=1, hess_mode=hess)
+    xs0, ref0 = wl.xs[0], wl.ref[0].reshape(N, 6).copy()
+    z, lam, res = O.fixed_point(cfg, wl.xu[0], xs0, ref0.reshape(-1), Qd=Qd, Rd=Rd)
+    assert res <= 1e-12
+    L0, dL = _config4_loss(O, plant, N, nx, nu, z, ref0)
+    sg = O.solution_grad(cfg, z, lam, xs0, ref0.reshape(-1), dL, Qd=Qd, Rd=Rd)
+    assert sg["F_inf"] < 1e-9
+    explicit_ref = np.zeros((N, 3))
+    for k in range(N):
+        p, _ = O.fk(plant, z[k * (nx + nu):k * (nx + nu) + nq])
+        explicit_ref[k] = -2 * (p - ref0[k, :3])
+    eps = 1e-3
+
+    def resolved(xs=xs0, ref=ref0, q=Qd, r=Rd):
+        zz, _, rr = O.fixed_point(cfg, z, xs, ref.reshape(-1), Qd=q, Rd=r)
+        assert rr <= 1e-11
+        return _config4_loss(O, plant, N, nx, nu, zz, ref)[0]
+
+    def fd(perturb, n):
+        return np.array([(resolved(**perturb(i, eps)) - resolved(**perturb(i, -eps))) / (2 * eps) for i in range(n)])
+
+    def p_xs(i, d):
+        x = xs0.copy(); x[i] += d
+        return dict(xs=x)
+
+    def p_ref(i, d):
+        r = ref0.copy(); r[i // 3, i % 3] += d
+        return dict(ref=r)
+
+    def p_Q(i, d):
+        q = Qd.copy(); q[i] *= np.exp(d)
+        return dict(q=q)
+
+    def p_R(i, d):
+        r = Rd.copy(); r[i] *= np.exp(d)
+        return dict(r=r)
+
+    one = O.step_grad(cfg, z, xs0, ref0.reshape(-1), 1e-3, dL, Qd=Qd, Rd=Rd)
+    pairs = [("xs", sg["xs"], one["xs"], fd(p_xs, nx)),
+             ("ref", (sg["ref"][:, :3] + explicit_ref).reshape(-1), (one["ref"][:, :3] + explicit_ref).reshape(-1),
+              fd(p_ref, 3 * N)),
+             ("logQ", sg["Qd"] * Qd, one["Qd"] * Qd, fd(p_Q, nx)), ("logR", sg["Rd"] * Rd, one["Rd"] * Rd, fd(p_R, nu))]
+    worst_one = 0.0
+    for name, an, an1, num in pairs:
+        scale = np.abs(num).max()
+        assert np.abs(an - num).max() <= 1e-5 * scale, name
+        worst_one = max(worst_one, np.abs(an1 - num).max() / scale)
+    assert worst_one > 0.05
