@@ -1536,6 +1536,136 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_grad(Args a, const f
     }
 }
 
+// ------------------------------------------------------------------ derivative of the solution (implicit rounds)
+// At a solution the step vanishes: F(z, l) = [g(z) + C(z)^T l; c(z)] = 0, and the solution's derivative needs the
+// adjoint of the exact Jacobian J = dF/d(z, l), whose upper-left block H is the Hessian of the Lagrangian; the step's
+// K has the model G~ there instead. With J = K - D, D = [[G~ - H, 0], [0, 0]], the iteration K w+ = [dL/dz; 0] + D w
+// contracts at the SQP step's own local rate (w = K^-1 [dL/dz; 0] is the one-step adjoint), and each of its steps is
+// a refinement round of the adjoint chain with the right-hand side dL/dz + (G~ - H) v. H v comes from central
+// differences of the stationarity rows along v: the KKT set-up at z +- eps v (k_lin on separate blocks), with the
+// step's multipliers l held fixed.
+
+// KKT set-up blocks in fp64 (the implicit rounds' perturbed linearizations)
+struct Lin64 {
+    double *A, *Bm, *c, *Qp, *Qv, *Rg, *q, *r;
+};
+
+// zp = z + eps v, zm = z - eps v for problem b (fp64), with eps = step / |v|_inf (stored per problem)
+__global__ void __launch_bounds__(256) k_perturb(const float* z, const float* vall, double* zp, double* zm,
+                                                 double* eps, int traj, int ndz, double step) {
+    __shared__ float red[8];
+    const int b = blockIdx.x;
+    const float* v = vall + (size_t)b * ndz;
+    float m = 0.f;
+    for (int i = threadIdx.x; i < traj; i += blockDim.x) m = fmaxf(m, fabsf(v[i]));
+    m = block_max(m, red);
+    const double e = step / fmax((double)m, 1e-300);
+    if (threadIdx.x == 0) eps[b] = e;
+    for (int i = threadIdx.x; i < traj; i += blockDim.x) {
+        const size_t j = (size_t)b * traj + i;
+        zp[j] = (double)z[j] + e * (double)v[i];
+        zm[j] = (double)z[j] - e * (double)v[i];
+    }
+}
+
+// The KKT set-up of phase_lin in fp64 at the trajectory zall (fp64), the solver's inputs (fp32: x_s, reference,
+// wrenches, joint weights and reference) widened in LDS. Knot groups loop over the knots; per group the fp64 lin_knot
+// table and the knot's reference.
+template <int PL>
+__host__ __device__ constexpr int lin64_group_doubles() {
+    return lin_lds_floats<PL>() + 8 + Plant<PL>::NX;
+}
+template <int PL>
+__global__ void __launch_bounds__(128) k_lin64(Args a, const double* zall, Lin64 o) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU, ST = NX + NU;
+    extern __shared__ __align__(16) double smd[];
+    const int b = blockIdx.x, N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
+    View<PL> v(a, b);
+    double* sxs = smd;
+    double* sQd = sxs + NX;
+    double* sRd = sQd + NX;
+    double* sfe = sRd + NU;
+    double* grp = sfe + 8;
+    for (int i = threadIdx.x; i < NX; i += blockDim.x) {
+        sxs[i] = v.xs[i];
+        sQd[i] = v.Qd[i];
+    }
+    for (int i = threadIdx.x; i < NU; i += blockDim.x) sRd[i] = v.Rd[i];
+    for (int i = threadIdx.x; i < 6; i += blockDim.x) sfe[i] = v.fext[i];
+    __syncthreads();
+    const double* z = zall + (size_t)b * a.traj;
+    double* shg = grp + g * lin64_group_doubles<PL>();
+    double* sref = shg + lin_lds_floats<PL>();
+    double* sxr = sref + 8;
+    const double* xt = z + (size_t)(a.cw.terminal_mode ? N - 1 : N - 2) * ST;
+    for (int k = g; k < N; k += G) {
+        if (lane < 6) sref[lane] = v.ref[6 * k + lane];
+        if (lane < NX && v.xref) sxr[lane] = v.xref[k * NX + lane];
+        wave_sync();
+        const size_t bk = (size_t)b * N + k;
+        const double* xk = z + (size_t)(k < N - 1 ? k : 0) * ST;
+        lin_knot<PL, double>(k, N, opaque(lane), (double)a.dt, a.cw, xk, xt, sref, v.xref ? sxr : nullptr, sQd, sRd,
+                             sfe, z, sxs, o.A + bk * NX * NX, o.Bm + bk * NX * NU,
+                             o.c + (size_t)b * N * NX + (k + 1) * NX, o.c + (size_t)b * N * NX, o.Qp + bk * NQ * NQ,
+                             o.Qv + bk * NQ, o.Rg + bk * NU, o.q + bk * NX, o.r + bk * NU, shg);
+    }
+}
+
+// rz = dL/dz + G~ v - H v for problem b (trajectory layout), H v = [F_z(z + eps v) - F_z(z - eps v)] / (2 eps) in fp64
+// from the two perturbed set-ups (p, m: q, r, A, B) and the step's multipliers lam; G~ from the blocks at z (a)
+template <int PL>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_hvp_rhs(Args a, Lin64 p, Lin64 m, const float* dL,
+                                                                    const float* vall, const float* lamall,
+                                                                    const double* eps, float* rz) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU, ST = NX + NU;
+    const int b = blockIdx.x;
+    View<PL> v0(a, b);
+    const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
+    const float rho = a.rho[b];
+    const double inv2e = 0.5 / eps[b];
+    const float* v = vall + (size_t)b * (a.traj + NU);
+    const float* l = lamall + (size_t)b * N * NX;
+    const float* dl = dL + (size_t)b * a.traj;
+    float* out = rz + (size_t)b * a.traj;
+    for (int k = g; k < N; k += G) {
+        const bool last = (k == N - 1);
+        const size_t bk = (size_t)b * N + k;
+        const float* vk = v + (size_t)k * ST;
+        const float* l1 = l + (k + 1) * NX;
+        if (lane < NX) {
+            const int i = lane;
+            float gv;
+            if (i < NQ) {
+                gv = rho * vk[i];
+#pragma unroll
+                for (int j = 0; j < NQ; j++) gv += v0.Qp[k * NQ * NQ + i * NQ + j] * vk[j];
+            } else {
+                gv = v0.Qv[k * NQ + i - NQ] * vk[i];
+            }
+            double dF = p.q[bk * NX + i] - m.q[bk * NX + i];
+            if (!last) {
+                const double* Ap = p.A + bk * NX * NX;
+                const double* Am = m.A + bk * NX * NX;
+#pragma unroll
+                for (int j = 0; j < NX; j++) dF += (Ap[j * NX + i] - Am[j * NX + i]) * (double)l1[j];
+            }
+            out[(size_t)k * ST + i] = (float)((double)dl[(size_t)k * ST + i] + (double)gv - dF * inv2e);
+        }
+        if (!last && lane < NU) {
+            const int i = lane;
+            const float gv = v0.Rg[k * NU + i] * vk[NX + i];
+            double dF = p.r[bk * NU + i] - m.r[bk * NU + i];
+            const double* Bp = p.Bm + bk * NX * NU;
+            const double* Bn = m.Bm + bk * NX * NU;
+#pragma unroll
+            for (int j = 0; j < NX; j++) dF += (Bp[j * NU + i] - Bn[j * NU + i]) * (double)l1[j];
+            out[(size_t)k * ST + NX + i] = (float)((double)dl[(size_t)k * ST + NX + i] + (double)gv - dF * inv2e);
+        }
+    }
+}
+
 // backward set-up: flags cleared, the step's multipliers warm-started from the solve's, the correction and
 // adjoint buffers (lam_a, dz_a, dz_b; dz_v, lam_v, lam_w) zeroed, one work-group per problem
 __global__ void k_bwd_begin(Args a, const float* lam_keep, float* lam_b, float* lam_a, float* dz_b, float* dz_a,
@@ -1984,6 +2114,15 @@ struct gato_solver {
     int bwd_max_pcg = -1;  // -1: one iteration per round with the factorization preconditioner, 1000 with P
     int bwd_refine = 2;
     float bwd_tol = 1e-6f;
+    // derivative of the solution (gato_set_backward_implicit): rounds, finite-difference step, and their workspace
+    // (allocated on first use): per-problem steps, z +- eps v, the augmented right-hand side, and the KKT set-up
+    // blocks at z + eps v (imp_p) and z - eps v (imp_m)
+    int bwd_implicit = 0;
+    float bwd_fd_step = 1e-4f;
+    char* imp_pool = nullptr;
+    double *imp_eps = nullptr, *imp_zp = nullptr, *imp_zm = nullptr;
+    float* imp_rz = nullptr;
+    gato::Lin64 imp_p{}, imp_m{};
     // graph mode: the captured solve, valid for the launch arguments it was captured with
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -2119,7 +2258,8 @@ void launch_merit(int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args&
 template <int PL>
 void launch_pcg_btd(int np, int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a, const float* Gm,
                     const float* Di) {
-    if (np == 2) hipLaunchKernelGGL((gato::k_pcg_btd<PL, 2>), dim3(B), blk, lds, st, a, Gm, Di);
+    if (np == 1) hipLaunchKernelGGL((gato::k_pcg_btd<PL, 1>), dim3(B), blk, lds, st, a, Gm, Di);
+    else if (np == 2) hipLaunchKernelGGL((gato::k_pcg_btd<PL, 2>), dim3(B), blk, lds, st, a, Gm, Di);
     else if (np == 4) hipLaunchKernelGGL((gato::k_pcg_btd<PL, 4>), dim3(B), blk, lds, st, a, Gm, Di);
     else hipLaunchKernelGGL((gato::k_pcg_btd<PL, 8>), dim3(B), blk, lds, st, a, Gm, Di);
 }
@@ -2417,8 +2557,68 @@ int launch_backward(gato_solver* s, const float* dL, float* gxs, float* gref, fl
         hipLaunchKernelGGL(gato::k_accumulate, dim3(B), dim3(256), 0, st, s->dz_v, s->dz_w, s->lam_v, s->lam_w, ndz,
                            nlam);
     }
+    if (s->bwd_implicit > 0) {
+        // derivative of the solution: rounds of K w+ = [dL/dz; 0] + (G~ - H) v on the adjoint chain (k_perturb,
+        // the set-up at z +- eps v, k_hvp_rhs, then a refinement round with that right-hand side), H from the step's
+        // multipliers lam_b; the step term of the parameter gradients is dropped (dz = 0 at a solution)
+        const gato::ResidChain ri{s->imp_rz, nullptr, nullptr, nullptr, s->dz_v, s->lam_v, s->q_w, s->r_w, s->c_w,
+                                  s->gam_w};
+        constexpr int NXp = gato::Plant<PL>::NX, NUp = gato::Plant<PL>::NU;
+        const size_t lds64 = (size_t)(2 * NXp + NUp + 8 + 8 * gato::lin64_group_doubles<PL>()) * sizeof(double);
+        for (int it = 0; it < s->bwd_implicit; it++) {
+            hipLaunchKernelGGL(gato::k_perturb, dim3(B), dim3(256), 0, st, ab.xu, s->dz_v, s->imp_zp, s->imp_zm,
+                               s->imp_eps, s->traj, ndz, (double)s->bwd_fd_step);
+            hipLaunchKernelGGL((gato::k_lin64<PL>), dim3(B), dim3(8 * gato::KG), lds64, st, ab, s->imp_zp, s->imp_p);
+            hipLaunchKernelGGL((gato::k_lin64<PL>), dim3(B), dim3(8 * gato::KG), lds64, st, ab, s->imp_zm, s->imp_m);
+            hipLaunchKernelGGL((gato::k_hvp_rhs<PL>), dim3(B), blk, 0, st, ab, s->imp_p, s->imp_m, dL, s->dz_v,
+                               s->lam_b, s->imp_eps, s->imp_rz);
+            hipLaunchKernelGGL((gato::k_residual2<PL>), dim3(B), blk, lds, st, ab, rs, ri, 0);
+            launch_pcg_btd<PL>(np, B, blk, lds_btd, st, aw, s->bwd_G, s->bwd_Di);
+            hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, aw, 0);
+            hipLaunchKernelGGL(gato::k_accumulate, dim3(B), dim3(256), 0, st, s->dz_v, s->dz_w, s->lam_v, s->lam_w,
+                               ndz, nlam);
+        }
+        hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, s->dz_b, (size_t)B * ndz);
+    }
     hipLaunchKernelGGL((gato::k_grad<PL>), dim3(B), blk, lds, st, ab, s->dz_v, s->lam_v, gxs, gref, gQ, gR);
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// The implicit rounds' workspace (allocated on the first gato_set_backward_implicit with rounds > 0): per-problem
+// steps and the perturbed trajectories z +- eps v in fp64, the augmented right-hand side, and the fp64 KKT set-up
+// blocks at the two perturbed trajectories.
+int alloc_implicit(gato_solver* s) {
+    if (s->imp_pool) return 0;
+    const size_t N = s->N, B = s->B, nx = s->nx, nu = s->nu, nq = s->nq, traj = s->traj;
+    const size_t set = N * (nx * nx + nx * nu + nx + nq * nq + nq + nu + nx + nu);  // A Bm c Qp Qv Rg q r
+    const size_t per = 1 + 2 * traj + 2 * set;                                     // doubles
+    const size_t bytes = (B * per + 32 * 16) * sizeof(double) + (B * traj + 64) * sizeof(float);
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(GATO_E_ALLOC, std::string("hipMalloc implicit workspace: ") + hipGetErrorString(e));
+    s->imp_pool = static_cast<char*>(p);
+    size_t off = 0;  // in doubles, 256-byte aligned arrays
+    auto D = [&](size_t n) {
+        off = (off + 31) & ~size_t(31);
+        double* q = reinterpret_cast<double*>(s->imp_pool) + off;
+        off += n;
+        return q;
+    };
+    s->imp_eps = D(B);
+    s->imp_zp = D(B * traj);
+    s->imp_zm = D(B * traj);
+    for (gato::Lin64* a : {&s->imp_p, &s->imp_m}) {
+        a->A = D(B * N * nx * nx);
+        a->Bm = D(B * N * nx * nu);
+        a->c = D(B * N * nx);
+        a->Qp = D(B * N * nq * nq);
+        a->Qv = D(B * N * nq);
+        a->Rg = D(B * N * nu);
+        a->q = D(B * N * nx);
+        a->r = D(B * N * nu);
+    }
+    s->imp_rz = reinterpret_cast<float*>(D((B * traj + 1) / 2 + 1));
     return 0;
 }
 
@@ -2594,6 +2794,7 @@ int gato_destroy(gato_solver* s) {
     drop_graph(s);
     if (s->cap_stream) hipStreamDestroy(s->cap_stream);
     if (s->pool) hipFree(s->pool);
+    if (s->imp_pool) hipFree(s->imp_pool);
     delete s;
     return 0;
 }
@@ -2977,6 +3178,19 @@ int gato_debug_copy(gato_solver* s, const char* name, void* d_dst, int64_t cap, 
         return 0;
     }
     return fail(GATO_E_ARG, std::string("unknown workspace array ") + name);
+}
+
+int gato_set_backward_implicit(gato_solver* s, int32_t rounds, float fd_step) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    if (rounds < 0 || rounds > 32 || !(fd_step > 0.f) || !(fd_step <= 1.f))
+        return fail(GATO_E_ARG, "need rounds in [0, 32] and fd_step in (0, 1]");
+    if (rounds > 0) {
+        const int rc = alloc_implicit(s);
+        if (rc) return rc;
+    }
+    s->bwd_implicit = rounds;
+    s->bwd_fd_step = fd_step;
+    return 0;
 }
 
 int gato_set_backward_params(gato_solver* s, int32_t max_pcg_iters, float pcg_tol, int32_t refine) {

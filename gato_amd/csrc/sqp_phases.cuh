@@ -45,43 +45,45 @@ __device__ __forceinline__ float barrier(float v, float lo, float hi) {
     return -logf(a) - logf(b);
 }
 // gradient: iiwa14 sign-preserving clamp (iiwa14_plant.cuh:114-136), indy7 plain clamp (indy7_plant.cuh:141-148)
-template <int PL>
-__device__ __forceinline__ float barrier_g(float v, float lo, float hi) {
-    float a = v - lo, b = hi - v;
-    const float eps = 1e-6f;
+template <int PL, typename S = float>
+__device__ __forceinline__ S barrier_g(S v, S lo, S hi) {
+    S a = v - lo, b = hi - v;
+    const S eps = S(1e-6f);
     if (PL == 1) {
         a = a <= eps ? eps : a;
         b = b <= eps ? eps : b;
     } else {
-        a = a >= 0.f ? (a < eps ? eps : a) : (a > -eps ? -eps : a);
-        b = b >= 0.f ? (b < eps ? eps : b) : (b > -eps ? -eps : b);
+        a = a >= S(0) ? (a < eps ? eps : a) : (a > -eps ? -eps : a);
+        b = b >= S(0) ? (b < eps ? eps : b) : (b > -eps ? -eps : b);
     }
-    return -__builtin_amdgcn_rcpf(a) + __builtin_amdgcn_rcpf(b);  // fast-math reciprocals (-use_fast_math)
+    return -rcp(a) + rcp(b);  // fp32: fast-math reciprocals (-use_fast_math)
 }
 // iiwa14 curvature 1/d_lo^2 + 1/d_hi^2 with |d| >= 1e-6 (iiwa14_plant.cuh:141-155)
-__device__ __forceinline__ float barrier_h(float v, float lo, float hi) {
-    float a = fabsf(v - lo), b = fabsf(hi - v);
-    a = a < 1e-6f ? 1e-6f : a;
-    b = b < 1e-6f ? 1e-6f : b;
-    return __builtin_amdgcn_rcpf(a * a) + __builtin_amdgcn_rcpf(b * b);
+template <typename S = float>
+__device__ __forceinline__ S barrier_h(S v, S lo, S hi) {
+    S a = fabs(v - lo), b = fabs(hi - v);
+    a = a < S(1e-6f) ? S(1e-6f) : a;
+    b = b < S(1e-6f) ? S(1e-6f) : b;
+    return rcp(a * a) + rcp(b * b);
 }
 
-template <typename Tarr>
-__device__ __forceinline__ float pick(const Tarr& arr, int n, int idx) {
-    float r = 0.f;
+template <typename Tarr, typename S = float>
+__device__ __forceinline__ S pick(const Tarr& arr, int n, int idx) {
+    S r = S(0);
 #pragma unroll
     for (int i = 0; i < n; i++) r = (i == idx) ? arr[i] : r;
     return r;
 }
 
-template <int PL>
-__device__ __forceinline__ float plim(int which, int i) {
+template <int PL, typename S = float>
+__device__ __forceinline__ S plim(int which, int i) {
     using P = Plant<PL>;
-    float r = 0.f;
+    S r = S(0);
 #pragma unroll
     for (int k = 0; k < P::NQ; k++) {
-        const float v = which == 0 ? P::qlo(k) : which == 1 ? P::qhi(k) : which == 2 ? P::vlo(k)
-                       : which == 3 ? P::vhi(k) : which == 4 ? P::ulo(k) : P::uhi(k);
+        const S v = which == 0 ? P::template qlo<S>(k) : which == 1 ? P::template qhi<S>(k)
+                  : which == 2 ? P::template vlo<S>(k) : which == 3 ? P::template vhi<S>(k)
+                  : which == 4 ? P::template ulo<S>(k) : P::template uhi<S>(k);
         r = (k == i) ? v : r;
     }
     return r;
@@ -100,33 +102,33 @@ __host__ __device__ constexpr int lin_lds_floats() {
 // k <= N-2: A_k, B_k, c_{k+1} and the cost model at (x_k, u_k, ref_k);
 // k == N-1: terminal cost model (quirk Q4: at x_{N-2} with ref_{N-1} and q_cost) and c_0 = x_0 - x_s.
 // xk points at [x_k u_k x_{k+1}] (k <= N-2) or is unused; xt is the terminal-block state.
-template <int PL>
-__device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const CostW& cw, const float* xk,
-                                         const float* xt, const float* refk, const float* xrefk, const float* Qd,
-                                         const float* Rd, const float* fext, const float* x0, const float* xs,
-                                         float* A, float* Bm, float* cn, float* c0, float* Qp, float* Qv, float* Rg,
-                                         float* qv, float* rv, float* shg) {
+template <int PL, typename S = float>
+__device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const CostW& cw, const S* xk,
+                                         const S* xt, const S* refk, const S* xrefk, const S* Qd,
+                                         const S* Rd, const S* fext, const S* x0, const S* xs,
+                                         S* A, S* Bm, S* cn, S* c0, S* Qp, S* Qv, S* Rg,
+                                         S* qv, S* rv, S* shg) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
     const bool term = (k == N - 1);
-    const float* xsrc = term ? xt : xk;
-    float q[NQ], qd[NQ], u[NQ];
+    const S* xsrc = term ? xt : xk;
+    S q[NQ], qd[NQ], u[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; i++) {
         q[i] = xsrc[i];
         qd[i] = xsrc[NQ + i];
-        u[i] = term ? 0.f : xk[NX + i];
+        u[i] = term ? S(0) : xk[NX + i];
     }
-    Kin<PL> K;
+    Kin<PL, S> K;
     K.set(q);
     // knot-group LDS: RNEA table, Cholesky factor of M (identical in every lane of the group,
     // published by lane 0 so that each lane's tangent pass keeps only its own derivative buffer)
-    float* tab = shg;
-    float* shL = tab + RneaTab<PL>::SIZE;
-    float* ax = shL + NQ * (NQ + 1) / 2;
+    S* tab = shg;
+    S* shL = tab + RneaTab<PL>::SIZE;
+    S* ax = shL + NQ * (NQ + 1) / 2;
     if (!term) {
-        float qdd[NQ];
-        FLds fs{ax, 1};
+        S qdd[NQ];
+        FLdsT<S> fs{ax, 1};
         forward_dynamics_fs<PL>(K, qd, u, fext, qdd, shL, fs);
         __builtin_amdgcn_sched_barrier(0);
         rnea_publish<PL>(K, qd, qdd, fext, tab);
@@ -135,14 +137,14 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const
         const int col = lane < NX ? lane : 0;
         const int j = col < NQ ? col : col - NQ;
         const int dir = col < NQ ? 0 : 1;
-        float dq[NQ];
+        S dq[NQ];
         rnea_tangent_ax<PL>(K, qd, qdd, tab, ax, j, dir, dq);
 #pragma unroll
         for (int i = 0; i < NQ; i++) dq[i] = -dq[i];
         chol_solve<NQ>(shL, dq);  // d qdd / d x_col
-        float mc[NQ];
+        S mc[NQ];
 #pragma unroll
-        for (int i = 0; i < NQ; i++) mc[i] = (i == j) ? 1.f : 0.f;
+        for (int i = 0; i < NQ; i++) mc[i] = (i == j) ? S(1) : S(0);
         chol_solve<NQ>(shL, mc);  // M^-1 e_j
         // materialise both columns here: otherwise the compiler sinks the whole tangent sweep into the
         // lane-predicated stores below and keeps every intermediate of it live across the branch
@@ -151,12 +153,12 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const
             dq[i] = opaque(dq[i]);
             mc[i] = opaque(mc[i]);
         }
-        const float h2 = 0.5f * dt * dt;
+        const S h2 = S(0.5) * dt * dt;
         if (lane < NX) {  // column `lane` of A (integrator.cuh:143-169)
 #pragma unroll
             for (int r = 0; r < NQ; r++) {
-                float top = (r == lane ? 1.f : 0.f) + ((lane >= NQ && r == lane - NQ) ? dt : 0.f) + h2 * dq[r];
-                float bot = ((NQ + r) == lane ? 1.f : 0.f) + dt * dq[r];
+                S top = (r == lane ? S(1) : S(0)) + ((lane >= NQ && r == lane - NQ) ? dt : S(0)) + h2 * dq[r];
+                S bot = ((NQ + r) == lane ? S(1) : S(0)) + dt * dq[r];
                 A[r * NX + lane] = top;
                 A[(NQ + r) * NX + lane] = bot;
             }
@@ -169,19 +171,19 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const
             }
         }
         if (lane < NQ) {  // c_{k+1} = x_{k+1} - x+ (integrator.cuh:49-62)
-            const float ql = xsrc[lane], qdl = xsrc[NQ + lane], al = pick(qdd, NQ, lane);
-            const float* xn = xk + NX + NU;
-            cn[lane] = xn[lane] - (ql + dt * qdl + 0.5f * al * dt * dt);
+            const S ql = xsrc[lane], qdl = xsrc[NQ + lane], al = pick<S[NQ], S>(qdd, NQ, lane);
+            const S* xn = xk + NX + NU;
+            cn[lane] = xn[lane] - (ql + dt * qdl + S(0.5) * al * dt * dt);
             cn[NQ + lane] = xn[NQ + lane] - (qdl + dt * al);
         }
     }
     // cost model (iiwa14_plant.cuh:339-424); joint-space diagonal weights generalise qd_cost/u_cost.
     // Lane scalars are re-read from memory and the kinematics restart from opaque copies, so nothing of
     // the dynamics block above stays live into this section.
-    const float w = term ? (cw.terminal_mode ? cw.N_cost : cw.q_cost) : cw.q_cost;
-    float p[3], Jc[3];
+    const S w = S(term ? (cw.terminal_mode ? cw.N_cost : cw.q_cost) : cw.q_cost);
+    S p[3], Jc[3];
     {
-        Kin<PL> K2;
+        Kin<PL, S> K2;
 #pragma unroll
         for (int i = 0; i < NQ; i++) {
             K2.s[i] = opaque(K.s[i]);
@@ -189,43 +191,43 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, float dt, const
         }
         ee_position<PL>(K2, lane < NQ ? lane : -1, p, Jc);
     }
-    const float e0 = p[0] - refk[0], e1 = p[1] - refk[1], e2 = p[2] - refk[2];
-    const float gl = Jc[0] * e0 + Jc[1] * e1 + Jc[2] * e2;
+    const S e0 = p[0] - refk[0], e1 = p[1] - refk[1], e2 = p[2] - refk[2];
+    const S gl = Jc[0] * e0 + Jc[1] * e1 + Jc[2] * e2;
     if (lane < NQ) {
         shg[lane] = gl;
         if (cw.hess_mode) { shg[NQ + 3 * lane] = Jc[0]; shg[NQ + 3 * lane + 1] = Jc[1]; shg[NQ + 3 * lane + 2] = Jc[2]; }
     }
     wave_sync();
     if (lane < NQ) {
-        const float ql = xsrc[lane], qdl = xsrc[NQ + lane];
-        const float bgl = barrier_g<PL>(ql, plim<PL>(0, lane), plim<PL>(1, lane));
+        const S ql = xsrc[lane], qdl = xsrc[NQ + lane];
+        const S bgl = barrier_g<PL, S>(ql, plim<PL, S>(0, lane), plim<PL, S>(1, lane));
 #pragma unroll
         for (int m = 0; m < NQ; m++) {
-            float v;
+            S v;
             if (cw.hess_mode)
                 v = w * (Jc[0] * shg[NQ + 3 * m] + Jc[1] * shg[NQ + 3 * m + 1] + Jc[2] * shg[NQ + 3 * m + 2]);
             else
                 v = w * gl * shg[m];  // rank-1 EE term (quirk Q5)
-            if (PL == 1) v += cw.q_lim * bgl * barrier_g<PL>(xsrc[m], P::qlo(m), P::qhi(m));
+            if (PL == 1) v += cw.q_lim * bgl * barrier_g<PL, S>(xsrc[m], P::template qlo<S>(m), P::template qhi<S>(m));
             if (m == lane) {
                 v += Qd[lane];
-                if (PL != 1) v += cw.q_lim * barrier_h(ql, plim<PL>(0, lane), plim<PL>(1, lane));
+                if (PL != 1) v += cw.q_lim * barrier_h<S>(ql, plim<PL, S>(0, lane), plim<PL, S>(1, lane));
             }
             Qp[lane * NQ + m] = v;
         }
-        const float xr_q = xrefk ? xrefk[lane] : 0.f, xr_v = xrefk ? xrefk[NQ + lane] : 0.f;
+        const S xr_q = xrefk ? xrefk[lane] : S(0), xr_v = xrefk ? xrefk[NQ + lane] : S(0);
         qv[lane] = w * gl + cw.q_lim * bgl + Qd[lane] * (ql - xr_q);
-        const float bv = barrier_g<PL>(qdl, plim<PL>(2, lane), plim<PL>(3, lane));
+        const S bv = barrier_g<PL, S>(qdl, plim<PL, S>(2, lane), plim<PL, S>(3, lane));
         qv[NQ + lane] = Qd[NQ + lane] * (qdl - xr_v) + cw.vel_lim * bv;
-        Qv[lane] = Qd[NQ + lane] + cw.vel_lim * (PL == 1 ? bv * bv : barrier_h(qdl, plim<PL>(2, lane), plim<PL>(3, lane)));
+        Qv[lane] = Qd[NQ + lane] + cw.vel_lim * (PL == 1 ? bv * bv : barrier_h<S>(qdl, plim<PL, S>(2, lane), plim<PL, S>(3, lane)));
         if (!term) {
-            const float ul = xk[NX + lane];
-            const float bu = barrier_g<PL>(ul, plim<PL>(4, lane), plim<PL>(5, lane));
+            const S ul = xk[NX + lane];
+            const S bu = barrier_g<PL, S>(ul, plim<PL, S>(4, lane), plim<PL, S>(5, lane));
             rv[lane] = Rd[lane] * ul + cw.ctrl_lim * bu;
-            Rg[lane] = Rd[lane] + cw.ctrl_lim * (PL == 1 ? bu * bu : barrier_h(ul, plim<PL>(4, lane), plim<PL>(5, lane)));
+            Rg[lane] = Rd[lane] + cw.ctrl_lim * (PL == 1 ? bu * bu : barrier_h<S>(ul, plim<PL, S>(4, lane), plim<PL, S>(5, lane)));
         } else {
-            rv[lane] = 0.f;
-            Rg[lane] = 1.f;
+            rv[lane] = S(0);
+            Rg[lane] = S(1);
         }
     }
     if (term && lane < NX) c0[lane] = x0[lane] - xs[lane];
@@ -520,7 +522,7 @@ __device__ __forceinline__ float merit_item(int k, int N, float alpha, float dt,
     Kin<PL> K;
     K.set(q);
     float p[3];
-    ee_position<PL>(K, -1, p, nullptr);
+    ee_position<PL>(K, -1, p, static_cast<float*>(nullptr));
     const float w = last ? cw.N_cost : cw.q_cost;
 #pragma unroll
     for (int i = 0; i < 3; i++) {

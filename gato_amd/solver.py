@@ -288,6 +288,11 @@ class Solver:
         g["adj_pcg_iters"] = it
         return g
 
+    def set_backward_implicit(self, rounds=8, fd_step=1e-4):
+        """Backward returns the derivative of the solution (rounds > 0; see gato_set_backward_implicit) instead of
+        the one-step derivative (rounds = 0)."""
+        check(self.lib.gato_set_backward_implicit(self._h, int(rounds), float(fd_step)), "gato_set_backward_implicit")
+
     def set_backward_params(self, max_pcg_iters=-1, pcg_tol=1e-6, refine=2):
         check(self.lib.gato_set_backward_params(self._h, int(max_pcg_iters), float(pcg_tol), int(refine)),
               "gato_set_backward_params")

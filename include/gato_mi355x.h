@@ -157,6 +157,17 @@ int gato_backward(gato_solver* s, const float* d_xu, const float* d_xs, const fl
  * residuals come from the KKT blocks, correct it. */
 int gato_set_backward_params(gato_solver* s, int32_t max_pcg_iters, float pcg_tol, int32_t refine);
 
+/* Backward, derivative of the solution (no reference counterpart). With rounds > 0 the backward returns the
+ * implicit derivative of the solution z*(theta), defined by F(z, lambda; theta) = [g + C^T lambda; c] = 0, instead of
+ * the derivative of one SQP step: the adjoint system then has the exact Jacobian of F (the Hessian of the Lagrangian,
+ * dynamics curvature included, in place of the step's model G~). Each round is one iteration K w+ = [dL/dz; 0] +
+ * (G~ - H) w, run as a refinement round of the adjoint solve, with H w from central differences of the stationarity
+ * rows at z +- eps w, eps = fd_step / |w|_inf (default 1e-4), the two KKT set-ups evaluated in fp64; the rounds
+ * contract at the SQP step's own local rate (about 6x per round where the solve converges). The result is the gradient of the solution where d_xu is one (a converged solve; with
+ * the reference's terminal block, terminal_mode 0, the step has no fixed point); 0 rounds (the default) keeps the
+ * one-step derivative. The first call with rounds > 0 allocates the rounds' workspace (about twice the KKT blocks). */
+int gato_set_backward_implicit(gato_solver* s, int32_t rounds, float fd_step);
+
 /* Extension: launch shape. `groups` 16-lane knot groups per work-group (a multiple of 4 in [4, 32], at most
  * the default 4*ceil(min(N,32)/4), with ceil(N / groups) in {1, 2, 4, 8}); threads = 16 * groups. */
 int gato_set_launch_shape(gato_solver* s, int32_t groups);

@@ -144,3 +144,103 @@ def test_autograd_backward_uses_its_own_forward_state(gpu):
     np.testing.assert_array_equal(now["rho"], after2["rho"])
     (XU2 ** 2).sum().backward()
     assert all(torch.isfinite(x.grad).all() for x in in2)
+
+
+def _config4_problems(O, n, N=32, seed=3):
+    """Problems of the differentiation configuration (iiwa14 figure-8, weights log-uniform on [1e-3, 1e2], the
+    consistent terminal block) whose step has a fixed point from the figure-8 start in FP64, with that solution."""
+    wl = W.drop_in_figure8("iiwa14", N=N, B=64, seed=seed)
+    rng = np.random.default_rng(seed)
+    Qd = np.exp(rng.uniform(np.log(1e-3), np.log(1e2), (64, 14)))
+    Rd = np.exp(rng.uniform(np.log(1e-3), np.log(1e2), (64, 7)))
+    cfg = O.make_cfg("iiwa14", N=N, dt=0.01, q_cost=10.0, N_cost=10.0, qd_cost=1e-3, u_cost=1e-3, q_lim_cost=0.0,
+                     terminal_mode=1, hess_mode=0)
+    out = []
+    for b in range(64):
+        z, lam, res = O.fixed_point(cfg, wl.xu[b], wl.xs[b], wl.ref[b], Qd=Qd[b], Rd=Rd[b], iters=40)
+        if res <= 1e-12:
+            out.append((wl.xs[b], wl.ref[b], Qd[b], Rd[b], z))
+        if len(out) == n:
+            break
+    assert len(out) == n
+    return cfg, out
+
+
+def _loss4(O, z, ref, N=32, nx=14, nu=7):
+    st = nx + nu
+    g = np.zeros_like(z)
+    g[nx:nx + nu] = 2 * z[nx:nx + nu]
+    L = z[nx:nx + nu] @ z[nx:nx + nu]
+    r = ref.reshape(N, 6)
+    for k in range(N):
+        p, J = O.fk("iiwa14", z[k * st:k * st + 7])
+        d = p - r[k, :3]
+        L += d @ d
+        g[k * st:k * st + 7] += 2 * J.T @ d
+    return L, g
+
+
+def test_implicit_backward_is_the_gradient_of_the_solution(gpu, O):
+    # The differentiation configuration at the problems whose step has a fixed point: the GPU solve settles at the
+    # FP64 solution (started there), then the backward with implicit rounds must give the oracle's derivative of the
+    # solution (exact KKT Jacobian, pinned by central differences of re-solves on the CPU) at the GPU's own point, to
+    # 1e-3 of each gradient's largest entry. Config 4's own check runs too: central differences with eps = 1e-3 of the
+    # FP64 re-solved loss for x_s and the control log-weights of one problem. The one-step gradient misses both by
+    # more than 5%.
+    from gato_amd.solver import Solver
+
+    N, B = 32, 3
+    cfg, probs = _config4_problems(O, B, N)
+    xs = np.array([p[0] for p in probs])
+    ref = np.array([p[1] for p in probs])
+    Qd = np.array([p[2] for p in probs])
+    Rd = np.array([p[3] for p in probs])
+    z0 = np.array([p[4] for p in probs])
+    s = Solver("iiwa14", N, B, mode="graph", terminal_mode=1, dt=0.01, max_sqp_iters=2, max_pcg_iters=200,
+               pcg_tol=1e-7, mu=1e3, q_cost=10.0, N_cost=10.0, qd_cost=1e-3, u_cost=1e-3, q_lim_cost=0.0, rho=1e-3)
+    s.set_joint_cost(Qd.astype(np.float32), Rd.astype(np.float32))
+    dev = s.device
+    t = lambda a: torch.tensor(a, dtype=torch.float32, device=dev).contiguous()
+    xu, xs_d, ref_d = t(z0), t(xs), t(ref)
+    s.solve_device(xu, xs_d, ref_d)
+    Z = xu.cpu().numpy().astype(np.float64)
+    dL = np.array([_loss4(O, Z[b], ref[b])[1] for b in range(B)])
+    s.set_backward_params(max_pcg_iters=-1, pcg_tol=1e-6, refine=2)
+    g1 = s.backward_device(xu, xs_d, ref_d, t(dL))
+    s.set_backward_implicit(rounds=8)
+    gi = s.backward_device(xu, xs_d, ref_d, t(dL))
+    keys = ("xs", "ref", "Qd", "Rd")
+    worst_one = 0.0
+    for b in range(B):
+        zb = Z[b]
+        _, lam, _ = O.step_map(cfg, zb, xs[b], ref[b], 1e-3, Qd=Qd[b], Rd=Rd[b])
+        o = O.solution_grad(cfg, zb, lam, xs[b], ref[b], dL[b], Qd=Qd[b], Rd=Rd[b])
+        assert o["F_inf"] < 1e-2, (b, o["F_inf"])  # the fp32 solution is a solution to its rounding
+        for k in keys:
+            ob = o[k][:, :3].reshape(-1) if k == "ref" else o[k]
+            gb = gi[k][b].cpu().numpy().reshape(N, 6)[:, :3].reshape(-1) if k == "ref" else gi[k][b].cpu().numpy()
+            g1b = g1[k][b].cpu().numpy().reshape(N, 6)[:, :3].reshape(-1) if k == "ref" else g1[k][b].cpu().numpy()
+            assert _rel(gb, ob) < 1e-3, (b, k, _rel(gb, ob))
+            worst_one = max(worst_one, _rel(g1b, ob))
+    assert worst_one > 0.05
+    # central differences of the re-solved FP64 loss, x_s and log diag R of problem 0
+    eps = 1e-3
+
+    def resolved(xs_b, Rd_b):
+        zz, _, rr = O.fixed_point(cfg, Z[0], xs_b, ref[0], Qd=Qd[0], Rd=Rd_b, iters=40, tol=1e-11)
+        assert rr <= 1e-11
+        return _loss4(O, zz, ref[0])[0]
+
+    fd_xs = np.zeros(14)
+    for i in range(14):
+        e = np.zeros(14)
+        e[i] = eps
+        fd_xs[i] = (resolved(xs[0] + e, Rd[0]) - resolved(xs[0] - e, Rd[0])) / (2 * eps)
+    fd_R = np.zeros(7)
+    for i in range(7):
+        f = np.ones(7)
+        f[i] = np.exp(eps)
+        fd_R[i] = (resolved(xs[0], Rd[0] * f) - resolved(xs[0], Rd[0] / f)) / (2 * eps)
+    assert _rel(gi["xs"][0].cpu().numpy(), fd_xs) < 1e-3
+    assert _rel(gi["Rd"][0].cpu().numpy() * Rd[0], fd_R) < 1e-3
+    assert _rel(g1["xs"][0].cpu().numpy(), fd_xs) > 0.05
