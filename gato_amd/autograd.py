@@ -71,3 +71,24 @@ def solve(solver, xs, ref, Qd, Rd, xu0, return_stats=False):
     box = {} if return_stats else None
     out = _SQPSolve.apply(solver, xs, ref, Qd, Rd, xu0, box)
     return (out, box["stats"]) if return_stats else out
+
+
+class _EEPositions(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, solver, xu):
+        pos, jac = solver.ee_positions(xu.detach().contiguous().float(), jacobian=True)
+        ctx.solver, ctx.jac = solver, jac
+        return pos
+
+    @staticmethod
+    def backward(ctx, grad_pos):
+        s = ctx.solver
+        g = (grad_pos.float().unsqueeze(-1) * ctx.jac).sum(2)  # [B, N, nq] = sum_i dL/dp_i dp_i/dq
+        out = torch.zeros(s.B, s.N * (s.nx + s.nu), dtype=torch.float32, device=grad_pos.device)
+        out.view(s.B, s.N, s.nx + s.nu)[:, :, :s.nq] = g
+        return None, out[:, :s.traj]
+
+
+def ee_positions(solver, xu):
+    """Differentiable end-effector positions [B, N, 3] of trajectories xu [B, traj] (gato_ee_positions)."""
+    return _EEPositions.apply(solver, xu)

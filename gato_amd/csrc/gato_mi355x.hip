@@ -2019,6 +2019,31 @@ __global__ void k_force_update(ForceEst* st, const float* dirs, const int* best,
 }
 
 // ------------------------------------------------------------------ sim_forward (sim.cuh:14-48)
+// End-effector position p(q_k) (origin of the last joint frame) of every knot of every problem, and optionally its
+// Jacobian dp/dq (row-major 3 x nq per knot): one thread per (problem, knot).
+template <int PL>
+__global__ void __launch_bounds__(256) k_ee_positions(const float* xu, int traj, int N, int B, float* pos, float* jac) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, ST = P::NX + P::NU;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B * N) return;
+    const int b = t / N, k = t % N;
+    const float* x = xu + (size_t)b * traj + (size_t)k * ST;
+    float q[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) q[i] = x[i];
+    Kin<PL> K;
+    K.set(q);
+    float p[3], Jc[3];
+    ee_position<PL>(K, -1, p, static_cast<float*>(nullptr));
+    for (int i = 0; i < 3; i++) pos[(size_t)t * 3 + i] = p[i];
+    if (!jac) return;
+    for (int j = 0; j < NQ; j++) {
+        ee_position<PL>(K, j, p, Jc);
+        for (int i = 0; i < 3; i++) jac[(size_t)t * 3 * NQ + i * NQ + j] = Jc[i];
+    }
+}
+
 template <int PL>
 __global__ void k_sim(float* xn, const float* x, const float* u, const float* fext, float dt, int B) {
     using P = Plant<PL>;
@@ -2919,6 +2944,20 @@ int gato_sim_forward(gato_solver* s, float* d_xnext, const float* d_x, const flo
         hipLaunchKernelGGL(gato::k_sim<1>, dim3(nb), dim3(64), 0, st, d_xnext, d_x, d_u, s->d_fext, dt, s->B);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int gato_ee_positions(gato_solver* s, const float* d_xu, float* d_pos, float* d_jac, void* stream) {
+    if (!s || !d_xu || !d_pos) return fail(GATO_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int n = s->B * s->N;
+    if (s->plant == GATO_PLANT_IIWA14)
+        hipLaunchKernelGGL(gato::k_ee_positions<0>, dim3((n + 255) / 256), dim3(256), 0, st, d_xu, s->traj, s->N, s->B,
+                           d_pos, d_jac);
+    else
+        hipLaunchKernelGGL(gato::k_ee_positions<1>, dim3((n + 255) / 256), dim3(256), 0, st, d_xu, s->traj, s->N, s->B,
+                           d_pos, d_jac);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 

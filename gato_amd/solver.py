@@ -288,6 +288,18 @@ class Solver:
         g["adj_pcg_iters"] = it
         return g
 
+    def ee_positions(self, xu, jacobian=False):
+        """End-effector positions [B, N, 3] of the device trajectories xu [B, traj] (and their Jacobians dp/dq
+        [B, N, 3, nq] when asked)."""
+        if not (isinstance(xu, torch.Tensor) and xu.is_cuda and xu.dtype == torch.float32 and xu.is_contiguous()
+                and xu.numel() == self.B * self.traj):
+            raise ValueError(f"xu: expected a contiguous float32 device tensor with {self.B * self.traj} elements")
+        pos = torch.empty(self.B, self.N, 3, dtype=torch.float32, device=self.device)
+        jac = torch.empty(self.B, self.N, 3, self.nq, dtype=torch.float32, device=self.device) if jacobian else None
+        check(self.lib.gato_ee_positions(self._h, _ptr(xu), _ptr(pos), None if jac is None else _ptr(jac),
+                                         self._stream()), "gato_ee_positions")
+        return (pos, jac) if jacobian else pos
+
     def set_backward_implicit(self, rounds=8, fd_step=1e-4):
         """Backward returns the derivative of the solution (rounds > 0; see gato_set_backward_implicit) instead of
         the one-step derivative (rounds = 0)."""

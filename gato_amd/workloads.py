@@ -128,6 +128,19 @@ def mixed_figure8(B=1024, seed=2, dt=0.01, period=4.0, knots=(8, 32, 128)):
     return out
 
 
+def differentiation_figure8(N=32, B=256, seed=3, dt=0.01, plant="iiwa14"):
+    """Differentiation workload: the drop-in figure-8 problems with per-problem diagonal joint weights Qd [B, nx] and
+    Rd [B, nu] drawn log-uniformly from [1e-3, 1e2] (the joint terms 1/2 Qd x^2 + 1/2 Rd u^2 beside the end-effector
+    term of weight q_cost = N_cost = 10). The loss of the configuration is sum_b |u_0|^2 + sum_k |p_k - ref_k|^2."""
+    wl = drop_in_figure8(plant, N=N, B=B, seed=seed, dt=dt)
+    rng = np.random.default_rng(seed)
+    nq = NQ[plant]
+    wl.Qd = np.exp(rng.uniform(np.log(1e-3), np.log(1e2), (B, 2 * nq)))
+    wl.Rd = np.exp(rng.uniform(np.log(1e-3), np.log(1e2), (B, nq)))
+    wl.params.update(q_cost=10.0, N_cost=10.0)
+    return wl
+
+
 def scale_figure8(N=128, B=4096, seed=4, dt=0.01, period=4.0, plant="iiwa14"):
     """Scale workload: B problems of horizon N, problem b tracking a figure-8 of seeded radius ~U(0.1, 0.25) m
     and phase ~U(0, 2 pi); weights are set by the caller (the learning loop owns diag Q and diag R)."""

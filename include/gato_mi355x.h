@@ -195,6 +195,12 @@ int gato_mixed_solve(gato_mixed* m, float* const* d_xu, const float* const* d_xs
 /* Device time of the last mixed graph launch (hipEvents on the caller's stream), microseconds. */
 int gato_mixed_last_time(gato_mixed* m, double* us);
 
+/* End-effector positions of a batch of trajectories d_xu [B * traj] (device): d_pos [B * N * 3] receives p(q_k), the
+ * origin of the last joint frame as the cost uses it (iiwa14_grid.cuh:2596-2664), and d_jac [B * N * 3 * nq] (may be
+ * NULL) its Jacobian dp/dq, row-major per knot. For losses on the solution's end-effector path (tracking error).
+ * Asynchronous. */
+int gato_ee_positions(gato_solver* s, const float* d_xu, float* d_pos, float* d_jac, void* stream);
+
 /* MPC hypothesis selection (the caller's per-step host work in mpc_controller.py:294-309 and 240-242, done on
  * the device): one trapezoidal step of (d_x_last [nx], d_u_last [nu]) under each problem's wrench
  * (gato_set_f_ext_batch), err_b = |x_next_b - d_x_curr|_2 into d_errors [B] (device, may be NULL), best =

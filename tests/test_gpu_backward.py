@@ -244,3 +244,50 @@ def test_implicit_backward_is_the_gradient_of_the_solution(gpu, O):
     assert _rel(gi["xs"][0].cpu().numpy(), fd_xs) < 1e-3
     assert _rel(gi["Rd"][0].cpu().numpy() * Rd[0], fd_R) < 1e-3
     assert _rel(g1["xs"][0].cpu().numpy(), fd_xs) > 0.05
+
+
+def test_differentiation_configuration_full_size(gpu, O):
+    # The differentiation configuration at full size: iiwa14, N = 32, B = 256, per-problem diagonal weights
+    # log-uniform on [1e-3, 1e2], loss sum_b |u_0|^2 + sum_k |p_k - ref_k|^2 through the autograd functions (the
+    # solve and the end-effector positions), gradients w.r.t. x_s, the reference, diag Q and diag R. Full size: every
+    # gradient finite, the end-effector op against the oracle's forward kinematics. On three sampled problems the
+    # one-step gradient (the backward's default) against the oracle's step gradient at the GPU's own solution, rho and
+    # weights, with the loss's explicit dependence on the reference added, to 1e-3.
+    from gato_amd.autograd import ee_positions, solve
+    from gato_amd.solver import Solver
+
+    wl = W.differentiation_figure8(N=32, B=256, seed=3)
+    p = wl.params
+    kw = {k: p[k] for k in ("dt", "max_sqp_iters", "max_pcg_iters", "pcg_tol", "mu", "q_cost", "qd_cost", "u_cost",
+                            "N_cost", "q_lim_cost", "vel_lim_cost", "ctrl_lim_cost", "rho")}
+    s = Solver("iiwa14", 32, 256, mode="graph", terminal_mode=1, **kw)
+    dev = s.device
+    t = lambda a, g=False: torch.tensor(a, dtype=torch.float32, device=dev, requires_grad=g)
+    xs, ref, Qt, Rt = t(wl.xs, True), t(wl.ref, True), t(wl.Qd, True), t(wl.Rd, True)
+    XU = solve(s, xs, ref, Qt, Rt, t(wl.xu))
+    pos = ee_positions(s, XU)
+    loss = (XU[:, 14:21] ** 2).sum() + ((pos - ref.view(256, 32, 6)[:, :, :3]) ** 2).sum()
+    loss.backward()
+    for x in (xs, ref, Qt, Rt):
+        assert x.grad is not None and torch.isfinite(x.grad).all()
+    Z = XU.detach().cpu().numpy().astype(np.float64)
+    P = pos.detach().cpu().numpy()
+    for b in (0, 100, 255):
+        for k in (0, 17, 31):
+            pk, _ = O.fk("iiwa14", Z[b, k * 21:k * 21 + 7])
+            assert np.abs(P[b, k] - pk).max() < 1e-5
+    _, rho, _ = s.get_state()
+    cfg = O.make_cfg("iiwa14", N=32, dt=p["dt"], q_cost=p["q_cost"], N_cost=p["N_cost"], qd_cost=p["qd_cost"],
+                     u_cost=p["u_cost"], q_lim_cost=0.0, terminal_mode=1)
+    f = lambda a: a.astype(np.float32).astype(np.float64)
+    for b in (0, 128, 255):
+        L, dL = _loss4(O, Z[b], f(wl.ref[b]))
+        o = O.step_grad(cfg, Z[b], f(wl.xs[b]), f(wl.ref[b]), float(rho[b]), dL, Qd=f(wl.Qd[b]), Rd=f(wl.Rd[b]))
+        expl = np.zeros((32, 6))
+        for k in range(32):
+            pk, _ = O.fk("iiwa14", Z[b, k * 21:k * 21 + 7])
+            expl[k, :3] = -2 * (pk - f(wl.ref[b]).reshape(32, 6)[k, :3])
+        assert _rel(xs.grad[b].cpu().numpy(), o["xs"]) < 1e-3, b
+        assert _rel(ref.grad[b].cpu().numpy().reshape(32, 6), o["ref"] + expl) < 1e-3, b
+        assert _rel(Qt.grad[b].cpu().numpy(), o["Qd"]) < 1e-3, b
+        assert _rel(Rt.grad[b].cpu().numpy(), o["Rd"]) < 1e-3, b
