@@ -43,6 +43,18 @@ def build():
 _lib = None
 
 
+def use_native():
+    """Build (make native: -march=native for this host) and load the oracle tuned for the machine it runs on, for
+    the CPU baseline's timings; returns the library path."""
+    import subprocess
+    global _lib, LIB_PATH
+    subprocess.run(["make", "-s", "-C", str(HERE), "native"], check=True, capture_output=True)
+    LIB_PATH = HERE / "_build_native" / "libgato_oracle.so"
+    _lib = None
+    lib()
+    return LIB_PATH
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -279,15 +291,34 @@ def _joint_weights(cfg, Qd, Rd):
     return Qw, Rw
 
 
-def step_map(cfg, xu, xs, ref, rho, fext=None, Qd=None, Rd=None, xref=None):
-    """dz and lambda of one SQP step at xu by a dense FP64 solve of the step's KKT system."""
+class _KKTSolver:
+    """Solves with one KKT matrix: LAPACK dense LU for small systems, a sparse LU (the matrix is block-banded) from
+    2000 unknowns on, factorised once for all right-hand sides. Both are direct FP64 solves."""
+
+    def __init__(self, K):
+        self.K = K
+        self.lu = None
+        if K.shape[0] >= 2000:
+            from scipy.sparse import csc_matrix
+            from scipy.sparse.linalg import splu
+            self.lu = splu(csc_matrix(K))
+
+    def solve(self, rhs):
+        return self.lu.solve(rhs) if self.lu is not None else np.linalg.solve(self.K, rhs)
+
+
+def step_map(cfg, xu, xs, ref, rho, fext=None, Qd=None, Rd=None, xref=None, _solver=None):
+    """dz and lambda of one SQP step at xu by a direct FP64 solve of the step's KKT system."""
     nq_, nx, nu = dims(cfg)
     Qw, Rw = _joint_weights(cfg, Qd, Rd)
     kkt = setup_kkt(cfg, xu, xs, ref, fext, Qw, Rw, xref)
     G, g, Cm, c = dense_kkt(kkt, rho, nq_)
     nz, nc = G.shape[0], Cm.shape[0]
     K = np.block([[G, Cm.T], [Cm, np.zeros((nc, nc))]])
-    w = np.linalg.solve(K, np.concatenate([-g, c]))
+    ks = _KKTSolver(K)
+    if _solver is not None:
+        _solver.append(ks)
+    w = ks.solve(np.concatenate([-g, c]))
     return w[:nz], w[nz:], K
 
 
@@ -298,9 +329,10 @@ def step_grad(cfg, xu, xs, ref, rho, dL_dz, fext=None, Qd=None, Rd=None, xref=No
     N = cfg.N
     Qw, Rw = _joint_weights(cfg, Qd, Rd)
     xu = _f64(xu)
-    dz_, lam, K = step_map(cfg, xu, xs, ref, rho, fext, Qw, Rw, xref)
+    ks = []
+    dz_, lam, K = step_map(cfg, xu, xs, ref, rho, fext, Qw, Rw, xref, _solver=ks)
     nz = dz_.shape[0]
-    v = np.linalg.solve(K, np.concatenate([_f64(dL_dz), np.zeros(N * nx)]))
+    v = ks[0].solve(np.concatenate([_f64(dL_dz), np.zeros(N * nx)]))
     vz, vl = v[:nz], v[nz:]
     xr = np.zeros((N, nx)) if xref is None else _f64(xref).reshape(N, nx)
     ref = _f64(ref).reshape(N, 6)
