@@ -43,6 +43,17 @@ SIGNATURES = {
     "gato_solve_noalloc": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(GatoParams), _VP, _VP, _VP, C.c_float,
                                      _VP, C.POINTER(GatoStats)]),
     "gato_set_joint_cost": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "gato_create_f64": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32, C.POINTER(GatoParams)]),
+    "gato_destroy_f64": (C.c_int, [_VP]),
+    "gato_solve_f64": (C.c_int, [_VP, _VP, _VP, _VP, C.c_double, _VP, C.POINTER(GatoStats)]),
+    "gato_last_merits_f64": (C.c_int, [_VP, _dp, _dp]),
+    "gato_set_joint_cost_f64": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
+    "gato_set_f_ext_batch_f64": (C.c_int, [_VP, _VP, _VP]),
+    "gato_set_batch_f64": (C.c_int, [_VP, C.c_int32, _dp, C.c_int32]),
+    "gato_reset_f64": (C.c_int, [_VP, C.c_int32]),
+    "gato_set_rho_adaptation_f64": (C.c_int, [_VP, C.c_int32]),
+    "gato_get_state_f64": (C.c_int, [_VP, _VP, _dp, _dp]),
+    "gato_sim_forward_f64": (C.c_int, [_VP, _VP, _VP, _VP, C.c_double, _VP]),
     "gato_set_f_ext_batch": (C.c_int, [_VP, _VP, _VP]),
     "gato_set_rho_penalty_batch": (C.c_int, [_VP, _fp, C.c_int32]),
     "gato_set_drho_batch": (C.c_int, [_VP, _fp, C.c_int32]),

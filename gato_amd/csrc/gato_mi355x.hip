@@ -27,32 +27,42 @@
 
 namespace gato {
 
-struct Args {
+template <typename S>
+struct ArgsT {
     int N, B, traj, max_sqp, max_pcg, exit_rule, adapt_rho;
-    float dt, kkt_tol, solve_ratio;
-    float pcg_abs_tol;  // absolute part of the PCG exit test (1e-6 in the solve, pcg.cuh:27; tiny for the adjoint)
+    S dt;
+    float kkt_tol, solve_ratio;
+    S pcg_abs_tol;  // absolute part of the PCG exit test (1e-6 in the solve, pcg.cuh:27; tiny for the adjoint)
     CostW cw;
-    float* xu;
-    const float *xs, *ref, *fext, *Qd, *Rd, *xref;
-    float *lam, *rho, *drho;
-    const float *drho_init, *mu, *tol;
-    float *A, *Bm, *c, *Qp, *Qv, *Rg, *q, *r, *Qip, *Qiv, *Ri, *Sd, *So, *Pd, *Po, *gam, *dz, *mpart, *phi;
+    S* xu;
+    const S *xs, *ref, *fext, *Qd, *Rd, *xref;
+    S *lam, *rho, *drho;
+    const S *drho_init, *mu, *tol;
+    S *A, *Bm, *c, *Qp, *Qv, *Rg, *q, *r, *Qip, *Qiv, *Ri, *Sd, *So, *Pd, *Po, *gam, *dz, *mpart, *phi;
     int *conv, *active, *solved;
     const int* stop;  // graph mode: set once the batch has converged; every later phase launch returns at once
     int* stop_w;
     int* st_pcg;
-    float *st_kres, *st_lsm, *st_step;
+    S *st_kres, *st_lsm, *st_step;
     int* st_sqp;
-    float *st_init, *st_final;
+    S *st_init, *st_final;
 };
+using Args = ArgsT<float>;
 
-template <int PL>
+// the kernels' dynamic LDS as an array of the scalar type (one declaration for every kernel)
+template <typename S>
+__device__ __forceinline__ S* dyn_lds() {
+    extern __shared__ __align__(16) float sm_dyn[];
+    return reinterpret_cast<S*>(sm_dyn);
+}
+
+template <int PL, typename S = float>
 struct View {  // per-problem pointers
     static constexpr int NQ = Plant<PL>::NQ, NX = Plant<PL>::NX, NU = Plant<PL>::NU;
-    float* xu;
-    const float *xs, *ref, *fext, *Qd, *Rd, *xref;
-    float *lam, *A, *Bm, *c, *Qp, *Qv, *Rg, *q, *r, *Qip, *Qiv, *Ri, *Sd, *So, *Pd, *Po, *gam, *dz, *mpart;
-    __device__ View(const Args& a, int b) {
+    S* xu;
+    const S *xs, *ref, *fext, *Qd, *Rd, *xref;
+    S *lam, *A, *Bm, *c, *Qp, *Qv, *Rg, *q, *r, *Qip, *Qiv, *Ri, *Sd, *So, *Pd, *Po, *gam, *dz, *mpart;
+    __device__ View(const ArgsT<S>& a, int b) {
         const int N = a.N;
         xu = a.xu + (size_t)b * a.traj;
         xs = a.xs + (size_t)b * NX;
@@ -84,43 +94,44 @@ struct View {  // per-problem pointers
 };
 
 // graph mode: the iteration's kernels are all enqueued up front; once the batch-level exit fires they return
-__device__ __forceinline__ bool stopped(const Args& a) { return a.stop && *a.stop; }
+template <typename S>
+__device__ __forceinline__ bool stopped(const ArgsT<S>& a) { return a.stop && *a.stop; }
 
 // ------------------------------------------------------------------ phases on one problem
-template <int PL>
-__device__ __forceinline__ void phase_lin(const Args& a, int b, float* sm) {
+template <int PL, typename S = float>
+__device__ __forceinline__ void phase_lin(const ArgsT<S>& a, int b, S* sm) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
-    View<PL> v(a, b);
+    View<PL, S> v(a, b);
     const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
-    float* shg = sm + g * lin_lds_floats<PL>();
-    const float* xt = v.xu + (size_t)(a.cw.terminal_mode ? N - 1 : N - 2) * (NX + NU);
+    S* shg = sm + g * lin_lds_floats<PL>();
+    const S* xt = v.xu + (size_t)(a.cw.terminal_mode ? N - 1 : N - 2) * (NX + NU);
     for (int k = g; k < N; k += G) {
         // the lane index is re-laundered per knot so that lane-only constants of the knot routine are
         // recomputed inside the loop rather than hoisted out of it and held in registers throughout
         const int ln = opaque(lane);
-        const float* xk = v.xu + (size_t)(k < N - 1 ? k : 0) * (NX + NU);
-        lin_knot<PL>(k, N, ln, a.dt, a.cw, xk, xt, v.ref + 6 * k, v.xref ? v.xref + k * NX : nullptr, v.Qd, v.Rd,
+        const S* xk = v.xu + (size_t)(k < N - 1 ? k : 0) * (NX + NU);
+        lin_knot<PL, S>(k, N, ln, a.dt, a.cw, xk, xt, v.ref + 6 * k, v.xref ? v.xref + k * NX : nullptr, v.Qd, v.Rd,
                      v.fext, v.xu, v.xs, v.A + (size_t)k * NX * NX, v.Bm + (size_t)k * NX * NU, v.c + (k + 1) * NX,
                      v.c, v.Qp + k * NQ * NQ, v.Qv + k * NQ, v.Rg + k * NU, v.q + k * NX, v.r + k * NU, shg);
     }
 }
 
-template <int PL>
-__device__ __forceinline__ void phase_schur(const Args& a, int b, float rho, float* sm) {
+template <int PL, typename S = float>
+__device__ __forceinline__ void phase_schur(const ArgsT<S>& a, int b, S rho, S* sm) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
-    View<PL> v(a, b);
+    View<PL, S> v(a, b);
     const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
-    float* tile = sm + g * SchurTile<PL>::SIZE;
-    float* piv = tile + SchurTile<PL>::PIV_OFF;
+    S* tile = sm + g * SchurTile<PL>::SIZE;
+    S* piv = tile + SchurTile<PL>::PIV_OFF;
     for (int k = g; k < N; k += G)
         invert_q_knot<PL>(lane, k < N - 1, rho, v.Qp + k * NQ * NQ, v.Qv + k * NQ, v.Rg + k * NU,
                           v.Qip + k * NQ * NQ, v.Qiv + k * NQ, v.Ri + k * NU, piv);
     __syncthreads();
     for (int k = g; k < N; k += G) {
         if (k < N - 1) {
-            float phi[NX], prow[2 * NX];
+            S phi[NX], prow[2 * NX];
             schur_row<PL>(lane, rho, v.A + (size_t)k * NX * NX, v.Bm + (size_t)k * NX * NU, v.Qip + k * NQ * NQ,
                           v.Qiv + k * NQ, v.Ri + k * NU, v.Qip + (k + 1) * NQ * NQ, v.Qiv + (k + 1) * NQ,
                           v.q + k * NX, v.q + (k + 1) * NX, v.r + k * NU, v.c + (k + 1) * NX,
@@ -133,9 +144,9 @@ __device__ __forceinline__ void phase_schur(const Args& a, int b, float rho, flo
     __syncthreads();
     for (int k = g; k < N - 1; k += G) {
         const int rr = lane < NX ? lane : 0;
-        float phi[NX], prow[NX];
-        const float* So1 = v.So + (size_t)(k + 1) * NX * NX;
-        const float* Pd1 = v.Pd + (size_t)(k + 1) * NX * NX;
+        S phi[NX], prow[NX];
+        const S* So1 = v.So + (size_t)(k + 1) * NX * NX;
+        const S* Pd1 = v.Pd + (size_t)(k + 1) * NX * NX;
 #pragma unroll
         for (int m = 0; m < NX; m++) {
             phi[m] = So1[rr * NX + m];
@@ -146,23 +157,23 @@ __device__ __forceinline__ void phase_schur(const Args& a, int b, float rho, flo
 }
 
 // y = (D, L) block-tridiagonal times the padded LDS vector x at row (k, l)
-template <int NX>
-__device__ __forceinline__ float btd_row(const float* D, const float* L, const float* x, int N, int k, int l) {
+template <int NX, typename S>
+__device__ __forceinline__ S btd_row(const S* D, const S* L, const S* x, int N, int k, int l) {
     constexpr int XP = 16;
-    const float* Dr = D + (size_t)k * NX * NX + l * NX;
-    const float* xk = x + (k + 1) * XP;
-    float s = 0.f;
+    const S* Dr = D + (size_t)k * NX * NX + l * NX;
+    const S* xk = x + (k + 1) * XP;
+    S s = S(0);
 #pragma unroll
     for (int c = 0; c < NX; c++) s += Dr[c] * xk[c];
     if (k > 0) {
-        const float* Lr = L + (size_t)k * NX * NX + l * NX;
-        const float* xm = x + k * XP;
+        const S* Lr = L + (size_t)k * NX * NX + l * NX;
+        const S* xm = x + k * XP;
 #pragma unroll
         for (int c = 0; c < NX; c++) s += Lr[c] * xm[c];
     }
     if (k < N - 1) {
-        const float* Lc = L + (size_t)(k + 1) * NX * NX + l;
-        const float* xp = x + (k + 2) * XP;
+        const S* Lc = L + (size_t)(k + 1) * NX * NX + l;
+        const S* xp = x + (k + 2) * XP;
 #pragma unroll
         for (int c = 0; c < NX; c++) s += Lc[c * NX] * xp[c];
     }
@@ -175,48 +186,48 @@ __device__ void btd_apply(int N, const float* Gp, const float* Dp, const float* 
 // Warm-started PCG on S lambda = gamma (pcg.cuh:13-148). Returns the iteration count. BTD: precondition with the
 // twisted factorization of S (Gp, Dp: the problem's blocks from k_btd_factor, btd_apply) instead of P, for the
 // backward pass; the LDS then also holds z and the sweeps' middle vectors (3 (N + 2) 16 + 48 floats in all).
-template <int PL, int NPASS, bool BTD = false>
-__device__ __forceinline__ int phase_pcg(const Args& a, int b, float eps, float* sm, const float* Gp = nullptr,
-                                         const float* Dp = nullptr) {
+template <int PL, int NPASS, bool BTD = false, typename S = float>
+__device__ __forceinline__ int phase_pcg(const ArgsT<S>& a, int b, S eps, S* sm, const S* Gp = nullptr,
+                                         const S* Dp = nullptr) {
     constexpr int NX = Plant<PL>::NX, XP = 16;
-    View<PL> v(a, b);
+    View<PL, S> v(a, b);
     const int N = a.N, tid = threadIdx.x, g = tid / KG, lane = tid % KG, G = blockDim.x / KG;
-    float* sp = sm;
-    float* sr = sp + (N + 2) * XP;
-    float* red = sr + (N + 2) * XP;
-    float* sz = red + 16;
-    float* pv = sz + (N + 2) * XP;
-    for (int i = tid; i < 2 * (N + 2) * XP; i += blockDim.x) sm[i] = 0.f;
+    S* sp = sm;
+    S* sr = sp + (N + 2) * XP;
+    S* red = sr + (N + 2) * XP;
+    S* sz = red + 16;
+    S* pv = sz + (N + 2) * XP;
+    for (int i = tid; i < 2 * (N + 2) * XP; i += blockDim.x) sm[i] = S(0);
     __syncthreads();
-    float x[NPASS], r[NPASS], z[NPASS], p[NPASS], Ap[NPASS];
+    S x[NPASS], r[NPASS], z[NPASS], p[NPASS], Ap[NPASS];
     bool val[NPASS];
     int kk[NPASS];
 #pragma unroll
     for (int s = 0; s < NPASS; s++) {
         kk[s] = g + s * G;
         val[s] = (lane < NX) && (kk[s] < N);
-        x[s] = val[s] ? v.lam[kk[s] * NX + lane] : 0.f;
+        x[s] = val[s] ? v.lam[kk[s] * NX + lane] : S(0);
         if (val[s]) sp[(kk[s] + 1) * XP + lane] = x[s];
     }
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < NPASS; s++) r[s] = val[s] ? v.gam[kk[s] * NX + lane] - btd_row<NX>(v.Sd, v.So, sp, N, kk[s], lane) : 0.f;
+    for (int s = 0; s < NPASS; s++) r[s] = val[s] ? v.gam[kk[s] * NX + lane] - btd_row<NX, S>(v.Sd, v.So, sp, N, kk[s], lane) : S(0);
 #pragma unroll
     for (int s = 0; s < NPASS; s++)
         if (val[s]) sr[(kk[s] + 1) * XP + lane] = r[s];
     __syncthreads();
     if constexpr (BTD) btd_apply<PL>(N, Gp, Dp, sr, sz, pv);
-    float loc = 0.f;
+    S loc = S(0);
 #pragma unroll
     for (int s = 0; s < NPASS; s++) {
-        if constexpr (BTD) z[s] = val[s] ? sz[(kk[s] + 1) * XP + lane] : 0.f;
-        else z[s] = val[s] ? btd_row<NX>(v.Pd, v.Po, sr, N, kk[s], lane) : 0.f;
+        if constexpr (BTD) z[s] = val[s] ? sz[(kk[s] + 1) * XP + lane] : S(0);
+        else z[s] = val[s] ? btd_row<NX, S>(v.Pd, v.Po, sr, N, kk[s], lane) : S(0);
         p[s] = z[s];
         loc += r[s] * z[s];
     }
-    float rho = block_sum(loc, red);
-    if (fabsf(rho) < a.pcg_abs_tol) return 0;
-    const float rho0 = fabsf(rho);
+    S rho = block_sum(loc, red);
+    if (sabs(rho) < a.pcg_abs_tol) return 0;
+    const S rho0 = sabs(rho);
     int it = 0;
     for (int i = 0; i < a.max_pcg; i++) {
         it++;
@@ -224,13 +235,13 @@ __device__ __forceinline__ int phase_pcg(const Args& a, int b, float eps, float*
         for (int s = 0; s < NPASS; s++)
             if (val[s]) sp[(kk[s] + 1) * XP + lane] = p[s];
         __syncthreads();
-        loc = 0.f;
+        loc = S(0);
 #pragma unroll
         for (int s = 0; s < NPASS; s++) {
-            Ap[s] = val[s] ? btd_row<NX>(v.Sd, v.So, sp, N, kk[s], lane) : 0.f;
+            Ap[s] = val[s] ? btd_row<NX, S>(v.Sd, v.So, sp, N, kk[s], lane) : S(0);
             loc += p[s] * Ap[s];
         }
-        const float alpha = rho / block_sum(loc, red);
+        const S alpha = rho / block_sum(loc, red);
 #pragma unroll
         for (int s = 0; s < NPASS; s++) {
             x[s] += alpha * p[s];
@@ -240,16 +251,16 @@ __device__ __forceinline__ int phase_pcg(const Args& a, int b, float eps, float*
         __syncthreads();
         if (it >= a.max_pcg) break;  // x is final at the cap: its last preconditioner application is not needed
         if constexpr (BTD) btd_apply<PL>(N, Gp, Dp, sr, sz, pv);
-        loc = 0.f;
+        loc = S(0);
 #pragma unroll
         for (int s = 0; s < NPASS; s++) {
-            if constexpr (BTD) z[s] = val[s] ? sz[(kk[s] + 1) * XP + lane] : 0.f;
-            else z[s] = val[s] ? btd_row<NX>(v.Pd, v.Po, sr, N, kk[s], lane) : 0.f;
+            if constexpr (BTD) z[s] = val[s] ? sz[(kk[s] + 1) * XP + lane] : S(0);
+            else z[s] = val[s] ? btd_row<NX, S>(v.Pd, v.Po, sr, N, kk[s], lane) : S(0);
             loc += r[s] * z[s];
         }
-        const float rn = block_sum(loc, red);
-        if (fabsf(rn) < a.pcg_abs_tol + eps * rho0) break;
-        const float beta = rn / rho;
+        const S rn = block_sum(loc, red);
+        if (sabs(rn) < a.pcg_abs_tol + eps * rho0) break;
+        const S beta = rn / rho;
         rho = rn;
 #pragma unroll
         for (int s = 0; s < NPASS; s++) p[s] = z[s] + beta * p[s];
@@ -824,18 +835,18 @@ __device__ __forceinline__ int pcg_dispatch(const Args& a, int b, float eps, flo
     else return phase_pcg<PL, NPASS>(a, b, eps, sm);
 }
 
-template <int PL>
-__device__ __forceinline__ float phase_dz(const Args& a, int b, float* sm) {
+template <int PL, typename S = float>
+__device__ __forceinline__ S phase_dz(const ArgsT<S>& a, int b, S* sm) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
-    View<PL> v(a, b);
+    View<PL, S> v(a, b);
     const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
-    float* shv = sm + g * KG;
-    float* red = sm + G * KG;
-    float res = 0.f;
+    S* shv = sm + g * KG;
+    S* red = sm + G * KG;
+    S res = S(0);
     for (int k = g; k < N; k += G) {
-        const float* lam1 = v.lam + (k < N - 1 ? (k + 1) * NX : 0);
-        res = fmaxf(res, dz_knot<PL>(k, N, lane, v.A + (size_t)k * NX * NX, v.Bm + (size_t)k * NX * NU,
+        const S* lam1 = v.lam + (k < N - 1 ? (k + 1) * NX : 0);
+        res = fmaxf(res, dz_knot<PL, S>(k, N, lane, v.A + (size_t)k * NX * NX, v.Bm + (size_t)k * NX * NU,
                                      v.Qip + k * NQ * NQ, v.Qiv + k * NQ, v.Ri + k * NU, v.q + k * NX, v.r + k * NU,
                                      v.c + k * NX, v.lam + k * NX, lam1, v.dz + (size_t)k * (NX + NU), shv));
     }
@@ -845,23 +856,23 @@ __device__ __forceinline__ float phase_dz(const Args& a, int b, float* sm) {
 // merit at nal step lengths alpha_j = 2^-j (merit.cuh:16-92): per-(alpha, knot) partial terms,
 // then fixed-order sums over knots. out[j] (LDS) receives phi(alpha_j). The reference's
 // initial / final merit (dz = 0, one alpha) is nal = 1.
-template <int PL>
-__device__ __forceinline__ void phase_merit(const Args& a, int b, float mu, int nal, float* out) {
+template <int PL, typename S = float>
+__device__ __forceinline__ void phase_merit(const ArgsT<S>& a, int b, S mu, int nal, S* out) {
     using P = Plant<PL>;
     constexpr int NX = P::NX, NU = P::NU;
-    View<PL> v(a, b);
+    View<PL, S> v(a, b);
     const int N = a.N;
     for (int it = threadIdx.x; it < N * nal; it += blockDim.x) {
         const int k = it % N, j = it / N;
-        const float alpha = 1.f / (float)(1 << j);
+        const S alpha = S(1) / (S)(1 << j);
         const size_t off = (size_t)k * (NX + NU);
-        v.mpart[j * N + k] = merit_item<PL>(k, N, alpha, a.dt, a.cw, mu, v.xu + off, v.dz + off, v.ref + 6 * k,
+        v.mpart[j * N + k] = merit_item<PL, S>(k, N, alpha, a.dt, a.cw, mu, v.xu + off, v.dz + off, v.ref + 6 * k,
                                             v.xref ? v.xref + k * NX : nullptr, v.Qd, v.Rd, v.fext, v.xu, v.dz,
                                             v.xs);
     }
     __syncthreads();
     if (threadIdx.x < (unsigned)nal) {
-        float s = 0.f;
+        S s = S(0);
         for (int k = 0; k < N; k++) s += v.mpart[threadIdx.x * N + k];
         out[threadIdx.x] = s;
     }
@@ -869,26 +880,26 @@ __device__ __forceinline__ void phase_merit(const Args& a, int b, float mu, int 
 }
 
 // line search + rho adaptation + update (line_search.cuh:12-98). merits in LDS; returns via shared.
-template <int PL>
-__device__ __forceinline__ void phase_ls(const Args& a, int b, int it, const float* merits, float* shs) {
+template <int PL, typename S = float>
+__device__ __forceinline__ void phase_ls(const ArgsT<S>& a, int b, int it, const S* merits, S* shs) {
     constexpr int NX = Plant<PL>::NX, NU = Plant<PL>::NU;
-    View<PL> v(a, b);
+    View<PL, S> v(a, b);
     if (threadIdx.x == 0) {
-        float best = 1e38f;
+        S best = S(1e38f);
         int jb = 0;
         for (int j = 0; j < NUM_ALPHAS; j++)
             if (merits[j] < best) { best = merits[j]; jb = j; }
         const bool ok = best < a.phi[b];
         if (a.adapt_rho) {
-            const float mult = ok ? fminf(a.drho[b] / RHO_FACTOR, 1.f / RHO_FACTOR) : fmaxf(a.drho[b] * RHO_FACTOR, RHO_FACTOR);
+            const S mult = ok ? smin(a.drho[b] / S(RHO_FACTOR), S(1) / S(RHO_FACTOR)) : smax(a.drho[b] * S(RHO_FACTOR), S(RHO_FACTOR));
             a.drho[b] = mult;
-            a.rho[b] = fminf(fmaxf(a.rho[b] * mult, RHO_MIN), RHO_MAX);
+            a.rho[b] = smin(smax(a.rho[b] * mult, S(RHO_MIN)), S(RHO_MAX));
         }
-        float step = -1.f;
+        S step = -S(1);
         if (!ok) {
-            if (a.rho[b] > RHO_MAX) a.rho[b] = RHO_INIT;
+            if (a.rho[b] > S(RHO_MAX)) a.rho[b] = S(RHO_INIT);
         } else {
-            step = 1.f / (float)(1 << jb);
+            step = S(1) / (S)(1 << jb);
             a.phi[b] = best;
         }
         a.st_step[it * a.B + b] = step;
@@ -896,8 +907,8 @@ __device__ __forceinline__ void phase_ls(const Args& a, int b, int it, const flo
         shs[0] = step;
     }
     __syncthreads();
-    const float step = shs[0];
-    if (step > 0.f)
+    const S step = shs[0];
+    if (step > S(0))
         for (int i = threadIdx.x; i < a.traj; i += blockDim.x) v.xu[i] += step * v.dz[i];
     (void)NX; (void)NU;
 }
@@ -906,9 +917,9 @@ __device__ __forceinline__ void phase_ls(const Args& a, int b, int it, const flo
 // which: 0 initial merit, 1 final merit, 2 line search, 3 line search preceded by the batch-level exit test
 // (bsqp.cuh:165-167) on the converged count solved[it] that k_pcg_dz accumulated: every work-group makes the
 // same decision from it, and work-group 0 raises the device stop flag for the launches that follow.
-template <int PL>
-__device__ __forceinline__ void merit_body(const Args& a, int nal, int which, int it) {
-    extern __shared__ __align__(16) float sm[];
+template <int PL, typename S = float>
+__device__ __forceinline__ void merit_body(const ArgsT<S>& a, int nal, int which, int it) {
+    S* sm = dyn_lds<S>();
     const int b = blockIdx.x;
     if (which >= 2 && stopped(a)) return;
     if (which == 3 && (double)a.solved[it] >= (double)a.B * (double)a.solve_ratio) {
@@ -916,17 +927,17 @@ __device__ __forceinline__ void merit_body(const Args& a, int nal, int which, in
         return;
     }
     if (which >= 2 && a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
-    phase_merit<PL>(a, b, a.mu[b], nal, sm);
+    phase_merit<PL, S>(a, b, a.mu[b], nal, sm);
     if (threadIdx.x == 0 && which < 2) {
         a.phi[b] = sm[0];
         (which == 0 ? a.st_init : a.st_final)[b] = sm[0];
     }
-    if (which >= 2) phase_ls<PL>(a, b, it, sm, sm + NUM_ALPHAS);
+    if (which >= 2) phase_ls<PL, S>(a, b, it, sm, sm + NUM_ALPHAS);
 }
 
-template <int PL>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_merit(Args a, int nal, int which, int it) {
-    merit_body<PL>(a, nal, which, it);
+template <int PL, typename S = float>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_merit(ArgsT<S> a, int nal, int which, int it) {
+    merit_body<PL, S>(a, nal, which, it);
 }
 
 // The same for N * NUM_ALPHAS <= 256: one thread per (step length, knot) item in a 256-thread work-group,
@@ -939,20 +950,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 
 // Solve-start state of the batch (the device stop flag, the converged counts) and of problem b (flags, SQP
 // count, the per-iteration records) for the solve of bsqp.cuh:111-114, with the records this build keeps.
-__device__ __forceinline__ void begin_batch(const Args& a) {
+template <typename S>
+__device__ __forceinline__ void begin_batch(const ArgsT<S>& a) {
     if (a.stop_w) *a.stop_w = 0;
     for (int i = 0; i < a.max_sqp; i++) a.solved[i] = 0;
 }
 
-__device__ __forceinline__ void begin_problem(const Args& a, int b) {
+template <typename S>
+__device__ __forceinline__ void begin_problem(const ArgsT<S>& a, int b) {
     a.conv[b] = 0;
     a.active[b] = 1;
     a.st_sqp[b] = 0;
     for (int i = 0; i < a.max_sqp; i++) {
         a.st_pcg[i * a.B + b] = 0;
-        a.st_kres[i * a.B + b] = 0.f;
-        a.st_lsm[i * a.B + b] = 0.f;
-        a.st_step[i * a.B + b] = 0.f;
+        a.st_kres[i * a.B + b] = S(0);
+        a.st_lsm[i * a.B + b] = S(0);
+        a.st_step[i * a.B + b] = S(0);
     }
 }
 
@@ -1031,20 +1044,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     }
 }
 
-template <int PL>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_lin(Args a) {
-    extern __shared__ __align__(16) float sm[];
+template <int PL, typename S = float>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_lin(ArgsT<S> a) {
+    S* sm = dyn_lds<S>();
     const int b = blockIdx.x;
     if (stopped(a) || (a.exit_rule == GATO_EXIT_KKT && !a.active[b])) return;
-    phase_lin<PL>(a, b, sm);
+    phase_lin<PL, S>(a, b, sm);
 }
 
-template <int PL>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_schur(Args a) {
-    extern __shared__ __align__(16) float sm[];
+template <int PL, typename S = float>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_schur(ArgsT<S> a) {
+    S* sm = dyn_lds<S>();
     const int b = blockIdx.x;
     if (stopped(a) || (a.exit_rule == GATO_EXIT_KKT && !a.active[b])) return;
-    phase_schur<PL>(a, b, a.rho[b], sm);
+    phase_schur<PL, S>(a, b, a.rho[b], sm);
 }
 
 // The same in 256-thread work-groups (N <= 32: at most two knots per knot group), with room for 256 VGPRs at two
@@ -1057,14 +1070,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     phase_schur<PL>(a, b, a.rho[b], sm);
 }
 
-template <int PL, int NPASS>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg(Args a, int it) {
-    extern __shared__ __align__(16) float sm[];
+template <int PL, int NPASS, typename S = float>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg(ArgsT<S> a, int it) {
+    S* sm = dyn_lds<S>();
     const int b = blockIdx.x;
     if (stopped(a)) return;
     const bool skip = (a.exit_rule == GATO_EXIT_KKT) ? !a.active[b] : a.conv[b];  // quirk Q9 (pcg.cuh:30-33)
     int n = 0;
-    if (!skip) n = pcg_dispatch<PL, NPASS>(a, b, a.tol[b], sm);
+    if (!skip) {
+        if constexpr (std::is_same_v<S, float>) n = pcg_dispatch<PL, NPASS>(a, b, a.tol[b], sm);
+        else n = phase_pcg<PL, NPASS, false, S>(a, b, a.tol[b], sm);  // fp64: the streamed PCG at every shape
+    }
     if (threadIdx.x == 0) a.st_pcg[it * a.B + b] = n;
 }
 
@@ -1118,23 +1134,24 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_btd(Args a, cons
     if (threadIdx.x == 0 && a.st_pcg) a.st_pcg[b] = n;
 }
 
-template <int PL>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_dz(Args a, int it) {
-    extern __shared__ __align__(16) float sm[];
+template <int PL, typename S = float>
+__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_dz(ArgsT<S> a, int it) {
+    S* sm = dyn_lds<S>();
     const int b = blockIdx.x;
     if (stopped(a)) return;
     if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) {
-        if (threadIdx.x == 0) a.st_kres[it * a.B + b] = 0.f;
+        if (threadIdx.x == 0) a.st_kres[it * a.B + b] = S(0);
         return;
     }
-    const float kr = phase_dz<PL>(a, b, sm);
+    const S kr = phase_dz<PL, S>(a, b, sm);
     if (threadIdx.x == 0) a.st_kres[it * a.B + b] = kr;
 }
 
 // Host-side convergence bookkeeping of bsqp.cuh:142-165 for problem b after SQP iteration it, given its PCG
 // iteration count and KKT residual: SQP count, converged and active flags, and the line-search record of a
 // frozen problem. Returns the converged flag.
-__device__ __forceinline__ int check_problem(const Args& a, int b, int it, int npcg, float kres) {
+template <typename S>
+__device__ __forceinline__ int check_problem(const ArgsT<S>& a, int b, int it, int npcg, S kres) {
     int conv = a.conv[b];
     if (a.exit_rule == GATO_EXIT_PCG0) {
         if (npcg == 0) conv = 1;
@@ -1144,7 +1161,7 @@ __device__ __forceinline__ int check_problem(const Args& a, int b, int it, int n
         if (kres <= a.kkt_tol) { conv = 1; a.active[b] = 0; }
     }
     if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) {  // frozen: no line search this iteration
-        a.st_step[it * a.B + b] = 0.f;
+        a.st_step[it * a.B + b] = S(0);
         a.st_lsm[it * a.B + b] = a.phi[b];
     }
     a.conv[b] = conv;
@@ -1192,7 +1209,8 @@ __global__ void __launch_bounds__(1024) GATO_TWO_WG_PER_CU k_pcg_dz_wide(Args a,
 
 // The convergence bookkeeping (check_problem) for every problem of the batch in one work-group, the converged
 // count, and, when a device stop flag is set, the batch-level exit test (bsqp.cuh:165-167) that raises it.
-__global__ void __launch_bounds__(1024) k_check(Args a, int it) {
+template <typename S = float>
+__global__ void __launch_bounds__(1024) k_check(ArgsT<S> a, int it) {
     __shared__ int count;
     if (stopped(a)) return;
     if (threadIdx.x == 0) count = 0;
@@ -1208,17 +1226,20 @@ __global__ void __launch_bounds__(1024) k_check(Args a, int it) {
     }
 }
 
-__global__ void k_begin(Args a) {
+template <typename S = float>
+__global__ void k_begin(ArgsT<S> a) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b == 0) begin_batch(a);
     if (b < a.B) begin_problem(a, b);
 }
 
-__global__ void k_zero(float* p, size_t n) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.f;
+template <typename S = float>
+__global__ void k_zero(S* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = S(0);
 }
 
-__global__ void k_end(Args a) {
+template <typename S = float>
+__global__ void k_end(ArgsT<S> a) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= a.B) return;
     a.drho[b] = a.drho_init[b];  // bsqp.cuh:189
@@ -2044,22 +2065,22 @@ __global__ void __launch_bounds__(256) k_ee_positions(const float* xu, int traj,
     }
 }
 
-template <int PL>
-__global__ void k_sim(float* xn, const float* x, const float* u, const float* fext, float dt, int B) {
+template <int PL, typename S = float>
+__global__ void k_sim(S* xn, const S* x, const S* u, const S* fext, S dt, int B) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
-    float q[NQ], qd[NQ], uu[NQ], qdd[NQ], L[NQ * (NQ + 1) / 2];
+    S q[NQ], qd[NQ], uu[NQ], qdd[NQ], L[NQ * (NQ + 1) / 2];
 #pragma unroll
     for (int i = 0; i < NQ; i++) { q[i] = x[i]; qd[i] = x[NQ + i]; uu[i] = u[i]; }
-    Kin<PL> K;
+    Kin<PL, S> K;
     K.set(q);
     forward_dynamics<PL>(K, qd, uu, fext + 6 * b, qdd, L);
 #pragma unroll
     for (int i = 0; i < NQ; i++) {
         xn[b * 2 * NQ + NQ + i] = qd[i] + dt * qdd[i];
-        xn[b * 2 * NQ + i] = q[i] + dt * qd[i] + 0.5f * qdd[i] * dt * dt;
+        xn[b * 2 * NQ + i] = q[i] + dt * qd[i] + S(0.5) * qdd[i] * dt * dt;
     }
 }
 
@@ -2337,7 +2358,7 @@ void fill_args(gato_solver* s, float* d_xu, const float* d_xs, const float* d_re
     a.dt = dt;
     a.kkt_tol = s->prm.kkt_tol;
     a.solve_ratio = s->prm.solve_ratio;
-    a.pcg_abs_tol = gato::PCG_ABS_TOL;
+    a.pcg_abs_tol = (float)gato::PCG_ABS_TOL;
     a.cw = gato::CostW{s->prm.q_cost, s->prm.N_cost, s->prm.q_lim_cost, s->prm.vel_lim_cost, s->prm.ctrl_lim_cost,
                        s->prm.terminal_mode, s->prm.hess_mode};
     a.xu = d_xu;
@@ -2743,9 +2764,320 @@ int autotune(gato_solver* s, float* d_xu, const float* d_xs, const float* d_ref,
 
 }  // namespace
 
+// =================================================================== FP64 mode
+// The reference's USE_DOUBLES build (gato/settings.h:7-11, bindings.cu:244-252): the same SQP iteration in double
+// precision. It runs the plain path's kernels instantiated at S = double (one kernel per phase, a host loop reading
+// the converged count every iteration, the streamed PCG), with library sin, cos, division and square root.
+struct gato_solver64 {
+    int plant, N, B, nq, nx, nu, traj;
+    gato_params prm;
+    int adapt_rho = 1;
+    int G = 4;
+    char* pool = nullptr;
+    gato::ArgsT<double> a{};
+    double *d_fext, *d_Qd, *d_Rd, *d_xref, *d_rho, *d_drho, *d_drho_init, *d_mu, *d_tol;
+    std::vector<double> h_rho_init, h_drho_init;
+    bool joint_cost = false;
+    std::vector<int> h_pcg, h_sqp, h_conv;
+    std::vector<double> h_kres, h_lsm, h_step, h_init, h_final;
+};
+
+namespace {
+
+// The FP64 solver's workspace (the fp32 solver's forward arrays, in doubles).
+size_t layout64(gato_solver64* s, bool assign) {
+    const size_t N = s->N, B = s->B, nx = s->nx, nu = s->nu, nq = s->nq;
+    size_t off = 0;
+    auto D = [&](size_t n) -> double* {
+        off = (off + 255) & ~size_t(255);
+        double* p = reinterpret_cast<double*>(s->pool + off);
+        off += n * sizeof(double);
+        return assign ? p : nullptr;
+    };
+    auto I = [&](size_t n) -> int* {
+        off = (off + 255) & ~size_t(255);
+        int* p = reinterpret_cast<int*>(s->pool + off);
+        off += n * sizeof(int);
+        return assign ? p : nullptr;
+    };
+    gato::ArgsT<double>& a = s->a;
+    const size_t it = std::max<uint32_t>(1u, s->prm.max_sqp_iters);
+    s->d_fext = D(B * 6); s->d_Qd = D(B * nx); s->d_Rd = D(B * nu); s->d_xref = D(B * N * nx);
+    a.lam = D(B * N * nx);
+    s->d_rho = D(B); s->d_drho = D(B); s->d_drho_init = D(B); s->d_mu = D(B); s->d_tol = D(B);
+    a.A = D(B * N * nx * nx); a.Bm = D(B * N * nx * nu); a.c = D(B * N * nx);
+    a.Qp = D(B * N * nq * nq); a.Qv = D(B * N * nq); a.Rg = D(B * N * nu); a.q = D(B * N * nx); a.r = D(B * N * nu);
+    a.Qip = D(B * N * nq * nq); a.Qiv = D(B * N * nq); a.Ri = D(B * N * nu);
+    a.Sd = D(B * N * nx * nx); a.So = D(B * N * nx * nx); a.Pd = D(B * N * nx * nx); a.Po = D(B * N * nx * nx);
+    a.gam = D(B * N * nx); a.dz = D(B * (s->traj + nu)); a.mpart = D(B * gato::NUM_ALPHAS * N); a.phi = D(B);
+    a.conv = I(B); a.active = I(B); a.solved = I(std::max<size_t>(B, s->prm.max_sqp_iters));
+    a.st_pcg = I(it * B); a.st_kres = D(it * B); a.st_lsm = D(it * B); a.st_step = D(it * B); a.st_sqp = I(B);
+    a.st_init = D(B); a.st_final = D(B);
+    return off + 256;
+}
+
+// knot groups per work-group: the fewest (4, 8, 16) whose knots per group fit the PCG's passes (at most 8)
+int groups64(int N) { return N <= 32 ? 4 : N <= 64 ? 8 : 16; }
+
+int upload_defaults64(gato_solver64* s) {
+    std::vector<double> Qd((size_t)s->B * s->nx), Rd((size_t)s->B * s->nu);
+    for (int b = 0; b < s->B; b++) {
+        for (int i = 0; i < s->nx; i++) Qd[(size_t)b * s->nx + i] = i < s->nq ? 0.0 : (double)s->prm.qd_cost;
+        for (int i = 0; i < s->nu; i++) Rd[(size_t)b * s->nu + i] = (double)s->prm.u_cost;
+    }
+    HIP_TRY(hipMemcpy(s->d_Qd, Qd.data(), Qd.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->d_Rd, Rd.data(), Rd.size() * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <int PL, int NP>
+void launch_pcg64(int B, dim3 blk, size_t lds, hipStream_t st, const gato::ArgsT<double>& a, int it) {
+    hipLaunchKernelGGL((gato::k_pcg<PL, NP, double>), dim3(B), blk, lds, st, a, it);
+}
+
+// bsqp.cuh:103-197 in double: the plain path's host loop
+template <int PL>
+int launch_solve64(gato_solver64* s, hipStream_t st) {
+    const int B = s->B, N = s->N;
+    const dim3 blk(gato::KG * s->G);
+    const int np = (N + s->G - 1) / s->G;
+    const size_t tile = s->plant == GATO_PLANT_IIWA14 ? gato::SchurTile<0>::SIZE : gato::SchurTile<1>::SIZE;
+    size_t f = 0;
+    f = std::max<size_t>(f, (size_t)s->G * gato::lin_lds_floats<PL>());
+    f = std::max<size_t>(f, (size_t)s->G * tile);
+    f = std::max<size_t>(f, (size_t)2 * (N + 2) * 16 + 64);
+    f = std::max<size_t>(f, (size_t)s->G * gato::KG + 16);
+    f = std::max<size_t>(f, (size_t)2 * gato::NUM_ALPHAS + 16);
+    const size_t lds = f * sizeof(double);
+    gato::ArgsT<double>& a = s->a;
+    const int sb = (B + 255) / 256;
+    const size_t nz = (size_t)B * (s->traj + s->nu);
+    hipLaunchKernelGGL(gato::k_begin<double>, dim3(sb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(gato::k_zero<double>, dim3(256), dim3(256), 0, st, a.dz, nz);
+    hipLaunchKernelGGL((gato::k_merit<PL, double>), dim3(B), blk, lds, st, a, 1, 0, 0);
+    HIP_TRY(hipGetLastError());
+    for (int it = 0; it < (int)s->prm.max_sqp_iters; it++) {
+        hipLaunchKernelGGL((gato::k_lin<PL, double>), dim3(B), blk, lds, st, a);
+        hipLaunchKernelGGL((gato::k_schur<PL, double>), dim3(B), blk, lds, st, a);
+        if (np == 1) launch_pcg64<PL, 1>(B, blk, lds, st, a, it);
+        else if (np == 2) launch_pcg64<PL, 2>(B, blk, lds, st, a, it);
+        else if (np == 4) launch_pcg64<PL, 4>(B, blk, lds, st, a, it);
+        else launch_pcg64<PL, 8>(B, blk, lds, st, a, it);
+        hipLaunchKernelGGL((gato::k_dz<PL, double>), dim3(B), blk, lds, st, a, it);
+        hipLaunchKernelGGL(gato::k_check<double>, dim3(1), dim3(check_threads(B)), 0, st, a, it);
+        HIP_TRY(hipGetLastError());
+        int solved = 0;
+        HIP_TRY(hipMemcpyAsync(&solved, a.solved + it, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if ((double)solved >= (double)B * s->prm.solve_ratio) break;
+        hipLaunchKernelGGL((gato::k_merit<PL, double>), dim3(B), blk, lds, st, a, gato::NUM_ALPHAS, 2, it);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(gato::k_zero<double>, dim3(256), dim3(256), 0, st, a.dz, nz);
+    hipLaunchKernelGGL((gato::k_merit<PL, double>), dim3(B), blk, lds, st, a, 1, 1, 0);
+    hipLaunchKernelGGL(gato::k_end<double>, dim3(sb), dim3(256), 0, st, a);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* gato_last_error(void) { return g_err.c_str(); }
+
+int gato_create_f64(gato_solver64** out, int32_t plant, int32_t knots, int32_t batch, const gato_params* params) {
+    if (!out || !params) return fail(GATO_E_ARG, "null argument");
+    const int nq = plant_nq(plant);
+    if (nq < 0) return fail(GATO_E_ARG, "unknown plant");
+    if (knots < 2 || knots > 128) return fail(GATO_E_ARG, "knots must be in [2, 128]");
+    if (batch < 1) return fail(GATO_E_ARG, "batch must be >= 1");
+    if (params->max_sqp_iters < 1 || params->max_sqp_iters > 1000) return fail(GATO_E_ARG, "max_sqp_iters must be in [1, 1000]");
+    if (params->exit_rule != GATO_EXIT_PCG0 && params->exit_rule != GATO_EXIT_KKT) return fail(GATO_E_ARG, "unknown exit rule");
+    gato_solver64* s = new gato_solver64();
+    s->plant = plant; s->N = knots; s->B = batch; s->nq = nq; s->nx = 2 * nq; s->nu = nq;
+    s->traj = (s->nx + s->nu) * knots - s->nu;
+    s->prm = *params;
+    s->prm.mode = GATO_MODE_PLAIN;
+    s->G = groups64(knots);
+    const size_t bytes = layout64(s, false);
+    hipError_t e = hipMalloc(&s->pool, bytes);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(GATO_E_ALLOC, std::string("hipMalloc workspace: ") + hipGetErrorString(e));
+    }
+    layout64(s, true);
+    auto cleanup = [&](int code) { gato_destroy_f64(s); return code; };
+    if ((e = hipMemset(s->pool, 0, bytes)) != hipSuccess) return cleanup(fail((int)e, "hipMemset workspace"));
+    s->h_rho_init.assign(batch, (double)params->rho);
+    s->h_drho_init.assign(batch, 1.0);
+    std::vector<double> mu(batch, (double)params->mu), tol(batch, (double)params->pcg_tol);
+    if ((e = hipMemcpy(s->d_rho, s->h_rho_init.data(), batch * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(s->d_drho, s->h_drho_init.data(), batch * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(s->d_drho_init, s->h_drho_init.data(), batch * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(s->d_mu, mu.data(), batch * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(s->d_tol, tol.data(), batch * 8, hipMemcpyHostToDevice)) != hipSuccess)
+        return cleanup(fail((int)e, "hipMemcpy initial state"));
+    int rc = upload_defaults64(s);
+    if (rc) return cleanup(rc);
+    const size_t it = params->max_sqp_iters;
+    s->h_pcg.resize(it * batch); s->h_kres.resize(it * batch); s->h_lsm.resize(it * batch); s->h_step.resize(it * batch);
+    s->h_sqp.resize(batch); s->h_conv.resize(batch); s->h_init.resize(batch); s->h_final.resize(batch);
+    *out = s;
+    return 0;
+}
+
+int gato_destroy_f64(gato_solver64* s) {
+    if (!s) return 0;
+    if (s->pool) hipFree(s->pool);
+    delete s;
+    return 0;
+}
+
+int gato_solve_f64(gato_solver64* s, double* d_xu, const double* d_xs, const double* d_ref, double dt, void* stream,
+                   gato_stats* stats) {
+    if (!s || !d_xu || !d_xs || !d_ref) return fail(GATO_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    gato::ArgsT<double>& a = s->a;
+    a.N = s->N; a.B = s->B; a.traj = s->traj; a.max_sqp = s->prm.max_sqp_iters; a.max_pcg = s->prm.max_pcg_iters;
+    a.exit_rule = s->prm.exit_rule; a.adapt_rho = s->adapt_rho; a.dt = dt; a.kkt_tol = s->prm.kkt_tol;
+    a.solve_ratio = s->prm.solve_ratio; a.pcg_abs_tol = gato::PCG_ABS_TOL;
+    a.cw = gato::CostW{s->prm.q_cost, s->prm.N_cost, s->prm.q_lim_cost, s->prm.vel_lim_cost, s->prm.ctrl_lim_cost,
+                       s->prm.terminal_mode, s->prm.hess_mode};
+    a.xu = d_xu; a.xs = d_xs; a.ref = d_ref; a.fext = s->d_fext; a.Qd = s->d_Qd; a.Rd = s->d_Rd;
+    a.xref = s->joint_cost ? s->d_xref : nullptr;
+    a.rho = s->d_rho; a.drho = s->d_drho; a.drho_init = s->d_drho_init; a.mu = s->d_mu; a.tol = s->d_tol;
+    a.stop = nullptr; a.stop_w = nullptr;
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    const int rc = s->plant == GATO_PLANT_IIWA14 ? launch_solve64<0>(s, st) : launch_solve64<1>(s, st);
+    if (rc) return rc;
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    if (!stats) return 0;
+    const int B = s->B;
+    const size_t it = s->prm.max_sqp_iters;
+    HIP_TRY(hipMemcpyAsync(s->h_pcg.data(), a.st_pcg, it * B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_kres.data(), a.st_kres, it * B * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_lsm.data(), a.st_lsm, it * B * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_step.data(), a.st_step, it * B * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_sqp.data(), a.st_sqp, B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_conv.data(), a.conv, B * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_init.data(), a.st_init, B * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s->h_final.data(), a.st_final, B * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    stats->solve_time_us = std::chrono::duration<double, std::micro>(t1 - t0).count();
+    int n_iters = 0, ls_iters = 0;
+    for (int b = 0; b < B; b++) n_iters = std::max(n_iters, s->h_sqp[b]);
+    for (size_t i = 0; i < it; i++)
+        for (int b = 0; b < B; b++)
+            if (s->h_step[i * B + b] != 0.0) ls_iters = std::max<int>(ls_iters, (int)i + 1);
+    stats->n_iters = n_iters;
+    stats->ls_num_iters = ls_iters;
+    auto cpi = [&](int32_t* dst, const std::vector<int>& v) { if (dst) std::memcpy(dst, v.data(), v.size() * 4); };
+    auto cpf = [&](float* dst, const std::vector<double>& v) {
+        if (dst) for (size_t i = 0; i < v.size(); i++) dst[i] = (float)v[i];
+    };
+    cpi(stats->sqp_iters, s->h_sqp); cpi(stats->kkt_converged, s->h_conv); cpi(stats->pcg_iters, s->h_pcg);
+    cpf(stats->initial_merit, s->h_init); cpf(stats->final_merit, s->h_final); cpf(stats->ls_min_merit, s->h_lsm);
+    cpf(stats->ls_step_size, s->h_step); cpf(stats->kkt_residual, s->h_kres);
+    return 0;
+}
+
+int gato_last_merits_f64(gato_solver64* s, double* h_initial, double* h_final) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    if (h_initial) std::memcpy(h_initial, s->h_init.data(), s->B * 8);
+    if (h_final) std::memcpy(h_final, s->h_final.data(), s->B * 8);
+    return 0;
+}
+
+int gato_set_joint_cost_f64(gato_solver64* s, const double* d_Qd, const double* d_Rd, const double* d_xref,
+                            void* stream) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    hipStream_t st = (hipStream_t)stream;
+    if (!d_Qd && !d_Rd && !d_xref) {
+        s->joint_cost = false;
+        return upload_defaults64(s);
+    }
+    if (d_Qd) HIP_TRY(hipMemcpyAsync(s->d_Qd, d_Qd, (size_t)s->B * s->nx * 8, hipMemcpyDeviceToDevice, st));
+    if (d_Rd) HIP_TRY(hipMemcpyAsync(s->d_Rd, d_Rd, (size_t)s->B * s->nu * 8, hipMemcpyDeviceToDevice, st));
+    if (d_xref) {
+        HIP_TRY(hipMemcpyAsync(s->d_xref, d_xref, (size_t)s->B * s->N * s->nx * 8, hipMemcpyDeviceToDevice, st));
+        s->joint_cost = true;
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+int gato_set_f_ext_batch_f64(gato_solver64* s, const double* d_f_ext, void* stream) {
+    if (!s || !d_f_ext) return fail(GATO_E_ARG, "null argument");
+    HIP_TRY(hipMemcpyAsync(s->d_fext, d_f_ext, (size_t)s->B * 6 * 8, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}
+
+int gato_set_batch_f64(gato_solver64* s, int32_t what, const double* h_values, int32_t set_as_reset_default) {
+    if (!s || !h_values) return fail(GATO_E_ARG, "null argument");
+    const size_t n = (size_t)s->B * 8;
+    switch (what) {
+        case GATO_SET_RHO:
+            if (set_as_reset_default) std::memcpy(s->h_rho_init.data(), h_values, n);
+            HIP_TRY(hipMemcpy(s->d_rho, h_values, n, hipMemcpyHostToDevice));
+            return 0;
+        case GATO_SET_DRHO:
+            if (set_as_reset_default) {
+                std::memcpy(s->h_drho_init.data(), h_values, n);
+                HIP_TRY(hipMemcpy(s->d_drho_init, h_values, n, hipMemcpyHostToDevice));
+            }
+            HIP_TRY(hipMemcpy(s->d_drho, h_values, n, hipMemcpyHostToDevice));
+            return 0;
+        case GATO_SET_MU: HIP_TRY(hipMemcpy(s->d_mu, h_values, n, hipMemcpyHostToDevice)); return 0;
+        case GATO_SET_PCG_TOL: HIP_TRY(hipMemcpy(s->d_tol, h_values, n, hipMemcpyHostToDevice)); return 0;
+        default: return fail(GATO_E_ARG, "unknown batch setting");
+    }
+}
+
+int gato_reset_f64(gato_solver64* s, int32_t what) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    if (what == GATO_RESET_DUAL) {
+        HIP_TRY(hipMemset(s->a.lam, 0, (size_t)s->B * s->N * s->nx * 8));
+        return 0;
+    }
+    if (what == GATO_RESET_RHO) {
+        HIP_TRY(hipMemcpy(s->d_rho, s->h_rho_init.data(), s->B * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->d_drho, s->h_drho_init.data(), s->B * 8, hipMemcpyHostToDevice));
+        return 0;
+    }
+    return fail(GATO_E_ARG, "unknown reset");
+}
+
+int gato_set_rho_adaptation_f64(gato_solver64* s, int32_t enabled) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    s->adapt_rho = enabled ? 1 : 0;
+    return 0;
+}
+
+int gato_get_state_f64(gato_solver64* s, double* d_lambda, double* h_rho, double* h_drho) {
+    if (!s) return fail(GATO_E_ARG, "null solver");
+    if (d_lambda) HIP_TRY(hipMemcpy(d_lambda, s->a.lam, (size_t)s->B * s->N * s->nx * 8, hipMemcpyDeviceToDevice));
+    if (h_rho) HIP_TRY(hipMemcpy(h_rho, s->d_rho, s->B * 8, hipMemcpyDeviceToHost));
+    if (h_drho) HIP_TRY(hipMemcpy(h_drho, s->d_drho, s->B * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int gato_sim_forward_f64(gato_solver64* s, double* d_xnext, const double* d_x, const double* d_u, double dt,
+                         void* stream) {
+    if (!s || !d_xnext || !d_x || !d_u) return fail(GATO_E_ARG, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = (s->B + 63) / 64;
+    if (s->plant == GATO_PLANT_IIWA14)
+        hipLaunchKernelGGL((gato::k_sim<0, double>), dim3(nb), dim3(64), 0, st, d_xnext, d_x, d_u, s->d_fext, dt, s->B);
+    else
+        hipLaunchKernelGGL((gato::k_sim<1, double>), dim3(nb), dim3(64), 0, st, d_xnext, d_x, d_u, s->d_fext, dt, s->B);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
 
 int gato_plant_dims(int32_t plant, int32_t* nq, int32_t* nx) {
     const int n = plant_nq(plant);

@@ -20,8 +20,9 @@ namespace gato {
 
 constexpr int KG = 16;       // lanes per knot group
 constexpr int NUM_ALPHAS = 8;  // settings.h:15
-constexpr float RHO_INIT = 1e-3f, RHO_FACTOR = 1.2f, RHO_MIN = 1e-8f, RHO_MAX = 10.f;  // settings.h:18-22
-constexpr float PCG_ABS_TOL = 1e-6f;  // pcg.cuh:27
+// exact in fp64; every use converts to the scalar type (fp32: the float constants)
+constexpr double RHO_INIT = 1e-3, RHO_FACTOR = 1.2, RHO_MIN = 1e-8, RHO_MAX = 10.0;  // settings.h:18-22
+constexpr double PCG_ABS_TOL = 1e-6;  // pcg.cuh:27
 
 struct CostW {
     float q_cost, N_cost, q_lim, vel_lim, ctrl_lim;
@@ -36,13 +37,23 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// max and |x| in the scalar type (fmaxf / fabsf for fp32)
+__device__ __forceinline__ float smax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double smax(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float smin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double smin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float sabs(float a) { return fabsf(a); }
+__device__ __forceinline__ double sabs(double a) { return fabs(a); }
+
 // ---------------------------------------------------------------- barrier terms
 // -log(max(v - lo, 1e-10)) - log(max(hi - v, 1e-10))  (iiwa14_plant.cuh:104-111)
-__device__ __forceinline__ float barrier(float v, float lo, float hi) {
-    float a = v - lo, b = hi - v;
-    a = a <= 1e-10f ? 1e-10f : a;
-    b = b <= 1e-10f ? 1e-10f : b;
-    return -logf(a) - logf(b);
+template <typename S = float>
+__device__ __forceinline__ S barrier(S v, S lo, S hi) {
+    S a = v - lo, b = hi - v;
+    a = a <= S(1e-10f) ? S(1e-10f) : a;
+    b = b <= S(1e-10f) ? S(1e-10f) : b;
+    if constexpr (std::is_same_v<S, float>) return -logf(a) - logf(b);
+    else return -log(a) - log(b);
 }
 // gradient: iiwa14 sign-preserving clamp (iiwa14_plant.cuh:114-136), indy7 plain clamp (indy7_plant.cuh:141-148)
 template <int PL, typename S = float>
@@ -237,13 +248,13 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const Cos
 // In-place Gauss-Jordan inverse without pivoting of an n x n matrix held one row per lane
 // (lane r < n owns row r in registers); the inverse overwrites the rows. Equivalent in exact
 // arithmetic to the reference's elimination on [A | I] (linalg.cuh:365-519). piv: n floats of LDS.
-template <int NX>
-__device__ __forceinline__ void gj_inplace(int lane, float* row, float* piv) {
+template <int NX, typename S>
+__device__ __forceinline__ void gj_inplace(int lane, S* row, S* piv) {
 #pragma unroll
     for (int p = 0; p < NX; p++) {
         if (lane == p) {
-            const float inv = __builtin_amdgcn_rcpf(row[p]);  // fast-math reciprocal of the pivot
-            row[p] = 1.f;
+            const S inv = rcp(row[p]);  // fast-math reciprocal of the pivot
+            row[p] = S(1);
 #pragma unroll
             for (int c = 0; c < NX; c++) {
                 row[c] *= inv;
@@ -252,8 +263,8 @@ __device__ __forceinline__ void gj_inplace(int lane, float* row, float* piv) {
         }
         wave_sync();
         if (lane < NX && lane != p) {
-            const float f = row[p];
-            row[p] = 0.f;
+            const S f = row[p];
+            row[p] = S(0);
 #pragma unroll
             for (int c = 0; c < NX; c++) row[c] -= f * piv[c];
         }
@@ -263,33 +274,33 @@ __device__ __forceinline__ void gj_inplace(int lane, float* row, float* piv) {
 
 // ---------------------------------------------------------------- Schur step 1: Q~_k^-1, R_k^-1
 // Gauss-Jordan without pivoting on the nq x nq position block (linalg.cuh:457-519), lane l owns row l.
-template <int PL>
-__device__ __forceinline__ void invert_q_knot(int lane, bool has_r, float rho, const float* Qp, const float* Qv,
-                                              const float* Rg, float* Qip, float* Qiv, float* Ri, float* piv) {
+template <int PL, typename S>
+__device__ __forceinline__ void invert_q_knot(int lane, bool has_r, S rho, const S* Qp, const S* Qv,
+                                              const S* Rg, S* Qip, S* Qiv, S* Ri, S* piv) {
     constexpr int NQ = Plant<PL>::NQ;
-    float row[NQ];
+    S row[NQ];
 #pragma unroll
-    for (int c = 0; c < NQ; c++) row[c] = lane < NQ ? Qp[lane * NQ + c] + (c == lane ? rho : 0.f) : 0.f;
+    for (int c = 0; c < NQ; c++) row[c] = lane < NQ ? Qp[lane * NQ + c] + (c == lane ? rho : S(0)) : S(0);
     gj_inplace<NQ>(lane, row, piv);
     if (lane < NQ) {
 #pragma unroll
         for (int c = 0; c < NQ; c++) Qip[lane * NQ + c] = row[c];
-        Qiv[lane] = __builtin_amdgcn_rcpf(Qv[lane]);
-        if (has_r) Ri[lane] = __builtin_amdgcn_rcpf(Rg[lane]);
+        Qiv[lane] = rcp(Qv[lane]);
+        if (has_r) Ri[lane] = rcp(Rg[lane]);
     }
 }
 
 // Row r of the block-diagonal Q~^-1 = blkdiag(Qip, diag(Qiv)), loaded without branches on r.
-template <int PL>
-__device__ __forceinline__ void qinv_row(const float* Qip, const float* Qiv, int r, float* out) {
+template <int PL, typename S>
+__device__ __forceinline__ void qinv_row(const S* Qip, const S* Qiv, int r, S* out) {
     constexpr int NQ = Plant<PL>::NQ;
     const int rq = r < NQ ? r : 0;
 #pragma unroll
     for (int m = 0; m < NQ; m++) {
-        const float a = Qip[rq * NQ + m];
-        const float d = Qiv[m];
-        out[m] = r < NQ ? a : 0.f;
-        out[NQ + m] = (NQ + m == r) ? d : 0.f;
+        const S a = Qip[rq * NQ + m];
+        const S d = Qiv[m];
+        out[m] = r < NQ ? a : S(0);
+        out[NQ + m] = (NQ + m == r) ? d : S(0);
     }
 }
 
@@ -307,19 +318,19 @@ struct SchurTile {
     static constexpr int SIZE = PIV_OFF + 2 * NX;            // floats per knot group
 };
 
-template <int PL>
-__device__ __forceinline__ void schur_row(int lane, float rho, const float* A, const float* Bm, const float* Qip0,
-                                          const float* Qiv0, const float* Ri0, const float* Qip1, const float* Qiv1,
-                                          const float* q0, const float* q1, const float* r0, const float* c1,
-                                          float* Sd1, float* So1, float* Pd1, float* g1, float* tile, float* phi,
-                                          float* prow) {
+template <int PL, typename S>
+__device__ __forceinline__ void schur_row(int lane, S rho, const S* A, const S* Bm, const S* Qip0,
+                                          const S* Qiv0, const S* Ri0, const S* Qip1, const S* Qiv1,
+                                          const S* q0, const S* q1, const S* r0, const S* c1,
+                                          S* Sd1, S* So1, S* Pd1, S* g1, S* tile, S* phi,
+                                          S* prow) {
     using P = Plant<PL>;
     using T = SchurTile<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
-    float* As = tile + T::A_OFF;
-    float* Bs = tile + T::B_OFF;
-    float* Qs = tile + T::Q_OFF;
-    float* piv = tile + T::PIV_OFF;
+    S* As = tile + T::A_OFF;
+    S* Bs = tile + T::B_OFF;
+    S* Qs = tile + T::Q_OFF;
+    S* piv = tile + T::PIV_OFF;
     const int r = lane < NX ? lane : 0;
     if (lane < NX) {
 #pragma unroll
@@ -334,22 +345,22 @@ __device__ __forceinline__ void schur_row(int lane, float rho, const float* A, c
     wave_sync();
 #pragma unroll
     for (int c = 0; c < NQ; c++) {
-        float s = 0.f;
+        S s = S(0);
 #pragma unroll
         for (int l = 0; l < NQ; l++) s += As[r * T::AS + l] * Qs[l * T::QS + c];
         phi[c] = s;
         phi[NQ + c] = As[r * T::AS + NQ + c] * Qiv0[c];
     }
-    float brr[NU];
+    S brr[NU];
 #pragma unroll
     for (int c = 0; c < NU; c++) brr[c] = Bs[r * T::BS + c] * Ri0[c];
-    float th[NX];
-    qinv_row<PL>(Qip1, Qiv1, r, th);
-    float g = c1[r];
+    S th[NX];
+    qinv_row<PL, S>(Qip1, Qiv1, r, th);
+    S g = c1[r];
 #pragma unroll
     for (int m = 0; m < NX; m++) {
         asm volatile("" ::: "memory");  // one row of A and B at a time: their LDS loads are not hoisted
-        float s = th[m];
+        S s = th[m];
         g -= s * q1[m];
 #pragma unroll
         for (int l = 0; l < NX; l++) s += phi[l] * As[m * T::AS + l];
@@ -369,7 +380,7 @@ __device__ __forceinline__ void schur_row(int lane, float rho, const float* A, c
         }
     }
 #pragma unroll
-    for (int m = 0; m < NX; m++) prow[m] = th[m] + ((m == r && m < NQ) ? rho : 0.f);
+    for (int m = 0; m < NX; m++) prow[m] = th[m] + ((m == r && m < NQ) ? rho : S(0));
     gj_inplace<NX>(lane, prow, piv);
 #pragma unroll
     for (int m = 0; m < NX; m++) prow[m] = -prow[m];
@@ -381,24 +392,24 @@ __device__ __forceinline__ void schur_row(int lane, float rho, const float* A, c
 }
 
 // Block row 0: S_00 = -Q~_0^-1, P_00 = -Q~_0, gamma_0 = c_0 - Q~_0^-1 q_0 (schur_linsys.cuh:164-210)
-template <int PL>
-__device__ __forceinline__ void schur_row0(int lane, float rho, const float* Qp0, const float* Qv0, const float* Qip0,
-                                           const float* Qiv0, const float* q0, const float* c0, float* Sd0,
-                                           float* Pd0, float* g0) {
+template <int PL, typename S>
+__device__ __forceinline__ void schur_row0(int lane, S rho, const S* Qp0, const S* Qv0, const S* Qip0,
+                                           const S* Qiv0, const S* q0, const S* c0, S* Sd0,
+                                           S* Pd0, S* g0) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX;
     if (lane >= NX) return;
     const int r = lane;
-    float g = c0[r];
-    float qir[NX];
-    qinv_row<PL>(Qip0, Qiv0, r, qir);
+    S g = c0[r];
+    S qir[NX];
+    qinv_row<PL, S>(Qip0, Qiv0, r, qir);
 #pragma unroll
     for (int m = 0; m < NX; m++) {
-        const float qi = qir[m];
+        const S qi = qir[m];
         g -= qi * q0[m];
         Sd0[r * NX + m] = -qi;
-        float qt = 0.f;
-        if (r < NQ && m < NQ) qt = Qp0[r * NQ + m] + (r == m ? rho : 0.f);
+        S qt = S(0);
+        if (r < NQ && m < NQ) qt = Qp0[r * NQ + m] + (r == m ? rho : S(0));
         else if (r >= NQ && m == r) qt = Qv0[r - NQ];
         Pd0[r * NX + m] = -qt;
     }
@@ -407,9 +418,9 @@ __device__ __forceinline__ void schur_row0(int lane, float rho, const float* Qp0
 
 // ---------------------------------------------------------------- Schur step 3: P_{k+1,k} = -P_{k+1,k+1} phi_k P_{k,k}
 // P_{k,k} is staged in the knot group's tile; the product phi_k P_{k,k} then replaces it there.
-template <int PL>
-__device__ __forceinline__ void precond_offdiag(int lane, const float* phi, const float* prow, const float* Pd0,
-                                                float* tile, float* Po1) {
+template <int PL, typename S>
+__device__ __forceinline__ void precond_offdiag(int lane, const S* phi, const S* prow, const S* Pd0,
+                                                S* tile, S* Po1) {
     constexpr int NX = Plant<PL>::NX, AS = SchurTile<PL>::AS;
     const int r = lane < NX ? lane : 0;
     if (lane < NX) {
@@ -417,11 +428,11 @@ __device__ __forceinline__ void precond_offdiag(int lane, const float* phi, cons
         for (int c = 0; c < NX; c++) tile[r * AS + c] = Pd0[r * NX + c];
     }
     wave_sync();
-    float t[NX];
+    S t[NX];
 #pragma unroll
     for (int c = 0; c < NX; c++) {
         asm volatile("" ::: "memory");
-        float s = 0.f;
+        S s = S(0);
 #pragma unroll
         for (int m = 0; m < NX; m++) s += phi[m] * tile[m * AS + c];
         t[c] = opaque(s);  // computed here, not sunk into the lane-predicated store below
@@ -435,7 +446,7 @@ __device__ __forceinline__ void precond_offdiag(int lane, const float* phi, cons
     if (lane < NX) {
 #pragma unroll
         for (int c = 0; c < NX; c++) {
-            float s = 0.f;
+            S s = S(0);
 #pragma unroll
             for (int m = 0; m < NX; m++) s += prow[m] * tile[m * AS + c];
             Po1[lane * NX + c] = -s;
@@ -448,29 +459,29 @@ __device__ __forceinline__ void precond_offdiag(int lane, const float* phi, cons
 // v = q_k - lambda_k + A_k^T lambda_{k+1}; dx_k = -Q~_k^-1 v;  w = r_k + B_k^T lambda_{k+1}; du_k = -R_k^-1 w.
 // Returns the lane's max |stationarity| and |defect| entries (the KKT residual of the problem is
 // the max over knots and lanes).
-template <int PL>
-__device__ __forceinline__ float dz_knot(int k, int N, int lane, const float* A, const float* Bm, const float* Qip,
-                                         const float* Qiv, const float* Ri, const float* q, const float* rr,
-                                         const float* c, const float* lam0, const float* lam1, float* dzk,
-                                         float* shv) {
+template <int PL, typename S>
+__device__ __forceinline__ S dz_knot(int k, int N, int lane, const S* A, const S* Bm, const S* Qip,
+                                         const S* Qiv, const S* Ri, const S* q, const S* rr,
+                                         const S* c, const S* lam0, const S* lam1, S* dzk,
+                                         S* shv) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
     const bool last = (k == N - 1);
-    float res = 0.f;
+    S res = S(0);
     if (lane < NX) {
-        float v = q[lane] - lam0[lane];
+        S v = q[lane] - lam0[lane];
         if (!last) {
 #pragma unroll
             for (int i = 0; i < NX; i++) v += A[i * NX + lane] * lam1[i];
         }
         shv[lane] = v;
-        res = fmaxf(fabsf(v), fabsf(c[lane]));
+        res = smax(sabs(v), sabs(c[lane]));
     }
     wave_sync();
     if (lane < NX) {
-        float s;
+        S s;
         if (lane < NQ) {
-            s = 0.f;
+            s = S(0);
 #pragma unroll
             for (int l = 0; l < NQ; l++) s += Qip[lane * NQ + l] * shv[l];
         } else {
@@ -479,11 +490,11 @@ __device__ __forceinline__ float dz_knot(int k, int N, int lane, const float* A,
         dzk[lane] = -s;
     }
     if (!last && lane < NU) {
-        float w = rr[lane];
+        S w = rr[lane];
 #pragma unroll
         for (int i = 0; i < NX; i++) w += Bm[i * NU + lane] * lam1[i];
         dzk[NX + lane] = -Ri[lane] * w;
-        res = fmaxf(res, fabsf(w));
+        res = smax(res, sabs(w));
     }
     wave_sync();
     return res;
@@ -491,59 +502,59 @@ __device__ __forceinline__ float dz_knot(int k, int N, int lane, const float* A,
 
 // ---------------------------------------------------------------- merit, one (knot, alpha) item
 // l_k + mu * |c|_1 at z + alpha dz (merit.cuh:16-92, iiwa14_plant.cuh:276-327).
-template <int PL>
-__device__ __forceinline__ float merit_item(int k, int N, float alpha, float dt, const CostW& cw, float mu,
-                                            const float* xuk, const float* dzk, const float* refk,
-                                            const float* xrefk, const float* Qd, const float* Rd,
-                                            const float* fext, const float* x0, const float* dz0, const float* xs) {
+template <int PL, typename S>
+__device__ __forceinline__ S merit_item(int k, int N, S alpha, S dt, const CostW& cw, S mu,
+                                            const S* xuk, const S* dzk, const S* refk,
+                                            const S* xrefk, const S* Qd, const S* Rd,
+                                            const S* fext, const S* x0, const S* dz0, const S* xs) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
     const bool last = (k == N - 1);
-    float q[NQ], qd[NQ], u[NQ];
+    S q[NQ], qd[NQ], u[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; i++) {
         q[i] = xuk[i] + alpha * dzk[i];
         qd[i] = xuk[NQ + i] + alpha * dzk[NQ + i];
-        u[i] = last ? 0.f : xuk[NX + i] + alpha * dzk[NX + i];
+        u[i] = last ? S(0) : xuk[NX + i] + alpha * dzk[NX + i];
     }
-    float cost = 0.f;
+    S cost = S(0);
 #pragma unroll
     for (int i = 0; i < NQ; i++) {
-        const float e = qd[i] - (xrefk ? xrefk[NQ + i] : 0.f);
-        const float eq = q[i] - (xrefk ? xrefk[i] : 0.f);
-        cost += 0.5f * Qd[NQ + i] * e * e + 0.5f * Qd[i] * eq * eq;
-        cost += cw.q_lim * barrier(q[i], P::qlo(i), P::qhi(i));
-        cost += cw.vel_lim * barrier(qd[i], P::vlo(i), P::vhi(i));
+        const S e = qd[i] - (xrefk ? xrefk[NQ + i] : S(0));
+        const S eq = q[i] - (xrefk ? xrefk[i] : S(0));
+        cost += S(0.5) * Qd[NQ + i] * e * e + S(0.5) * Qd[i] * eq * eq;
+        cost += S(cw.q_lim) * barrier(q[i], P::template qlo<S>(i), P::template qhi<S>(i));
+        cost += S(cw.vel_lim) * barrier(qd[i], P::template vlo<S>(i), P::template vhi<S>(i));
         if (!last) {
-            cost += 0.5f * Rd[i] * u[i] * u[i];
-            cost += cw.ctrl_lim * barrier(u[i], P::ulo(i), P::uhi(i));
+            cost += S(0.5) * Rd[i] * u[i] * u[i];
+            cost += S(cw.ctrl_lim) * barrier(u[i], P::template ulo<S>(i), P::template uhi<S>(i));
         }
     }
-    Kin<PL> K;
+    Kin<PL, S> K;
     K.set(q);
-    float p[3];
-    ee_position<PL>(K, -1, p, static_cast<float*>(nullptr));
-    const float w = last ? cw.N_cost : cw.q_cost;
+    S p[3];
+    ee_position<PL>(K, -1, p, static_cast<S*>(nullptr));
+    const S w = S(last ? cw.N_cost : cw.q_cost);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const float e = p[i] - refk[i];
-        cost += 0.5f * w * e * e;
+        const S e = p[i] - refk[i];
+        cost += S(0.5) * w * e * e;
     }
-    float cons = 0.f;
+    S cons = S(0);
     if (!last) {
-        float qdd[NQ], L[NQ * (NQ + 1) / 2];
+        S qdd[NQ], L[NQ * (NQ + 1) / 2];
         forward_dynamics<PL>(K, qd, u, fext, qdd, L);
-        const float* xn = xuk + NX + NU;
-        const float* dn = dzk + NX + NU;
+        const S* xn = xuk + NX + NU;
+        const S* dn = dzk + NX + NU;
 #pragma unroll
         for (int i = 0; i < NQ; i++) {
-            const float qn = xn[i] + alpha * dn[i], qdn = xn[NQ + i] + alpha * dn[NQ + i];
-            cons += fabsf(qn - (q[i] + dt * qd[i] + 0.5f * qdd[i] * dt * dt));
-            cons += fabsf(qdn - (qd[i] + dt * qdd[i]));
+            const S qn = xn[i] + alpha * dn[i], qdn = xn[NQ + i] + alpha * dn[NQ + i];
+            cons += sabs(qn - (q[i] + dt * qd[i] + S(0.5) * qdd[i] * dt * dt));
+            cons += sabs(qdn - (qd[i] + dt * qdd[i]));
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < NX; i++) cons += fabsf(x0[i] + alpha * dz0[i] - xs[i]);
+        for (int i = 0; i < NX; i++) cons += sabs(x0[i] + alpha * dz0[i] - xs[i]);
     }
     return cost + mu * cons;
 }
@@ -556,15 +567,30 @@ template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
 }
+// fp64: the two 32-bit halves moved by the same DPP pattern
+template <int CTRL>
+__device__ __forceinline__ double dpp_f(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
 __device__ __forceinline__ float lane_f(float v, int l) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double lane_f(double v, int l) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 constexpr int DPP_QUAD_1032 = 0xB1;   // quad_perm [1,0,3,2]
 constexpr int DPP_QUAD_2301 = 0x4E;   // quad_perm [2,3,0,1]
 constexpr int DPP_ROW_HALF_MIRROR = 0x141;
 constexpr int DPP_ROW_MIRROR = 0x140;
 
-__device__ __forceinline__ float wave_sum(float v) {
+template <typename S>
+__device__ __forceinline__ S wave_sum(S v) {
     v += dpp_f<DPP_QUAD_1032>(v);
     v += dpp_f<DPP_QUAD_2301>(v);
     v += dpp_f<DPP_ROW_HALF_MIRROR>(v);
@@ -572,40 +598,44 @@ __device__ __forceinline__ float wave_sum(float v) {
     return (lane_f(v, 0) + lane_f(v, 16)) + (lane_f(v, 32) + lane_f(v, 48));
 }
 // Sum over one 16-lane row (a knot group); every lane of the row receives the sum.
-__device__ __forceinline__ float row16_sum(float v) {
+template <typename S>
+__device__ __forceinline__ S row16_sum(S v) {
     v += dpp_f<DPP_QUAD_1032>(v);
     v += dpp_f<DPP_QUAD_2301>(v);
     v += dpp_f<DPP_ROW_HALF_MIRROR>(v);
     v += dpp_f<DPP_ROW_MIRROR>(v);
     return v;
 }
-__device__ __forceinline__ float wave_max(float v) {
-    v = fmaxf(v, dpp_f<DPP_QUAD_1032>(v));
-    v = fmaxf(v, dpp_f<DPP_QUAD_2301>(v));
-    v = fmaxf(v, dpp_f<DPP_ROW_HALF_MIRROR>(v));
-    v = fmaxf(v, dpp_f<DPP_ROW_MIRROR>(v));
-    return fmaxf(fmaxf(lane_f(v, 0), lane_f(v, 16)), fmaxf(lane_f(v, 32), lane_f(v, 48)));
+template <typename S>
+__device__ __forceinline__ S wave_max(S v) {
+    v = smax(v, dpp_f<DPP_QUAD_1032>(v));
+    v = smax(v, dpp_f<DPP_QUAD_2301>(v));
+    v = smax(v, dpp_f<DPP_ROW_HALF_MIRROR>(v));
+    v = smax(v, dpp_f<DPP_ROW_MIRROR>(v));
+    return smax(smax(lane_f(v, 0), lane_f(v, 16)), smax(lane_f(v, 32), lane_f(v, 48)));
 }
 
-// Sum over the work-group; red holds one float per wave. Every thread returns the total.
-__device__ __forceinline__ float block_sum(float v, float* red) {
+// Sum over the work-group; red holds one value per wave. Every thread returns the total.
+template <typename S>
+__device__ __forceinline__ S block_sum(S v, S* red) {
     const int nw = (blockDim.x + 63) >> 6;
     v = wave_sum(v);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    float s = 0.f;
+    S s = S(0);
     for (int i = 0; i < nw; i++) s += red[i];
     return s;
 }
-__device__ __forceinline__ float block_max(float v, float* red) {
+template <typename S>
+__device__ __forceinline__ S block_max(S v, S* red) {
     const int nw = (blockDim.x + 63) >> 6;
     v = wave_max(v);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    float s = 0.f;
-    for (int i = 0; i < nw; i++) s = fmaxf(s, red[i]);
+    S s = S(0);
+    for (int i = 0; i < nw; i++) s = smax(s, red[i]);
     return s;
 }
 

@@ -433,3 +433,159 @@ class Solver:
         n = C.c_int32()
         check(self.lib.gato_last_kernel_time(self._h, C.byref(us), C.byref(n)), "gato_last_kernel_time")
         return us.value, n.value
+
+
+class Solver64:
+    """The reference's BSQP<double, B> (USE_DOUBLES, gato/settings.h:7-11; BSQP_{B}_double in
+    python/bindings.cu:244-252): the same constructor, solve result and setters as ``Solver``, in float64 (numpy
+    float64 or float64 device tensors). Runs the one-kernel-per-phase path at double precision."""
+
+    SET_RHO, SET_DRHO, SET_MU, SET_PCG_TOL = 0, 1, 2, 3
+    RESET_DUAL, RESET_RHO = 0, 1
+
+    def __init__(self, plant, N, B, *args, exit_rule="pcg0", terminal_mode=0, hess_mode=0, device=None, **kw):
+        cfg = _settings(args, kw)
+        if plant not in _lib.PLANT_IDS:
+            raise ValueError(f"unknown plant {plant!r}")
+        self.plant, self.N, self.B = plant, int(N), int(B)
+        self.nq, self.nx = _lib.plant_dims(plant)
+        self.nu = self.nq
+        self.traj = (self.nx + self.nu) * self.N - self.nu
+        self.cfg = cfg
+        self.device = torch.device(device or "cuda:0")
+        self.lib = _lib.load()
+        p = _params(cfg, "plain", exit_rule, terminal_mode, hess_mode)
+        self._h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            check(self.lib.gato_create_f64(C.byref(self._h), _lib.PLANT_IDS[plant], self.N, self.B, C.byref(p)),
+                  "gato_create_f64")
+        f = dict(dtype=torch.float64, device=self.device)
+        self._xu = torch.zeros(self.B, self.traj, **f)
+        self._xs = torch.zeros(self.B, self.nx, **f)
+        self._ref = torch.zeros(self.B, 6 * self.N, **f)
+        it = int(cfg["max_sqp_iters"])
+        self._st = dict(sqp_iters=np.zeros(self.B, np.int32), kkt_converged=np.zeros(self.B, np.int32),
+                        initial_merit=np.zeros(self.B, np.float32), final_merit=np.zeros(self.B, np.float32),
+                        pcg_iters=np.zeros((it, self.B), np.int32), ls_min_merit=np.zeros((it, self.B), np.float32),
+                        ls_step_size=np.zeros((it, self.B), np.float32),
+                        kkt_residual=np.zeros((it, self.B), np.float32))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.gato_destroy_f64(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def _stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _to_dev(self, x, dst, name):
+        n = dst.numel()
+        if isinstance(x, torch.Tensor):
+            if x.numel() != n:
+                raise ValueError(f"{name}: expected {n} values, got {x.numel()}")
+            dst.view(-1).copy_(x.reshape(-1).to(dtype=torch.float64))
+        else:
+            a = np.asarray(x, dtype=np.float64)
+            if a.size != n:
+                raise ValueError(f"{name}: expected {n} values, got {a.size}")
+            dst.view(-1).copy_(torch.from_numpy(np.ascontiguousarray(a).reshape(-1)))
+        return dst
+
+    def _host_f64(self, x, name):
+        a = np.ascontiguousarray(np.asarray(x.detach().cpu() if isinstance(x, torch.Tensor) else x,
+                                            dtype=np.float64).reshape(-1))
+        if a.size != self.B:
+            raise ValueError(f"{name}: expected {self.B} values, got {a.size}")
+        return a
+
+    def solve(self, xu_traj_batch, timestep, x_s_batch, reference_traj_batch, return_torch=False):
+        """BSQP<double, B>::solve: the result dict of ``Solver.solve`` with XU in float64 and the merits in double."""
+        xu = self._to_dev(xu_traj_batch, self._xu, "xu_traj_batch")
+        xs = self._to_dev(x_s_batch, self._xs, "x_s_batch")
+        ref = self._to_dev(reference_traj_batch, self._ref, "reference_traj_batch")
+        st = self._st
+        gs = GatoStats(0.0, 0, 0, *[st[k].ctypes.data_as(C.POINTER(C.c_int32 if st[k].dtype == np.int32 else C.c_float))
+                                    for k in ("sqp_iters", "kkt_converged", "initial_merit", "final_merit", "pcg_iters",
+                                              "ls_min_merit", "ls_step_size", "kkt_residual")])
+        with torch.cuda.device(self.device):
+            check(self.lib.gato_solve_f64(self._h, _ptr(xu), _ptr(xs), _ptr(ref), float(timestep), self._stream(),
+                                          C.byref(gs)), "gato_solve_f64")
+        init, final = np.zeros(self.B), np.zeros(self.B)
+        dp = C.POINTER(C.c_double)
+        check(self.lib.gato_last_merits_f64(self._h, init.ctypes.data_as(dp), final.ctypes.data_as(dp)),
+              "gato_last_merits_f64")
+        n_ls = gs.ls_num_iters
+        return {
+            "XU": xu.clone() if return_torch else xu.cpu().numpy(),
+            "sqp_time_us": float(gs.solve_time_us),
+            "sqp_iters": st["sqp_iters"].copy(),
+            "kkt_converged": st["kkt_converged"].copy(),
+            "final_merit": final,
+            "initial_merit": init,
+            "ls_num_iters": int(n_ls),
+            "pcg_times_us": np.zeros(n_ls, np.float32),
+            "pcg_iters": st["pcg_iters"][:n_ls].copy(),
+            "ls_min_merit": st["ls_min_merit"][:n_ls].copy(),
+            "ls_step_size": st["ls_step_size"][:n_ls].copy(),
+            "kkt_residual": st["kkt_residual"][: gs.n_iters].copy(),
+            "n_iters": int(gs.n_iters),
+            "all_pcg_iters": st["pcg_iters"][: gs.n_iters].copy(),
+        }
+
+    def set_joint_cost(self, Qd=None, Rd=None, xref=None):
+        bufs = []
+        for x, n, name in ((Qd, self.B * self.nx, "Qd"), (Rd, self.B * self.nu, "Rd"), (xref, self.B * self.N * self.nx, "xref")):
+            bufs.append(None if x is None else self._to_dev(x, torch.zeros(n, dtype=torch.float64, device=self.device), name))
+        check(self.lib.gato_set_joint_cost_f64(self._h, *[None if b is None else _ptr(b) for b in bufs], self._stream()),
+              "gato_set_joint_cost_f64")
+
+    def set_f_ext_batch(self, f_ext_batch):
+        f = self._to_dev(f_ext_batch, torch.zeros(self.B * 6, dtype=torch.float64, device=self.device), "f_ext_batch")
+        check(self.lib.gato_set_f_ext_batch_f64(self._h, _ptr(f), self._stream()), "gato_set_f_ext_batch_f64")
+
+    def _set(self, what, values, name, reset_default=1):
+        a = self._host_f64(values, name)
+        check(self.lib.gato_set_batch_f64(self._h, what, a.ctypes.data_as(C.POINTER(C.c_double)), int(reset_default)),
+              "gato_set_batch_f64")
+
+    def set_rho_penalty_batch(self, rho_batch, set_as_reset_default=True):
+        self._set(self.SET_RHO, rho_batch, "rho_batch", bool(set_as_reset_default))
+
+    def set_drho_batch(self, drho_batch, set_as_reset_default=True):
+        self._set(self.SET_DRHO, drho_batch, "drho_batch", bool(set_as_reset_default))
+
+    def set_mu_batch(self, mu_batch):
+        self._set(self.SET_MU, mu_batch, "mu_batch")
+
+    def set_pcg_tol_batch(self, eps_batch):
+        self._set(self.SET_PCG_TOL, eps_batch, "eps_batch")
+
+    def reset_dual(self):
+        check(self.lib.gato_reset_f64(self._h, self.RESET_DUAL), "gato_reset_f64")
+
+    def reset_rho(self):
+        check(self.lib.gato_reset_f64(self._h, self.RESET_RHO), "gato_reset_f64")
+
+    def set_rho_adaptation(self, enabled):
+        check(self.lib.gato_set_rho_adaptation_f64(self._h, int(bool(enabled))), "gato_set_rho_adaptation_f64")
+
+    def get_state(self):
+        lam = torch.zeros(self.B, self.N * self.nx, dtype=torch.float64, device=self.device)
+        rho, drho = np.zeros(self.B), np.zeros(self.B)
+        dp = C.POINTER(C.c_double)
+        check(self.lib.gato_get_state_f64(self._h, _ptr(lam), rho.ctypes.data_as(dp), drho.ctypes.data_as(dp)),
+              "gato_get_state_f64")
+        return lam.cpu().numpy(), rho, drho
+
+    def sim_forward(self, xk, uk, dt):
+        f = dict(dtype=torch.float64, device=self.device)
+        x = self._to_dev(xk, torch.zeros(self.nx, **f), "xk")
+        u = self._to_dev(uk, torch.zeros(self.nu, **f), "uk")
+        out = torch.zeros(self.B, self.nx, **f)
+        check(self.lib.gato_sim_forward_f64(self._h, _ptr(out), _ptr(x), _ptr(u), float(dt), self._stream()),
+              "gato_sim_forward_f64")
+        return out.cpu().numpy()

@@ -103,6 +103,38 @@ int gato_solve(gato_solver* s, float* d_xu, const float* d_xs, const float* d_re
 int gato_solve_noalloc(int32_t plant, int32_t knots, int32_t batch, const gato_params* params, float* d_xu,
                        const float* d_xs, const float* d_ref, float dt, void* stream, gato_stats* stats);
 
+/* FP64 mode: the reference's USE_DOUBLES build (gato/settings.h:7-11), classes BSQP_{B}_double of bindings.cu:244-252.
+ * The same SQP iteration in double precision on the device: the one-kernel-per-phase path (GATO_MODE_PLAIN; params->mode
+ * is ignored) with every phase instantiated at double, library sin, cos, division and square root. All device arrays
+ * are double with the layouts of the fp32 entry points; the statistics come back as gato_stats (merits rounded to
+ * float; gato_last_merits_f64 gives them in double). For high-accuracy offline solves and oracle cross-checks. */
+typedef struct gato_solver64 gato_solver64;
+#define GATO_SET_RHO 0     /* set_rho_penalty_batch */
+#define GATO_SET_DRHO 1    /* set_drho_batch */
+#define GATO_SET_MU 2      /* set_mu_batch */
+#define GATO_SET_PCG_TOL 3 /* set_pcg_tol_batch */
+#define GATO_RESET_DUAL 0
+#define GATO_RESET_RHO 1
+/* BSQP<double, B>(...) (bsqp.cuh:43-59) */
+int gato_create_f64(gato_solver64** out, int32_t plant, int32_t knots, int32_t batch, const gato_params* params);
+int gato_destroy_f64(gato_solver64* s);
+/* BSQP<double, B>::solve (bsqp.cuh:103-197): d_xu [B * traj], d_xs [B * nx], d_ref [B * 6N], device doubles */
+int gato_solve_f64(gato_solver64* s, double* d_xu, const double* d_xs, const double* d_ref, double dt, void* stream,
+                   gato_stats* stats);
+/* initial and final merits of the last solve in double (host arrays [B]; either may be NULL) */
+int gato_last_merits_f64(gato_solver64* s, double* h_initial, double* h_final);
+/* set_joint_cost, set_f_ext_batch, the batched setters (GATO_SET_*, host arrays [B]), reset_dual / reset_rho
+ * (GATO_RESET_*), set_rho_adaptation, get_state and sim_forward, as the fp32 entry points, in double */
+int gato_set_joint_cost_f64(gato_solver64* s, const double* d_Qd, const double* d_Rd, const double* d_xref,
+                            void* stream);
+int gato_set_f_ext_batch_f64(gato_solver64* s, const double* d_f_ext, void* stream);
+int gato_set_batch_f64(gato_solver64* s, int32_t what, const double* h_values, int32_t set_as_reset_default);
+int gato_reset_f64(gato_solver64* s, int32_t what);
+int gato_set_rho_adaptation_f64(gato_solver64* s, int32_t enabled);
+int gato_get_state_f64(gato_solver64* s, double* d_lambda, double* h_rho, double* h_drho);
+int gato_sim_forward_f64(gato_solver64* s, double* d_xnext, const double* d_x, const double* d_u, double dt,
+                         void* stream);
+
 /* Extension: joint-space tracking terms 1/2 sum_i Qd_i (x_i - xref_{k,i})^2 + 1/2 sum_i Rd_i u_i^2
  * replacing qd_cost / u_cost (NULL pointers restore the reference cost). Device pointers
  * d_Qd [B * nx], d_Rd [B * nu], d_xref [B * N * nx]; copied, so they may be freed afterwards. */
