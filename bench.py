@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--scale-steps", type=int, default=200,
                     help="Adam steps of the scale workload (iiwa14 N=128 B=4096, shared diag Q/R learned through "
                          "the backward pass) timed (0: skip)")
+    ap.add_argument("--fp64-steps", type=int, default=3,
+                    help="steps of FP64 mode on the drop-in workload at B=128 timed (0: skip)")
     ap.add_argument("--diff-steps", type=int, default=3,
                     help="steps of the differentiation workload (iiwa14 N=32 B=256, forward + backward) timed (0: skip)")
     ap.add_argument("--ref-mpc-steps", type=int, default=40,
@@ -325,6 +327,34 @@ def time_differentiation(steps):
 REFERENCE_HEATMAP_MS = 2.84  # indy7 N=32 B=512, plots/fig8_benchmark_heatmap.ipynb (GPU not stated)
 
 
+def time_fp64(steps):
+    """FP64 mode (BSQP_{B}_double) on the drop-in workload at B = 128, the largest batch the reference registers in
+    double: plain path, <= 10 SQP iterations."""
+    import torch
+
+    from gato_amd.solver import Solver64
+
+    wl = workload(32, 128)
+    s = Solver64(wl.plant, wl.N, wl.B, **_weights(wl.params))
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=s.device).contiguous()
+    xu0, xs, ref = t(wl.xu), t(wl.xs), t(wl.ref)
+
+    def one():
+        s.reset_dual()
+        s.reset_rho()
+        return s.solve(xu0, wl.dt, xs, ref)
+
+    one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = one()
+    el = time.perf_counter() - t0
+    return {"workload": "iiwa14 N=32 B=128 figure-8 (the drop-in workload at the largest double batch), fp64",
+            "problem_solves_per_s": 128 * steps / el, "ms_per_step": 1e3 * el / steps,
+            "mean_sqp_iters": float(r["sqp_iters"].mean())}
+
+
 def time_reference_mpc(steps):
     """The reference's published benchmark point: figure-8 MPC of indy7 with N=32, B=512 force hypotheses and
     its default MPC settings (one SQP iteration, <= 200 PCG iterations, pcg_tol 1e-4, warm start from the
@@ -562,6 +592,8 @@ def main():
         out["reference_heatmap_point"] = time_reference_mpc(args.ref_mpc_steps)
     if args.mixed_steps > 0:
         out["mixed_batch"] = time_mixed(args.mixed_steps)
+    if args.fp64_steps > 0:
+        out["fp64_mode"] = time_fp64(args.fp64_steps)
     if args.diff_steps > 0:
         out["differentiation"] = time_differentiation(args.diff_steps)
     if args.scale_steps > 0:

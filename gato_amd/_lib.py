@@ -31,7 +31,12 @@ _dp = C.POINTER(C.c_double)
 class GatoStats(C.Structure):
     _fields_ = [("solve_time_us", C.c_double), ("n_iters", C.c_int32), ("ls_num_iters", C.c_int32),
                 ("sqp_iters", _ip), ("kkt_converged", _ip), ("initial_merit", _fp), ("final_merit", _fp),
-                ("pcg_iters", _ip), ("ls_min_merit", _fp), ("ls_step_size", _fp), ("kkt_residual", _fp)]
+                ("pcg_iters", _ip), ("ls_min_merit", _fp), ("ls_step_size", _fp), ("kkt_residual", _fp),
+                ("status", _ip)]
+
+
+# Per-problem status bits (gato_stats.status)
+STATUS_KKT_CONVERGED, STATUS_ITER_CAP, STATUS_PCG_AT_CAP, STATUS_LS_FAILED, STATUS_NONFINITE = 1, 2, 4, 8, 16
 
 
 # Every symbol declared in include/gato_mi355x.h with its ctypes signature.

@@ -1285,6 +1285,35 @@ __global__ void __launch_bounds__(256) k_solve_summary(Args a, int* out) {
     }
 }
 
+// Per-problem status bits (GATO_STATUS_*) of the last solve, from its statistics and a scan of the returned
+// trajectory; one wavefront per problem. Non-finite values are found from the exponent bits, which no
+// floating-point mode folds away.
+__device__ __forceinline__ bool not_finite(float x) { return (__float_as_uint(x) & 0x7f800000u) == 0x7f800000u; }
+__device__ __forceinline__ bool not_finite(double x) {
+    return ((unsigned long long)__double_as_longlong(x) & 0x7ff0000000000000ull) == 0x7ff0000000000000ull;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(64) k_status(ArgsT<S> a, int* status) {
+    const int b = blockIdx.x;
+    const S* xu = a.xu + (size_t)b * a.traj;
+    bool bad = false;
+    for (int i = threadIdx.x; i < a.traj; i += 64) bad |= not_finite(xu[i]);
+    bad = __any(bad);
+    if (threadIdx.x != 0) return;
+    const int n = min(a.st_sqp[b], (int)a.max_sqp);
+    const bool conv = a.conv[b] != 0;
+    bool pcg_cap = false, ls_fail = false;
+    for (int i = 0; i < n; i++) {
+        pcg_cap |= a.st_pcg[(size_t)i * a.B + b] >= a.max_pcg;
+        ls_fail |= a.st_step[(size_t)i * a.B + b] < S(0);
+    }
+    bad |= not_finite(a.st_init[b]) || not_finite(a.st_final[b]);
+    status[b] = (conv ? GATO_STATUS_KKT_CONVERGED : 0) | (!conv && n >= (int)a.max_sqp ? GATO_STATUS_ITER_CAP : 0) |
+                (pcg_cap ? GATO_STATUS_PCG_AT_CAP : 0) | (ls_fail ? GATO_STATUS_LS_FAILED : 0) |
+                (bad ? GATO_STATUS_NONFINITE : 0);
+}
+
 // ------------------------------------------------------------------ fused persistent kernel
 // One work-group per problem; problems are independent (a converged problem stops iterating;
 // the reference instead keeps stepping converged problems until the whole batch has converged).
@@ -2143,6 +2172,7 @@ struct gato_solver {
     bool joint_cost = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int* d_stop = nullptr;
+    int* d_status = nullptr;  // per-problem GATO_STATUS_* bits of the last solve
     int G = 4;  // knot groups per work-group (launch shape; see grid_threads)
     int gkey_G = 0;
     float *lam_save = nullptr, *xu_tune = nullptr;  // launch-shape tuner: saved multipliers, scratch trajectory
@@ -2243,6 +2273,7 @@ size_t layout(gato_solver* s, bool assign, bool forward_only = false) {
     ip = I(4); if (assign) s->d_stop = ip;
     t = F(B); if (assign) a.st_init = t;
     t = F(B); if (assign) a.st_final = t;
+    ip = I(B); if (assign) s->d_status = ip;
     if (forward_only) return off + 256;
     t = F(B * N * nx); if (assign) s->lam_save = t;
     t = F(B * s->traj); if (assign) s->xu_tune = t;
@@ -2694,6 +2725,11 @@ int collect_stats(gato_solver* s, hipStream_t st, gato_stats* stats, double t_us
     HIP_TRY(hipMemcpyAsync(s->h_conv.data(), a.conv, B * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(s->h_init.data(), a.st_init, B * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(s->h_final.data(), a.st_final, B * 4, hipMemcpyDeviceToHost, st));
+    if (stats->status) {
+        hipLaunchKernelGGL(gato::k_status<float>, dim3(B), dim3(64), 0, st, a, s->d_status);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(stats->status, s->d_status, B * 4, hipMemcpyDeviceToHost, st));
+    }
     HIP_TRY(hipStreamSynchronize(st));
     stats->solve_time_us = t_us;
     int n_iters = 0;
@@ -2776,6 +2812,7 @@ struct gato_solver64 {
     char* pool = nullptr;
     gato::ArgsT<double> a{};
     double *d_fext, *d_Qd, *d_Rd, *d_xref, *d_rho, *d_drho, *d_drho_init, *d_mu, *d_tol;
+    int* d_status = nullptr;
     std::vector<double> h_rho_init, h_drho_init;
     bool joint_cost = false;
     std::vector<int> h_pcg, h_sqp, h_conv;
@@ -2813,6 +2850,7 @@ size_t layout64(gato_solver64* s, bool assign) {
     a.conv = I(B); a.active = I(B); a.solved = I(std::max<size_t>(B, s->prm.max_sqp_iters));
     a.st_pcg = I(it * B); a.st_kres = D(it * B); a.st_lsm = D(it * B); a.st_step = D(it * B); a.st_sqp = I(B);
     a.st_init = D(B); a.st_final = D(B);
+    s->d_status = I(B);
     return off + 256;
 }
 
@@ -2964,6 +3002,11 @@ int gato_solve_f64(gato_solver64* s, double* d_xu, const double* d_xs, const dou
     HIP_TRY(hipMemcpyAsync(s->h_conv.data(), a.conv, B * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(s->h_init.data(), a.st_init, B * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(s->h_final.data(), a.st_final, B * 8, hipMemcpyDeviceToHost, st));
+    if (stats->status) {
+        hipLaunchKernelGGL(gato::k_status<double>, dim3(B), dim3(64), 0, st, a, s->d_status);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(stats->status, s->d_status, B * 4, hipMemcpyDeviceToHost, st));
+    }
     HIP_TRY(hipStreamSynchronize(st));
     stats->solve_time_us = std::chrono::duration<double, std::micro>(t1 - t0).count();
     int n_iters = 0, ls_iters = 0;
@@ -3813,6 +3856,10 @@ int gato_solve_noalloc(int32_t plant, int32_t knots, int32_t batch, const gato_p
     if (!stats) return 0;
     hipLaunchKernelGGL(gato::k_solve_summary, dim3(1), dim3(256), 0, st, s.a, s.d_stop + 2);
     HIP_TRY(hipGetLastError());
+    if (stats->status) {
+        hipLaunchKernelGGL(gato::k_status<float>, dim3(batch), dim3(64), 0, st, s.a, s.d_status);
+        HIP_TRY(hipGetLastError());
+    }
     const size_t B = batch, it = params->max_sqp_iters;
     int32_t counts[2] = {0, 0};
     auto cp = [&](void* dst, const void* src, size_t bytes) -> int {
@@ -3823,7 +3870,7 @@ int gato_solve_noalloc(int32_t plant, int32_t knots, int32_t batch, const gato_p
         (rc = cp(stats->kkt_converged, s.a.conv, B * 4)) || (rc = cp(stats->initial_merit, s.a.st_init, B * 4)) ||
         (rc = cp(stats->final_merit, s.a.st_final, B * 4)) || (rc = cp(stats->pcg_iters, s.a.st_pcg, it * B * 4)) ||
         (rc = cp(stats->ls_min_merit, s.a.st_lsm, it * B * 4)) || (rc = cp(stats->ls_step_size, s.a.st_step, it * B * 4)) ||
-        (rc = cp(stats->kkt_residual, s.a.st_kres, it * B * 4)))
+        (rc = cp(stats->kkt_residual, s.a.st_kres, it * B * 4)) || (rc = cp(stats->status, s.d_status, B * 4)))
         return rc;
     HIP_TRY(hipStreamSynchronize(st));
     const auto t1 = std::chrono::high_resolution_clock::now();

@@ -93,13 +93,14 @@ class Solver:
                         initial_merit=np.zeros(self.B, np.float32), final_merit=np.zeros(self.B, np.float32),
                         pcg_iters=np.zeros((it, self.B), np.int32), ls_min_merit=np.zeros((it, self.B), np.float32),
                         ls_step_size=np.zeros((it, self.B), np.float32),
-                        kkt_residual=np.zeros((it, self.B), np.float32))
+                        kkt_residual=np.zeros((it, self.B), np.float32), status=np.zeros(self.B, np.int32))
 
     def _stats_struct(self):
         st = self._st
         return GatoStats(0.0, 0, 0, *[st[k].ctypes.data_as(C.POINTER(C.c_int32 if st[k].dtype == np.int32 else C.c_float))
                                       for k in ("sqp_iters", "kkt_converged", "initial_merit", "final_merit",
-                                                "pcg_iters", "ls_min_merit", "ls_step_size", "kkt_residual")])
+                                                "pcg_iters", "ls_min_merit", "ls_step_size", "kkt_residual",
+                                                "status")])
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -148,14 +149,15 @@ class Solver:
     def solve(self, xu_traj_batch, timestep, x_s_batch, reference_traj_batch, return_torch=False):
         """BSQP::solve on B problems (bindings.cu:68-148). Returns the reference's result dict
         (XU, sqp_time_us, sqp_iters, kkt_converged, final_merit, initial_merit, ls_num_iters,
-        pcg_times_us, pcg_iters, ls_min_merit, ls_step_size) plus kkt_residual and n_iters."""
+        pcg_times_us, pcg_iters, ls_min_merit, ls_step_size) plus kkt_residual, n_iters and status (per-problem
+        GATO_STATUS_* bits: 1 converged, 2 iteration cap, 4 PCG at its cap, 8 line-search failure, 16 non-finite)."""
         xu = self._to_dev(xu_traj_batch, self._xu, "xu_traj_batch")
         xs = self._to_dev(x_s_batch, self._xs, "x_s_batch")
         ref = self._to_dev(reference_traj_batch, self._ref, "reference_traj_batch")
         st = self._st
         gs = GatoStats(0.0, 0, 0, *[st[k].ctypes.data_as(C.POINTER(C.c_int32 if st[k].dtype == np.int32 else C.c_float))
                                     for k in ("sqp_iters", "kkt_converged", "initial_merit", "final_merit", "pcg_iters",
-                                              "ls_min_merit", "ls_step_size", "kkt_residual")])
+                                              "ls_min_merit", "ls_step_size", "kkt_residual", "status")])
         with torch.cuda.device(self.device):
             self._touch()
             check(self.lib.gato_solve(self._h, _ptr(xu), _ptr(xs), _ptr(ref), float(timestep), self._stream(),
@@ -166,6 +168,7 @@ class Solver:
             "sqp_time_us": float(gs.solve_time_us),
             "sqp_iters": st["sqp_iters"].copy(),
             "kkt_converged": st["kkt_converged"].copy(),
+            "status": st["status"].copy(),
             "final_merit": st["final_merit"].copy(),
             "initial_merit": st["initial_merit"].copy(),
             "ls_num_iters": int(n_ls),
@@ -192,7 +195,8 @@ class Solver:
         if stats:
             gs = GatoStats(0.0, 0, 0, *[st[k].ctypes.data_as(C.POINTER(C.c_int32 if st[k].dtype == np.int32 else C.c_float))
                                         for k in ("sqp_iters", "kkt_converged", "initial_merit", "final_merit",
-                                                  "pcg_iters", "ls_min_merit", "ls_step_size", "kkt_residual")])
+                                                  "pcg_iters", "ls_min_merit", "ls_step_size", "kkt_residual",
+                                                  "status")])
         self._touch()
         check(self.lib.gato_solve(self._h, _ptr(xu), _ptr(xs), _ptr(ref), float(dt), self._stream(),
                                   C.byref(gs) if gs is not None else None), "gato_solve")
@@ -468,7 +472,7 @@ class Solver64:
                         initial_merit=np.zeros(self.B, np.float32), final_merit=np.zeros(self.B, np.float32),
                         pcg_iters=np.zeros((it, self.B), np.int32), ls_min_merit=np.zeros((it, self.B), np.float32),
                         ls_step_size=np.zeros((it, self.B), np.float32),
-                        kkt_residual=np.zeros((it, self.B), np.float32))
+                        kkt_residual=np.zeros((it, self.B), np.float32), status=np.zeros(self.B, np.int32))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -510,7 +514,7 @@ class Solver64:
         st = self._st
         gs = GatoStats(0.0, 0, 0, *[st[k].ctypes.data_as(C.POINTER(C.c_int32 if st[k].dtype == np.int32 else C.c_float))
                                     for k in ("sqp_iters", "kkt_converged", "initial_merit", "final_merit", "pcg_iters",
-                                              "ls_min_merit", "ls_step_size", "kkt_residual")])
+                                              "ls_min_merit", "ls_step_size", "kkt_residual", "status")])
         with torch.cuda.device(self.device):
             check(self.lib.gato_solve_f64(self._h, _ptr(xu), _ptr(xs), _ptr(ref), float(timestep), self._stream(),
                                           C.byref(gs)), "gato_solve_f64")
@@ -524,6 +528,7 @@ class Solver64:
             "sqp_time_us": float(gs.solve_time_us),
             "sqp_iters": st["sqp_iters"].copy(),
             "kkt_converged": st["kkt_converged"].copy(),
+            "status": st["status"].copy(),
             "final_merit": final,
             "initial_merit": init,
             "ls_num_iters": int(n_ls),

@@ -71,7 +71,17 @@ typedef struct gato_stats {
     float* ls_min_merit;    /* [max_sqp_iters * B] */
     float* ls_step_size;    /* [max_sqp_iters * B]; -1 = line-search failure */
     float* kkt_residual;    /* [max_sqp_iters * B] */
+    int32_t* status;        /* [B] GATO_STATUS_* bits, or NULL (no status pass is run) */
 } gato_stats;
+
+/* Per-problem failure status of a solve (gato_stats.status). The reference declares PCGStats::converged and never
+ * sets it (gato/types.cuh:28), and reports failures only through its per-iteration arrays; these bits summarise
+ * them per problem, and add a scan of the returned trajectory for non-finite values, on the device. */
+#define GATO_STATUS_KKT_CONVERGED 1  /* kkt_converged: the problem met the stopping rule */
+#define GATO_STATUS_ITER_CAP      2  /* ran max_sqp_iters iterations without meeting it */
+#define GATO_STATUS_PCG_AT_CAP    4  /* in some iteration the PCG ran max_pcg_iters iterations */
+#define GATO_STATUS_LS_FAILED     8  /* in some iteration no step length lowered the merit (step -1) */
+#define GATO_STATUS_NONFINITE     16 /* a NaN or Inf in the returned trajectory or the initial or final merit */
 
 typedef struct gato_solver gato_solver;
 

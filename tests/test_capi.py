@@ -32,7 +32,35 @@ def test_struct_layouts_match_header():
 
     # 15 reference scalars + 4 int32 extensions, 4 bytes each
     assert C.sizeof(GatoParams) == 19 * 4
-    assert C.sizeof(GatoStats) == 8 + 4 + 4 + 8 * 8
+    assert C.sizeof(GatoStats) == 8 + 4 + 4 + 9 * 8
+
+
+def test_struct_offsets_match_the_c_compiler(tmp_path):
+    # the ctypes mirrors against what a C compiler makes of the header: size and every field offset
+    import shutil
+    import subprocess
+
+    from gato_amd._lib import GatoParams, GatoStats
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "gato_mi355x.h"', "int main(void) {"]
+    for cname, ct in (("gato_params", GatoParams), ("gato_stats", GatoStats)):
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in ct._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                         text=True).stdout.splitlines())
+    for cname, ct in (("gato_params", GatoParams), ("gato_stats", GatoStats)):
+        assert int(got[f"{cname} size"]) == C.sizeof(ct)
+        for f, _ in ct._fields_:
+            assert int(got[f"{cname} {f}"]) == getattr(ct, f).offset, (cname, f)
 
 
 def test_argument_validation_without_device():

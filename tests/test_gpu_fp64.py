@@ -132,3 +132,18 @@ def test_drop_in_double_classes(gpu):
     r = s.solve(wl.xu, wl.dt, wl.xs, wl.ref)
     assert r["XU"].dtype == np.float64 and r["XU"].shape == (2, s.traj)
     assert not hasattr(mod, "BSQP_256_double")  # USE_DOUBLES registers B <= 128 (bindings.cu:244-252)
+
+
+def test_drop_in_configuration_at_the_largest_double_batch(gpu, O):
+    # The drop-in figure-8 configuration (iiwa14, N = 32, <= 10 SQP iterations) at B = 128, the largest batch the
+    # reference registers in double: the north-star parity bar on whole solves - identical SQP iteration counts on at
+    # least 99% of the problems and trajectories within 1e-4 relative - plus identical line-search steps.
+    wl = W.drop_in_figure8("iiwa14", N=32, B=128, seed=1)
+    r = make64(wl).solve(wl.xu, wl.dt, wl.xs, wl.ref)
+    o = oracle(O, wl)
+    assert (r["sqp_iters"] == o["sqp_iters"]).mean() >= 0.99
+    e = xu_rel(r["XU"], o["XU"])
+    assert e.max() < 1e-4, e.max()
+    same_steps = (r["ls_step_size"] == o["ls_step"][: r["ls_num_iters"]].astype(np.float32)).all(0)
+    assert same_steps.mean() >= 0.99
+    assert rel_err(r["final_merit"], o["final_merit"]) < 1e-6

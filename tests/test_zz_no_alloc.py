@@ -21,14 +21,15 @@ pytestmark = pytest.mark.gpu
 SCALARS = ("dt", "max_sqp_iters", "kkt_tol", "max_pcg_iters", "pcg_tol", "solve_ratio", "mu", "q_cost", "qd_cost",
            "u_cost", "N_cost", "q_lim_cost", "vel_lim_cost", "ctrl_lim_cost", "rho")
 STAT_KEYS = ("sqp_iters", "kkt_converged", "initial_merit", "final_merit", "pcg_iters", "ls_min_merit",
-             "ls_step_size", "kkt_residual")
+             "ls_step_size", "kkt_residual", "status")
 
 
 def _stats(B, iters):
     st = dict(sqp_iters=np.zeros(B, np.int32), kkt_converged=np.zeros(B, np.int32),
               initial_merit=np.zeros(B, np.float32), final_merit=np.zeros(B, np.float32),
               pcg_iters=np.zeros((iters, B), np.int32), ls_min_merit=np.zeros((iters, B), np.float32),
-              ls_step_size=np.zeros((iters, B), np.float32), kkt_residual=np.zeros((iters, B), np.float32))
+              ls_step_size=np.zeros((iters, B), np.float32), kkt_residual=np.zeros((iters, B), np.float32),
+              status=np.zeros(B, np.int32))
     ptrs = [st[k].ctypes.data_as(C.POINTER(C.c_int32 if st[k].dtype == np.int32 else C.c_float)) for k in STAT_KEYS]
     return st, _lib.GatoStats(0.0, 0, 0, *ptrs)
 
