@@ -24,6 +24,10 @@
 // caps every phase kernel at 128 VGPRs, so two problems share a CU; left free, the scheduler spends
 // registers on latency hiding and drops to one work-group per CU.
 #define GATO_TWO_WG_PER_CU __attribute__((amdgpu_waves_per_eu(4)))
+// The phase kernels templated on the scalar type: fp32 as above; the fp64 instantiations run at most 16 knot
+// groups (256 lanes) and may use 256 VGPRs (two waves per SIMD), which their doubled register footprint needs.
+#define GATO_PHASE_BOUNDS(S) \
+    __launch_bounds__(sizeof(S) == 8 ? 256 : 512) __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? 2 : 4)))
 
 namespace gato {
 
@@ -936,7 +940,7 @@ __device__ __forceinline__ void merit_body(const ArgsT<S>& a, int nal, int which
 }
 
 template <int PL, typename S = float>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_merit(ArgsT<S> a, int nal, int which, int it) {
+__global__ void GATO_PHASE_BOUNDS(S) k_merit(ArgsT<S> a, int nal, int which, int it) {
     merit_body<PL, S>(a, nal, which, it);
 }
 
@@ -1045,7 +1049,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 }
 
 template <int PL, typename S = float>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_lin(ArgsT<S> a) {
+__global__ void GATO_PHASE_BOUNDS(S) k_lin(ArgsT<S> a) {
     S* sm = dyn_lds<S>();
     const int b = blockIdx.x;
     if (stopped(a) || (a.exit_rule == GATO_EXIT_KKT && !a.active[b])) return;
@@ -1053,7 +1057,7 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_lin(ArgsT<S> a) {
 }
 
 template <int PL, typename S = float>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_schur(ArgsT<S> a) {
+__global__ void GATO_PHASE_BOUNDS(S) k_schur(ArgsT<S> a) {
     S* sm = dyn_lds<S>();
     const int b = blockIdx.x;
     if (stopped(a) || (a.exit_rule == GATO_EXIT_KKT && !a.active[b])) return;
@@ -1071,7 +1075,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 }
 
 template <int PL, int NPASS, typename S = float>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg(ArgsT<S> a, int it) {
+__global__ void GATO_PHASE_BOUNDS(S) k_pcg(ArgsT<S> a, int it) {
     S* sm = dyn_lds<S>();
     const int b = blockIdx.x;
     if (stopped(a)) return;
@@ -1135,7 +1139,7 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_btd(Args a, cons
 }
 
 template <int PL, typename S = float>
-__global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_dz(ArgsT<S> a, int it) {
+__global__ void GATO_PHASE_BOUNDS(S) k_dz(ArgsT<S> a, int it) {
     S* sm = dyn_lds<S>();
     const int b = blockIdx.x;
     if (stopped(a)) return;
@@ -2854,8 +2858,8 @@ size_t layout64(gato_solver64* s, bool assign) {
     return off + 256;
 }
 
-// knot groups per work-group: the fewest (4, 8, 16) whose knots per group fit the PCG's passes (at most 8)
-int groups64(int N) { return N <= 32 ? 4 : N <= 64 ? 8 : 16; }
+// knot groups per work-group: one per knot up to 16 (the PCG then makes at most 8 passes for N <= 128)
+int groups64(int N) { return std::min(16, default_groups(N)); }
 
 int upload_defaults64(gato_solver64* s) {
     std::vector<double> Qd((size_t)s->B * s->nx), Rd((size_t)s->B * s->nu);
