@@ -359,26 +359,30 @@ def time_reference_mpc(steps):
     """The reference's published benchmark point: figure-8 MPC of indy7 with N=32, B=512 force hypotheses and
     its default MPC settings (one SQP iteration, <= 200 PCG iterations, pcg_tol 1e-4, warm start from the
     previous solution). The loop follows the reference's harness (mpc_controller.py:136-262) with every step on
-    the device: the figure-8 path is generated there (common.py:10-46), the wrench estimator
-    (force_estimator.py, with the harness's settings, mpc_controller.py:123-131 and :307) writes the B hypotheses
-    into the solver in its frame, the plant advances one dt = 10 ms in RK4 sub-steps of 1 ms under the applied
-    control (no true external wrench), the hypothesis whose one-step prediction best matches the new state is
-    selected, its trajectory becomes every problem's warm start and the estimator updates. Reported: the solve's
-    wall clock as the module reports it (sqp_time_us, the reference's metric), its device time, and the device
-    time of the whole control step."""
+    the device: the figure-8 path is generated there (common.py:10-46); each control step the plant (indy7 with its
+    URDF centres of mass, as the harness simulates it with pinocchio) advances by the previous solve's time in RK4
+    sub-steps of 1 ms under the best trajectory's per-knot controls, the remainders accumulated into extra
+    sub-steps (mpc_controller.py:187-213; no true external wrench); the reference window starts at the knot the
+    simulated clock has reached; the wrench estimator (force_estimator.py, with the harness's settings,
+    mpc_controller.py:123-131 and :307) writes the B hypotheses into the solver in its frame; after the solve the
+    hypothesis whose one-step prediction best matches the new state is selected, its trajectory becomes every
+    problem's warm start and the estimator updates. The solve time that drives the clock is the module's own
+    (sqp_time_us; the harness times its numpy call, host copies included). Reported: that solve time (the
+    reference's metric), its device time, and the device time of the whole control step."""
     import numpy as np
     import torch
 
     from gato_amd import workloads as W
-    from gato_amd.mpc import ForceEstimator, figure8_path
+    from gato_amd.mpc import ForceEstimator, PlantSimulator, figure8_path
     from gato_amd.solver import Solver
 
-    N, B, dt = 32, 512, 0.01
+    N, B, dt, h = 32, 512, 0.01, 1e-3
     prm = dict(W.REFERENCE_MPC_PARAMS)
     s = Solver("indy7", N, B, mode="graph", dt=dt, **prm)
     nx, nu = s.nx, s.nu
     dev = s.device
     est = ForceEstimator(s, initial_radius=5.0, min_radius=2.0, max_radius=20.0, smoothing_factor=0.5, seed=6)
+    sim = PlantSimulator("indy7_urdf", N, dt, h, device=dev)
     t = lambda a: torch.tensor(a, dtype=torch.float32, device=dev).contiguous()
     path = figure8_path(dt, device=dev)
     npts = path.numel() // 6
@@ -389,26 +393,33 @@ def time_reference_mpc(steps):
     xu_d = t(xu)
     xs_d = t(np.tile(x, (B, 1)))
     ref_d = torch.zeros(B, 6 * N, dtype=torch.float32, device=dev)
-    plant = xs_d.clone()
-    no_wrench = torch.zeros(B, 6, dtype=torch.float32, device=dev)
+    plant = t(x[None, :])
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s.reset_dual()
+    # the harness's warm-up solve from the start state (mpc_controller.py:176-181)
+    s.window_reference(ref_d, path.view(npts, 6), 0)
+    est.generate_batch(plant[0, :nu])
+    s.solve_device(xu_d, xs_d, ref_d)  # XU_best = its row 0 (mpc_controller.py:181)
+    timestep, clock, substeps = dt, 0.0, 0
     wall, devt, stept, pcg, best_ids = [], [], [], [], []
     for step in range(steps + 3):
         ev0.record()
-        s.window_reference(ref_d, path.view(npts, 6), step)
-        xs_d.copy_(plant[0].expand(B, nx))
+        x_last = plant[0].clone()
+        u_last = xu_d[0, nx:nx + nu].clone()
+        run = sim.advance(plant, xu_d[0:1], timestep)
+        substeps += run
+        clock += run * h
+        s.window_reference(ref_d, path.view(npts, 6), int(clock / dt))
+        xs_d.copy_(plant.expand(B, nx))
         xu_d[:, :nx] = xs_d
         est.generate_batch(plant[0, :nu])
         s.reset_rho()
         st = s.solve_device(xu_d, xs_d, ref_d, stats=True)
-        x_last = plant[0].clone()
-        u_app = xu_d[0, nx:nx + nu].clone()
-        s.simulate_rk4(plant, u_app.expand(B, nu).contiguous(), no_wrench, 1e-3, 10)
-        best, _ = s.select_hypothesis(x_last, u_app, plant[0], dt, xu_d)
+        best, _ = s.select_hypothesis(x_last, u_last, plant[0], max(h, round(timestep / h) * h), xu_d)
         est.update(alpha=0.6, beta=0.5)
         ev1.record()
         torch.cuda.synchronize()
+        timestep = st["sqp_time_us"] * 1e-6
         if step >= 3:
             wall.append(st["sqp_time_us"])
             devt.append(s.last_kernel_time()[0])
@@ -420,7 +431,8 @@ def time_reference_mpc(steps):
             "solves": steps, "ms_per_solve_wall": ms_wall, "ms_per_solve_device": float(np.mean(devt)) / 1e3,
             "ms_per_control_step": float(np.mean(stept)), "mean_pcg_iters": float(np.mean(pcg)),
             "reference_ms": REFERENCE_HEATMAP_MS, "speedup_vs_reference": REFERENCE_HEATMAP_MS / ms_wall,
-            "estimator_radius_final": est.state()["radius"],
+            "estimator_radius_final": est.state()["radius"], "plant_model": "indy7 with URDF centres of mass",
+            "simulated_seconds": clock, "plant_substeps": substeps,
             "exploit_hypothesis_selected": float(np.mean(np.array(best_ids) < 3)),
             "note": "reference hardware not stated; same configuration, different GPU"}
 

@@ -9,6 +9,8 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libgato_mi355x.so"
 
 PLANT_IDS = {"iiwa14": 0, "indy7": 1}
+# plant models of the simulator: the solvers' two and indy7 with its URDF centres of mass (GATO_PLANT_INDY7_URDF)
+SIM_MODELS = {"iiwa14": 0, "indy7": 1, "indy7_urdf": 2}
 MODE_PLAIN, MODE_FUSED, MODE_GRAPH = 0, 1, 2
 FORCE_EST_STATE_LEN = 30  # estimate[6] momentum[6] smoothed[6] rotation[9] radius confidence updates
 EXIT_PCG0, EXIT_KKT = 0, 1
@@ -80,6 +82,8 @@ SIGNATURES = {
     "gato_select_hypothesis": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _VP, _ip, _VP]),
     "gato_simulate_rk4": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, C.c_int32, _VP]),
     "gato_window_reference": (C.c_int, [_VP, _VP, _VP, C.c_int32, C.c_int32, _VP]),
+    "gato_simulate_plant": (C.c_int, [C.c_int32, C.c_int32, _VP, _VP, C.c_int32, _VP, C.c_double, C.c_int32, C.c_int32,
+                                      C.c_double, _VP]),
     "gato_figure8_path": (C.c_int, [_VP, C.c_int32, C.c_int32, C.c_double, C.c_double, _dp, C.c_double, _VP]),
     "gato_force_est_create": (C.c_int, [C.POINTER(_VP), C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float,
                                         C.c_uint64]),

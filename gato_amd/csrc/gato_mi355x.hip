@@ -1815,47 +1815,75 @@ __global__ void k_broadcast_row(float* xu, int traj, int B, const int* best) {
 
 
 // ------------------------------------------------------------------ plant simulation and reference windows (MPC)
-// RK4 of qdd = FD(q, qd, u; f_ext) over `steps` sub-steps of h, the integration the reference's MPC harness
-// runs on the host with pinocchio (common.py:49-91), here for every problem b with its own control u_b and
-// wrench f_ext_b (x_b updated in place).
+// One RK4 step of length h of qdd = FD(q, qd, u; f_ext) for one system (q, v updated in place): the integration the
+// reference's MPC harness runs on the host with pinocchio (common.py:49-91: k1..k4 from ABA, revolute joints
+// integrated as q + v dt). fe may be null (no wrench).
 template <int PL>
-__global__ void __launch_bounds__(64) k_rk4(float* x, const float* u, const float* fext, float h, int steps, int B) {
-    using P = Plant<PL>;
-    constexpr int NQ = P::NQ;
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    float q[NQ], v[NQ], uu[NQ];
-#pragma unroll
-    for (int i = 0; i < NQ; i++) { q[i] = x[b * 2 * NQ + i]; v[i] = x[b * 2 * NQ + NQ + i]; uu[i] = u[b * NQ + i]; }
-    const float* fe = fext + 6 * b;
+__device__ __forceinline__ void rk4_step(float* q, float* v, const float* uu, const float* fe, float h) {
+    constexpr int NQ = Plant<PL>::NQ;
     auto accel = [&](const float* qq, const float* vv, float* a) {
         Kin<PL> K;
         K.set(qq);
         float L[NQ * (NQ + 1) / 2];
         forward_dynamics<PL>(K, vv, uu, fe, a, L);
     };
-    for (int st = 0; st < steps; st++) {
-        float k1v[NQ], k2v[NQ], k3v[NQ], k4v[NQ], qt[NQ], vt[NQ];
-        accel(q, v, k1v);
+    float k1v[NQ], k2v[NQ], k3v[NQ], k4v[NQ], qt[NQ], vt[NQ];
+    accel(q, v, k1v);
 #pragma unroll
-        for (int i = 0; i < NQ; i++) { qt[i] = q[i] + 0.5f * h * v[i]; vt[i] = v[i] + 0.5f * h * k1v[i]; }
-        accel(qt, vt, k2v);
-        float v2[NQ];
+    for (int i = 0; i < NQ; i++) { qt[i] = q[i] + 0.5f * h * v[i]; vt[i] = v[i] + 0.5f * h * k1v[i]; }
+    accel(qt, vt, k2v);
+    float v2[NQ];
 #pragma unroll
-        for (int i = 0; i < NQ; i++) { v2[i] = vt[i]; qt[i] = q[i] + 0.5f * h * v2[i]; vt[i] = v[i] + 0.5f * h * k2v[i]; }
-        accel(qt, vt, k3v);
-        float v3[NQ];
+    for (int i = 0; i < NQ; i++) { v2[i] = vt[i]; qt[i] = q[i] + 0.5f * h * v2[i]; vt[i] = v[i] + 0.5f * h * k2v[i]; }
+    accel(qt, vt, k3v);
+    float v3[NQ];
 #pragma unroll
-        for (int i = 0; i < NQ; i++) { v3[i] = vt[i]; qt[i] = q[i] + h * v3[i]; vt[i] = v[i] + h * k3v[i]; }
-        accel(qt, vt, k4v);
+    for (int i = 0; i < NQ; i++) { v3[i] = vt[i]; qt[i] = q[i] + h * v3[i]; vt[i] = v[i] + h * k3v[i]; }
+    accel(qt, vt, k4v);
 #pragma unroll
-        for (int i = 0; i < NQ; i++) {
-            q[i] += h / 6.f * (v[i] + 2.f * v2[i] + 2.f * v3[i] + vt[i]);
-            v[i] += h / 6.f * (k1v[i] + 2.f * k2v[i] + 2.f * k3v[i] + k4v[i]);
-        }
+    for (int i = 0; i < NQ; i++) {
+        q[i] += h / 6.f * (v[i] + 2.f * v2[i] + 2.f * v3[i] + vt[i]);
+        v[i] += h / 6.f * (k1v[i] + 2.f * k2v[i] + 2.f * k3v[i] + k4v[i]);
     }
+}
+
+// `steps` RK4 sub-steps of h for every problem b with its own control u_b and wrench f_ext_b (x_b in place).
+template <int PL>
+__global__ void __launch_bounds__(64) k_rk4(float* x, const float* u, const float* fext, float h, int steps, int B) {
+    constexpr int NQ = Plant<PL>::NQ;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    float q[NQ], v[NQ], uu[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) { q[i] = x[b * 2 * NQ + i]; v[i] = x[b * 2 * NQ + NQ + i]; uu[i] = u[b * NQ + i]; }
+    for (int st = 0; st < steps; st++) rk4_step<PL>(q, v, uu, fext + 6 * b, h);
 #pragma unroll
     for (int i = 0; i < NQ; i++) { x[b * 2 * NQ + i] = q[i]; x[b * 2 * NQ + NQ + i] = v[i]; }
+}
+
+// The MPC harness's plant advance (mpc_controller.py:194-213): sub-steps i = first .. first + steps - 1 of length h,
+// sub-step i applying the control of knot min(int(i / (knot_dt / h)), knots - 1) of system b's trajectory xu_b
+// (the quotient in double, as the harness computes it in Python), under wrench fext_b (null: none).
+template <int PL>
+__global__ void __launch_bounds__(64) k_plant_rk4(float* x, const float* xu, int traj, int knots, const float* fext,
+                                                  double h, int first, int steps, double knot_dt, int n) {
+    constexpr int NQ = Plant<PL>::NQ, NX = 2 * NQ;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    float q[NQ], v[NQ], uu[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) { q[i] = x[b * NX + i]; v[i] = x[b * NX + NQ + i]; }
+    const float* fe = fext ? fext + 6 * b : nullptr;
+    const double per_knot = knot_dt / h;
+    for (int i = first; i < first + steps; i++) {
+        const int k = min((int)((double)i / per_knot), knots - 1);
+        const float* uk = xu + (size_t)b * traj + (size_t)k * (NX + NQ) + NX;
+#pragma unroll
+        for (int j = 0; j < NQ; j++) uu[j] = uk[j];
+        rk4_step<PL>(q, v, uu, fe, (float)h);
+    }
+#pragma unroll
+    for (int i = 0; i < NQ; i++) { x[b * NX + i] = q[i]; x[b * NX + NQ + i] = v[i]; }
 }
 
 // ref_b[k] = path[offset + k] (6 floats per point) for every problem: the reference window the MPC caller
@@ -3373,6 +3401,29 @@ int gato_simulate_rk4(gato_solver* s, float* d_x, const float* d_u, const float*
         hipLaunchKernelGGL(gato::k_rk4<0>, dim3(nb), dim3(64), 0, st, d_x, d_u, d_f_ext, h, steps, s->B);
     else
         hipLaunchKernelGGL(gato::k_rk4<1>, dim3(nb), dim3(64), 0, st, d_x, d_u, d_f_ext, h, steps, s->B);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int gato_simulate_plant(int32_t model, int32_t n, float* d_x, const float* d_xu, int32_t knots, const float* d_f_ext,
+                        double h, int32_t first_substep, int32_t steps, double knot_dt, void* stream) {
+    if (!d_x || !d_xu) return fail(GATO_E_ARG, "null argument");
+    if (model < 0 || model > 2) return fail(GATO_E_ARG, "unknown plant model");
+    if (n <= 0 || knots <= 0 || steps < 0 || first_substep < 0 || !(h > 0.0) || !(knot_dt > 0.0))
+        return fail(GATO_E_ARG, "need n, knots > 0, steps, first_substep >= 0 and h, knot_dt > 0");
+    const int nq = model == GATO_PLANT_IIWA14 ? 7 : 6;
+    const int traj = (3 * nq) * knots - nq;
+    hipStream_t st = (hipStream_t)stream;
+    const int nb = (n + 63) / 64;
+    if (model == GATO_PLANT_IIWA14)
+        hipLaunchKernelGGL(gato::k_plant_rk4<0>, dim3(nb), dim3(64), 0, st, d_x, d_xu, traj, knots, d_f_ext, h,
+                           first_substep, steps, knot_dt, n);
+    else if (model == GATO_PLANT_INDY7)
+        hipLaunchKernelGGL(gato::k_plant_rk4<1>, dim3(nb), dim3(64), 0, st, d_x, d_xu, traj, knots, d_f_ext, h,
+                           first_substep, steps, knot_dt, n);
+    else
+        hipLaunchKernelGGL(gato::k_plant_rk4<2>, dim3(nb), dim3(64), 0, st, d_x, d_xu, traj, knots, d_f_ext, h,
+                           first_substep, steps, knot_dt, n);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -58,6 +58,18 @@ PLANTS = {
         ),
         False,
     ),
+    # The plant the reference's MPC examples simulate: pinocchio's model of the same URDF (common.py:49-91), which
+    # reads the inertial origins, so indy7's links keep their centres of mass. Used by the plant simulator only; for
+    # iiwa14 the solver's model already is the URDF's.
+    "indy7_urdf": (
+        "examples/indy7_description/indy7.urdf",
+        dict(
+            pos=[3.0543, 3.0543, 3.0543, 3.0543, 3.0543, 3.7520],
+            vel=[2.61, 2.61, 2.61, 3.14, 3.14, 3.14],
+            ctrl=[431.97, 431.97, 197.23, 79.79, 79.79, 79.79],
+        ),
+        True,
+    ),
 }
 
 
@@ -204,6 +216,8 @@ def emit(ref_root):
         f"static constexpr double T[GATO_IIWA14_NQ][GATO_JOINT_REC] = GATO_IIWA14_JOINTS_INIT; }};",
         f"template <> struct Joints<1> {{ static constexpr int NQ = GATO_INDY7_NQ; "
         f"static constexpr double T[GATO_INDY7_NQ][GATO_JOINT_REC] = GATO_INDY7_JOINTS_INIT; }};",
+        f"template <> struct Joints<2> {{ static constexpr int NQ = GATO_INDY7_URDF_NQ; "
+        f"static constexpr double T[GATO_INDY7_URDF_NQ][GATO_JOINT_REC] = GATO_INDY7_URDF_JOINTS_INIT; }};",
         "}  // namespace gato_model",
         "#endif",
         "",

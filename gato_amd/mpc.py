@@ -45,6 +45,54 @@ def figure8_path(dt, A_x=0.4, A_z=0.4, offset=(0.0, 0.5, 0.6), period=6, cycles=
     return path
 
 
+class PlantSimulator:
+    """The MPC harness's plant advance (mpc_controller.py:187-213, rk4 of common.py:49-91) on the device.
+
+    ``model``: "iiwa14", "indy7" (the solver's GRiD model) or "indy7_urdf" (the plant the harness simulates with
+    pinocchio: indy7 with its links' centres of mass from the URDF). ``advance(x, xu, seconds)`` integrates the
+    states x [n, nx] (device tensor, in place) over the time the harness gives one control step: int(seconds / h)
+    RK4 sub-steps of h, sub-step i applying the control of knot min(int(i / (dt / h)), N - 1) of the trajectories xu
+    [n, traj], then, as the harness does with the remainder, one more sub-step each time the accumulated remainders
+    reach h. Returns the number of sub-steps run (the harness adds h per sub-step to its clock).
+    """
+
+    def __init__(self, model, knots, dt, h=1e-3, device="cuda:0"):
+        if model not in _lib.SIM_MODELS:
+            raise ValueError(f"unknown plant model {model!r}")
+        self.model, self.knots, self.dt, self.h = model, int(knots), float(dt), float(h)
+        nq = 7 if model == "iiwa14" else 6
+        self.nx, self.traj = 2 * nq, 3 * nq * int(knots) - nq
+        self.device = device
+        self.accumulated = 0.0
+        self.lib = _lib.load()
+
+    def substeps(self, x, xu, first, steps, f_ext=None):
+        """Sub-steps first .. first + steps - 1 (gato_simulate_plant); x [n, nx] in place."""
+        n = x.numel() // self.nx
+        x = _dev_f32(x, n * self.nx, "x", self.device)
+        xu = _dev_f32(xu, n * self.traj, "xu", self.device)
+        fe = None if f_ext is None else _dev_f32(f_ext, n * 6, "f_ext", self.device)
+        st = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        check(self.lib.gato_simulate_plant(_lib.SIM_MODELS[self.model], n, _ptr(x), _ptr(xu), self.knots,
+                                           None if fe is None else _ptr(fe), self.h, int(first), int(steps), self.dt,
+                                           st), "gato_simulate_plant")
+        return x
+
+    def advance(self, x, xu, seconds, f_ext=None):
+        nsteps = int(seconds / self.h)
+        if nsteps > 0:
+            self.substeps(x, xu, 0, nsteps, f_ext)
+        run = nsteps
+        rem = seconds % self.h
+        if rem > 1e-5:
+            self.accumulated += rem
+            if self.accumulated >= self.h:
+                self.accumulated = 0.0
+                self.substeps(x, xu, nsteps, 1, f_ext)
+                run += 1
+        return run
+
+
 class ForceEstimator:
     """The wrench estimator of examples/force_estimator.py with its state on the device, bound to a solver
     whose batch holds one wrench hypothesis per problem.

@@ -27,6 +27,9 @@ extern "C" {
 
 #define GATO_PLANT_IIWA14 0
 #define GATO_PLANT_INDY7 1
+/* indy7 with its links' centres of mass from the URDF, as the reference's MPC examples simulate it with pinocchio
+ * (common.py:49-91); the solver's indy7 model (GRiD's) places them at the joint origins. Plant simulation only. */
+#define GATO_PLANT_INDY7_URDF 2
 
 /* execution paths */
 #define GATO_MODE_PLAIN 0 /* one kernel per phase, host loop with a device->host sync per iteration */
@@ -258,6 +261,16 @@ int gato_select_hypothesis(gato_solver* s, const float* d_x_last, const float* d
  * Asynchronous. */
 int gato_simulate_rk4(gato_solver* s, float* d_x, const float* d_u, const float* d_f_ext, float h, int32_t steps,
                       void* stream);
+/* Plant advance of the MPC harness (mpc_controller.py:194-213, with rk4 of common.py:49-91) on the device for n
+ * independent systems: sub-steps i = first_substep .. first_substep + steps - 1 of RK4 with step h, sub-step i applying
+ * the control u_k of knot k = min(int(i / (knot_dt / h)), knots - 1) of the system's trajectory (the harness's offset
+ * rule, evaluated in double). d_x [n * nx] states (in place); d_xu [n * ((nx+nu) knots - nu)] trajectories in the
+ * solver's layout; d_f_ext [n * 6] wrench on the last link in its frame, or NULL for none. model: GATO_PLANT_IIWA14
+ * (the URDF's model, which is also the solver's), GATO_PLANT_INDY7 (the solver's model) or GATO_PLANT_INDY7_URDF (the
+ * plant the harness simulates). The harness's residual-time step (mpc_controller.py:206-213) is one more call with
+ * steps = 1 and first_substep = the sub-step count. Asynchronous. */
+int gato_simulate_plant(int32_t model, int32_t n, float* d_x, const float* d_xu, int32_t knots, const float* d_f_ext,
+                        double h, int32_t first_substep, int32_t steps, double knot_dt, void* stream);
 /* Reference window for every problem from a device-resident path d_path [npoints * 6]:
  * d_ref[b][k] = path[min(offset + k, npoints - 1)] (mpc_controller.py:226-229 tiles it on the host). Asynchronous. */
 int gato_window_reference(gato_solver* s, float* d_ref, const float* d_path, int32_t npoints, int32_t offset,

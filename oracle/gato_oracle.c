@@ -41,7 +41,7 @@ typedef struct {
     double qlo[MAXQ], qhi[MAXQ], vlo[MAXQ], vhi[MAXQ], ulo[MAXQ], uhi[MAXQ];
 } plant_t;
 
-static plant_t g_plants[2];
+static plant_t g_plants[3];
 static int g_init = 0;
 
 static void init_plant(plant_t* P, const double (*tab)[GATO_JOINT_REC], int nq, int indy) {
@@ -71,11 +71,12 @@ static const plant_t* get_plant(int plant) {
             if (!g_init) {
                 init_plant(&g_plants[0], gato_iiwa14_joints, GATO_IIWA14_NQ, 0);
                 init_plant(&g_plants[1], gato_indy7_joints, GATO_INDY7_NQ, 1);
+                init_plant(&g_plants[2], gato_indy7_urdf_joints, GATO_INDY7_URDF_NQ, 1);
                 g_init = 1;
             }
         }
     }
-    return &g_plants[plant == ORC_PLANT_INDY7 ? 1 : 0];
+    return &g_plants[plant == ORC_PLANT_INDY7 ? 1 : plant == ORC_PLANT_INDY7_URDF ? 2 : 0];
 }
 
 int orc_nq(int plant) { return get_plant(plant)->nq; }

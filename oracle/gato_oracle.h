@@ -11,6 +11,9 @@ extern "C" {
 
 #define ORC_PLANT_IIWA14 0
 #define ORC_PLANT_INDY7 1
+/* indy7 as the reference's MPC examples simulate it (pinocchio on the URDF, inertial origins kept): plant dynamics
+ * only (orc_rnea, orc_mass, orc_fd, orc_fk, orc_sim_forward); the solver's indy7 is ORC_PLANT_INDY7 */
+#define ORC_PLANT_INDY7_URDF 2
 
 #define ORC_EXIT_PCG0 0 /* reference rule: converged <=> PCG took 0 iterations (bsqp.cuh:149-163) */
 #define ORC_EXIT_KKT 1  /* converged <=> max(|stationarity|_inf, |defect|_inf) <= kkt_tol, per problem */

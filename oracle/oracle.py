@@ -12,7 +12,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "libgato_oracle.so"
 
-PLANTS = {"iiwa14": 0, "indy7": 1}
+PLANTS = {"iiwa14": 0, "indy7": 1, "indy7_urdf": 2}  # indy7_urdf: the simulated plant (URDF inertial origins kept)
 EXIT_PCG0, EXIT_KKT = 0, 1
 
 _dp = C.POINTER(C.c_double)

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 import torch
 
+from conftest import rel_err
 from gato_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -91,3 +92,51 @@ def test_force_estimator_follows_the_solver_selection(gpu, O):
         np.testing.assert_allclose(st["estimate"], ref.est, rtol=1e-5, atol=1e-5 * scale)
         assert st["radius"] == pytest.approx(ref.radius, rel=1e-12)
         x = x_next
+
+
+@pytest.mark.parametrize("model", ["iiwa14", "indy7", "indy7_urdf"])
+def test_plant_simulation_follows_the_harness(gpu, O, model):
+    # Sub-steps 7 .. 31 of 1 ms under per-knot controls (knot min(int(i / 10), N - 1), mpc_controller.py:197-201)
+    # against RK4 (common.py:49-91) on the oracle's forward dynamics of the same plant model, with and without
+    # wrenches; the simulated indy7 (URDF centres of mass) must differ from the solver's model.
+    from gato_amd.mpc import PlantSimulator
+
+    n, N, dt, h = 3, 8, 0.01, 1e-3
+    nq = 7 if model == "iiwa14" else 6
+    nx, st = 2 * nq, 3 * nq
+    rng = np.random.default_rng(41)
+    x0 = rng.uniform(-1, 1, (n, nx)).astype(np.float32)
+    xu = rng.normal(0, 5, (n, st * N - nq)).astype(np.float32)
+    fe = rng.normal(0, 2, (n, 6)).astype(np.float32)
+    sim = PlantSimulator(model, N, dt, h)
+    t = lambda a: torch.tensor(a, dtype=torch.float32, device="cuda:0").contiguous()
+    first, steps = 7, 25
+    outs = {}
+    for wrench in (None, fe):
+        x = t(x0)
+        sim.substeps(x, t(xu), first, steps, None if wrench is None else t(wrench))
+        outs[wrench is None] = x.cpu().numpy()
+        f = lambda a: a.astype(np.float64)
+        for b in range(n):
+            q, v = f(x0[b, :nq]), f(x0[b, nq:])
+            w = None if wrench is None else f(wrench[b])
+            for i in range(first, first + steps):
+                k = min(int(i / (dt / h)), N - 1)
+                u = f(xu[b, k * st + nx:k * st + nx + nq])
+                acc = lambda qq, vv: O.fd(model, qq, vv, u, w)
+                k1 = acc(q, v)
+                v2 = v + 0.5 * h * k1
+                k2 = acc(q + 0.5 * h * v, v2)
+                v3 = v + 0.5 * h * k2
+                k3 = acc(q + 0.5 * h * v2, v3)
+                v4 = v + h * k3
+                k4 = acc(q + h * v3, v4)
+                q, v = q + h / 6 * (v + 2 * v2 + 2 * v3 + v4), v + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+            assert rel_err(outs[wrench is None][b], np.concatenate([q, v])) < 2e-5, (b, wrench is None)
+    if model == "indy7_urdf":
+        x = t(x0)
+        PlantSimulator("indy7", N, dt, h).substeps(x, t(xu), first, steps)
+        assert rel_err(x.cpu().numpy(), outs[True]) > 1e-3
+    # the harness's clock: int(seconds / h) sub-steps, the remainders accumulated into one more sub-step
+    x = t(x0)
+    assert [sim.advance(x, t(xu), 0.0025) for _ in range(2)] == [2, 3]

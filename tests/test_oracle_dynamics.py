@@ -11,7 +11,7 @@ import pytest
 
 from conftest import ROOT, rel_err
 
-PLANTS = ["iiwa14", "indy7"]
+PLANTS = ["iiwa14", "indy7", "indy7_urdf"]  # indy7_urdf: the simulated plant (inertial origins kept)
 GOLDEN = Path(__file__).resolve().parent / "golden"
 
 
@@ -66,7 +66,7 @@ def test_frames_are_rotations(O, plant):
         assert abs(np.linalg.det(Ri) - 1) < 1e-12
 
 
-@pytest.mark.parametrize("plant", PLANTS)
+@pytest.mark.parametrize("plant", ["iiwa14", "indy7"])
 def test_inertias_equal_reference_generated_tables(plant):
     # GRiD's spatial inertias (the reference's model, iiwa14_grid.cuh h_XImats[252..503],
     # indy7_grid.cuh h_XImats[216..431]) vs the URDF-derived tables: catches a wrong parallel-axis
@@ -79,6 +79,28 @@ def test_inertias_equal_reference_generated_tables(plant):
         mine = np.block([[Ib, hx], [hx.T, m * np.eye(3)]])
         ref = np.array(grid[i]).reshape(6, 6).T
         np.testing.assert_allclose(mine, ref, atol=1e-12, err_msg=f"{plant} link {i}")
+
+
+def test_urdf_plant_is_the_solver_model_with_the_link_centres_of_mass():
+    # The simulated indy7 (pinocchio reads the URDF's <inertial><origin>, common.py:49-91) differs from the solver's
+    # GRiD model, which places every centre of mass at its joint origin, only in the first moments and the
+    # parallel-axis terms: h = m c and Ibar = Ibar_solver + m (|c|^2 1 - c c^T) with c the URDF centre of mass in the
+    # joint frame. The COMs are read from indy7.urdf as data (links 1-6, rpy 0 on every inertial origin, each link
+    # frame is its joint frame): catches an inertial origin dropped, taken in the wrong frame or with a sign error.
+    com = [(-0.00023749, -0.04310313, 0.13245396), (-0.29616699, 2.254e-05, 0.04483069),
+           (-0.16804016, 0.00021421, -0.07000383), (-0.00026847, -0.0709844, 0.07649128),
+           (-0.09796232, -0.00023114, 0.06445892), (8.147e-05, -0.00046556, 0.03079097)]
+    solver, urdf = table("indy7"), table("indy7_urdf")
+    assert len(solver) == len(urdf) == 6
+    for i, (a, b) in enumerate(zip(solver, urdf)):
+        np.testing.assert_array_equal(a[:13], b[:13])  # joint transforms and masses unchanged
+        np.testing.assert_array_equal(a[22:], b[22:])  # limits unchanged
+        m, c = a[12], np.array(com[i])
+        assert np.all(np.array(a[13:16]) == 0)
+        np.testing.assert_allclose(b[13:16], m * c, rtol=1e-12, atol=1e-15, err_msg=f"h, link {i + 1}")
+        pa = m * (c @ c * np.eye(3) - np.outer(c, c))
+        want = np.array(a[16:22]) + np.array([pa[0, 0], pa[1, 1], pa[2, 2], pa[0, 1], pa[0, 2], pa[1, 2]])
+        np.testing.assert_allclose(b[16:22], want, rtol=1e-12, atol=1e-15, err_msg=f"Ibar, link {i + 1}")
 
 
 @pytest.mark.parametrize("plant", PLANTS)
