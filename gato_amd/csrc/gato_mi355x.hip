@@ -325,32 +325,26 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
             pu[c] = hasu ? PU[c * NX] : 0.f;
         }
     }
-    // block-tridiagonal row times the padded LDS vector: 128-bit loads of the three 16-float blocks
-    auto blk = [&](const float* M, const float4* xb, float s) {
-        float xs[XP];
-#pragma unroll
-        for (int q = 0; q < XP / 4; q++) {
-            const float4 a = xb[q];
-            xs[4 * q] = a.x; xs[4 * q + 1] = a.y; xs[4 * q + 2] = a.z; xs[4 * q + 3] = a.w;
-        }
-#pragma unroll
-        for (int c = 0; c < NX; c++) s += M[c] * xs[c];
-        return s;
-    };
-    auto mv = [&](const float* M0, const float* M1, const float* M2, const float* x) {
-        const float4* xk = reinterpret_cast<const float4*>(x + (kk + 1) * XP);
-        float s = blk(M0, xk, 0.f);
-        s = blk(M1, xk - XP / 4, s);
-        return blk(M2, xk + XP / 4, s);
+    // block-tridiagonal row times a vector held one entry per lane (own: this knot's entry, 0 on padding lanes):
+    // the knot's own block from the row's lanes by DPP broadcast, the neighbours' entries (one 32-bit LDS load
+    // each) likewise; the terms are summed in the order diagonal, left, right block, column by column
+    const int ln = lane;  // 0 .. 15; padding lanes read the zero padding of the LDS vectors
+    auto mv = [&](const float* M0, const float* M1, const float* M2, float own, const float* x) {
+        const float xm = x[kk * XP + ln], xp = x[(kk + 2) * XP + ln];
+        float s = row_bcast_dot<NX>(0.f, own, M0);
+        s = row_bcast_dot<NX>(s, xm, M1);
+        return row_bcast_dot<NX>(s, xp, M2);
     };
     float x = val ? v.lam[kk * NX + l] : 0.f;
     __syncthreads();
     if (val) sp[(kk + 1) * XP + l] = x;
     __syncthreads();
-    float r = val ? v.gam[kk * NX + l] - mv(sd, sl, su, sp) : 0.f;
+    const float sx = mv(sd, sl, su, x, sp);
+    float r = val ? v.gam[kk * NX + l] - sx : 0.f;
     if (val) sr[(kk + 1) * XP + l] = r;
     __syncthreads();
-    float z = val ? mv(pd, pl, pu, sr) : 0.f;
+    const float pr0 = mv(pd, pl, pu, r, sr);
+    float z = val ? pr0 : 0.f;
     float p = z;
     int slot = 0;
     float rho = block_sum1(r * z, red + 32 * slot);
@@ -362,7 +356,8 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
         it++;
         if (val) sp[(kk + 1) * XP + l] = p;
         __syncthreads();
-        const float Ap = val ? mv(sd, sl, su, sp) : 0.f;
+        const float sp_ = mv(sd, sl, su, p, sp);
+        const float Ap = val ? sp_ : 0.f;
         // quotients by reciprocal and product, as the reference's fast-math build divides (-use_fast_math)
         const float alpha = rho * __builtin_amdgcn_rcpf(block_sum1(p * Ap, red + 32 * slot));
         slot ^= 1;
@@ -371,7 +366,8 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
         if (val) sr[(kk + 1) * XP + l] = r;
         __syncthreads();
         if (it >= a.max_pcg) break;  // x is final at the cap: its last preconditioner application is not needed
-        z = val ? mv(pd, pl, pu, sr) : 0.f;
+        const float pr = mv(pd, pl, pu, r, sr);
+        z = val ? pr : 0.f;
         const float rn = block_sum1(r * z, red + 32 * slot);
         slot ^= 1;
         if (fabsf(rn) < a.pcg_abs_tol + eps * rho0) break;

@@ -589,6 +589,38 @@ constexpr int DPP_QUAD_2301 = 0x4E;   // quad_perm [2,3,0,1]
 constexpr int DPP_ROW_HALF_MIRROR = 0x141;
 constexpr int DPP_ROW_MIRROR = 0x140;
 
+// s + sum_{c < NX} x_c * m[c] in c order, where x_c is x in lane c of this lane's 16-lane row: one
+// v_fmac_f32_dpp with row_newbcast:c per term, so a block row times a vector that is spread one entry per lane over
+// the row reads the vector from the row's lanes instead of from LDS. Bitwise equal to the chain s = fma(m[c], x_c, s).
+// Lanes 0 .. NX-1 of the row must be active. The s_nop before covers the hazard of a DPP read of a VGPR written by a
+// VALU instruction in the previous two wait states, the one after a DPP read of s right after the block.
+#define GATO_FMAC_BCAST(J, C) "v_fmac_f32_dpp %0, %1, %" #J " row_newbcast:" #C " row_mask:0xf bank_mask:0xf\n"
+template <int NX>
+__device__ __forceinline__ float row_bcast_dot(float s, float x, const float* m);
+template <>
+__device__ __forceinline__ float row_bcast_dot<14>(float s, float x, const float* m) {
+    asm volatile("s_nop 1\n" GATO_FMAC_BCAST(2, 0) GATO_FMAC_BCAST(3, 1) GATO_FMAC_BCAST(4, 2) GATO_FMAC_BCAST(5, 3)
+                 GATO_FMAC_BCAST(6, 4) GATO_FMAC_BCAST(7, 5) GATO_FMAC_BCAST(8, 6) GATO_FMAC_BCAST(9, 7)
+                 GATO_FMAC_BCAST(10, 8) GATO_FMAC_BCAST(11, 9) GATO_FMAC_BCAST(12, 10) GATO_FMAC_BCAST(13, 11)
+                 GATO_FMAC_BCAST(14, 12) GATO_FMAC_BCAST(15, 13) "s_nop 1\n"
+                 : "+v"(s)
+                 : "v"(x), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]),
+                   "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]));
+    return s;
+}
+template <>
+__device__ __forceinline__ float row_bcast_dot<12>(float s, float x, const float* m) {
+    asm volatile("s_nop 1\n" GATO_FMAC_BCAST(2, 0) GATO_FMAC_BCAST(3, 1) GATO_FMAC_BCAST(4, 2) GATO_FMAC_BCAST(5, 3)
+                 GATO_FMAC_BCAST(6, 4) GATO_FMAC_BCAST(7, 5) GATO_FMAC_BCAST(8, 6) GATO_FMAC_BCAST(9, 7)
+                 GATO_FMAC_BCAST(10, 8) GATO_FMAC_BCAST(11, 9) GATO_FMAC_BCAST(12, 10) GATO_FMAC_BCAST(13, 11)
+                 "s_nop 1\n"
+                 : "+v"(s)
+                 : "v"(x), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]),
+                   "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]));
+    return s;
+}
+#undef GATO_FMAC_BCAST
+
 template <typename S>
 __device__ __forceinline__ S wave_sum(S v) {
     v += dpp_f<DPP_QUAD_1032>(v);
