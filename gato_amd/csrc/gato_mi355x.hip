@@ -379,6 +379,153 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
     return it;
 }
 
+// Sum over a 1024-lane work-group (64 knot groups) with one barrier, as block_sum1 over 64 slots. red: 64 floats,
+// double-buffered by the caller.
+__device__ __forceinline__ float block_sum64(float v, float* red) {
+    v = row16_sum(v);
+    if ((threadIdx.x & 15) == 0) red[threadIdx.x >> 4] = v;
+    __syncthreads();
+    float s = row16_sum(red[threadIdx.x & 63]);
+    return (lane_f(s, 0) + lane_f(s, 16)) + (lane_f(s, 32) + lane_f(s, 48));
+}
+
+// LDS floats of phase_pcg_res: the problem's off-diagonal blocks of S, two padded vectors, two reduction buffers
+template <int PL>
+__host__ __device__ constexpr int pcg_res_lds_floats(int N) {
+    return N * Plant<PL>::NX * Plant<PL>::NX + 2 * (N + 2) * 16 + 128;
+}
+
+// The iteration of phase_pcg for 64 < N <= 128 with the problem held on its CU for the whole solve: 1024 lanes, knot
+// group g owning knots g and g + 64, the diagonal blocks of S and P in registers (lane l: row l), the off-diagonal
+// blocks of S (So, 100 KB for iiwa14 at N = 128) in LDS, nothing streamed from HBM per iteration. Every block row
+// times a vector uses DPP broadcasts within the knot group (row_bcast_dot), the neighbour knots' entries read as one
+// 32-bit LDS load per lane. S p sums the diagonal, left and right blocks in btd_row's order. P is applied in factored
+// form: the preconditioner's off-diagonal blocks are P_{k+1,k} = -P_{k+1,k+1} So_{k+1} P_{k,k} by construction
+// (schur_linsys.cuh:213-260, precond_offdiag), so with w = Pd r (block diagonal),
+//   (P r)_k = Pd_k r_k + P_{k,k-1} r_{k-1} + P_{k+1,k}^T r_{k+1} = Pd_k (r_k - So_k w_{k-1} - So_{k+1}^T w_{k+1}),
+// exactly in real arithmetic (Pd_k is symmetric); in fp32 it differs from the explicit product by rounding. Four
+// work-group barriers per iteration, as phase_pcg_reg.
+template <int PL>
+__device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, float* sm) {
+    constexpr int NX = Plant<PL>::NX, NB = NX * NX, XP = 16, NP = 2, G = 64;
+    View<PL> v(a, b);
+    const int N = a.N, tid = threadIdx.x, g = tid / KG, lane = tid % KG;
+    float* So = sm;
+    float* sp = So + N * NB;
+    float* sw = sp + (N + 2) * XP;
+    float* red = sw + (N + 2) * XP;  // 2 x 64 floats
+    {
+        const float4* src = reinterpret_cast<const float4*>(v.So);
+        float4* dst = reinterpret_cast<float4*>(So);
+        for (int i = tid; i < N * NB / 4; i += blockDim.x) dst[i] = src[i];
+        for (int i = tid; i < 2 * (N + 2) * XP; i += blockDim.x) sp[i] = 0.f;
+    }
+    const int l = lane < NX ? lane : 0;
+    bool val[NP];
+    int kk[NP];
+    float sd[NP][NX], pd[NP][NX];
+#pragma unroll
+    for (int s = 0; s < NP; s++) {
+        kk[s] = g + s * G < N ? g + s * G : 0;
+        val[s] = lane < NX && g + s * G < N;
+        const float* D = v.Sd + (size_t)kk[s] * NB + l * NX;
+        const float* PD = v.Pd + (size_t)kk[s] * NB + l * NX;
+#pragma unroll
+        for (int c = 0; c < NX; c++) {
+            sd[s][c] = D[c];
+            pd[s][c] = PD[c];
+        }
+    }
+    // the off-diagonal blocks of block row k times the vector y: row l of So_k (k > 0) with y_{k-1}, then column l
+    // of So_{k+1} (k < N-1) with y_{k+1}, starting from acc
+    auto offdiag = [&](int k, const float* y, float acc) {
+        float L[NX], U[NX];
+        const float* Lr = So + k * NB + l * NX;
+        const float* Uc = So + (k + 1 < N ? k + 1 : k) * NB + l;
+#pragma unroll
+        for (int c = 0; c < NX; c++) {
+            L[c] = k > 0 ? Lr[c] : 0.f;
+            U[c] = k < N - 1 ? Uc[c * NX] : 0.f;
+        }
+        acc = row_bcast_dot<NX>(acc, y[k * XP + lane], L);
+        return row_bcast_dot<NX>(acc, y[(k + 2) * XP + lane], U);
+    };
+    float x[NP], r[NP], p[NP], z[NP];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NP; s++) {
+        x[s] = val[s] ? v.lam[kk[s] * NX + l] : 0.f;
+        if (val[s]) sp[(kk[s] + 1) * XP + l] = x[s];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NP; s++) {
+        const float sx = offdiag(kk[s], sp, row_bcast_dot<NX>(0.f, x[s], sd[s]));
+        r[s] = val[s] ? v.gam[kk[s] * NX + l] - sx : 0.f;
+    }
+    // z = P r in factored form; w published for the neighbours, the rest per knot group
+    auto precond = [&](float* zz) {
+#pragma unroll
+        for (int s = 0; s < NP; s++) {
+            const float w = row_bcast_dot<NX>(0.f, r[s], pd[s]);
+            if (val[s]) sw[(kk[s] + 1) * XP + l] = w;
+        }
+        __syncthreads();
+        float loc = 0.f;
+#pragma unroll
+        for (int s = 0; s < NP; s++) {
+            const float t = offdiag(kk[s], sw, 0.f);
+            const float u = val[s] ? r[s] - t : 0.f;
+            const float zs = row_bcast_dot<NX>(0.f, u, pd[s]);
+            zz[s] = val[s] ? zs : 0.f;
+            loc += r[s] * zz[s];
+        }
+        return loc;
+    };
+    int slot = 0;
+    float rho = block_sum64(precond(z), red + 64 * slot);
+    slot ^= 1;
+#pragma unroll
+    for (int s = 0; s < NP; s++) p[s] = z[s];
+    if (fabsf(rho) < a.pcg_abs_tol) return 0;
+    const float rho0 = fabsf(rho);
+    int it = 0;
+    for (int i = 0; i < a.max_pcg; i++) {
+        it++;
+#pragma unroll
+        for (int s = 0; s < NP; s++)
+            if (val[s]) sp[(kk[s] + 1) * XP + l] = p[s];
+        __syncthreads();
+        float Ap[NP], loc = 0.f;
+#pragma unroll
+        for (int s = 0; s < NP; s++) {
+            const float a_ = offdiag(kk[s], sp, row_bcast_dot<NX>(0.f, p[s], sd[s]));
+            Ap[s] = val[s] ? a_ : 0.f;
+            loc += p[s] * Ap[s];
+        }
+        // quotients by reciprocal and product, as the reference's fast-math build divides (-use_fast_math)
+        const float alpha = rho * __builtin_amdgcn_rcpf(block_sum64(loc, red + 64 * slot));
+        slot ^= 1;
+#pragma unroll
+        for (int s = 0; s < NP; s++) {
+            x[s] += alpha * p[s];
+            r[s] -= alpha * Ap[s];
+        }
+        if (it >= a.max_pcg) break;  // x is final at the cap: its last preconditioner application is not needed
+        const float rn = block_sum64(precond(z), red + 64 * slot);
+        slot ^= 1;
+        if (fabsf(rn) < a.pcg_abs_tol + eps * rho0) break;
+        const float beta = rn * __builtin_amdgcn_rcpf(rho);
+        rho = rn;
+#pragma unroll
+        for (int s = 0; s < NP; s++) p[s] = z[s] + beta * p[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NP; s++)
+        if (val[s]) v.lam[kk[s] * NX + l] = x[s];
+    return it;
+}
+
 // Work-group sums of two values with one barrier, each summed exactly as block_sum1 sums one. red: 64 floats per
 // call, double-buffered by the caller like block_sum1's.
 __device__ __forceinline__ float2 block_sum2(float v1, float v2, float* red) {
@@ -832,6 +979,7 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
 template <int PL, int NPASS>
 __device__ __forceinline__ int pcg_dispatch(const Args& a, int b, float eps, float* sm) {
     if constexpr (NPASS == 1) return phase_pcg_reg<PL>(a, b, eps, sm);
+    else if constexpr (NPASS == 0) return phase_pcg_res<PL>(a, b, eps, sm);
     else return phase_pcg<PL, NPASS>(a, b, eps, sm);
 }
 
@@ -1205,6 +1353,13 @@ template <int PL, int NPASS>
 __global__ void __launch_bounds__(1024) GATO_TWO_WG_PER_CU k_pcg_dz_wide(Args a, int it) {
     static_assert(NPASS >= 2, "the register-resident PCG's reductions hold at most 32 knot groups");
     pcg_dz_body<PL, NPASS>(a, it);
+}
+
+// The graph path's step for 64 < N <= 128 with the PCG held on the CU (phase_pcg_res): 1024 lanes, the problem's
+// off-diagonal blocks of S in LDS (pcg_res_lds_floats), one work-group per CU.
+template <int PL>
+__global__ void __launch_bounds__(1024) GATO_TWO_WG_PER_CU k_pcg_dz_res(Args a, int it) {
+    pcg_dz_body<PL, 0>(a, it);
 }
 
 // The convergence bookkeeping (check_problem) for every problem of the batch in one work-group, the converged
@@ -2465,14 +2620,14 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     hipLaunchKernelGGL(gato::k_begin, dim3(sb), dim3(256), 0, st, a);
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
     launch_merit<PL>(B, blk, lds, st, a, 1, 0, 0);
-    // 64 < N <= 128: the PCG in 1024-thread work-groups, two knots per knot group (k_pcg_dz_wide)
+    // 64 < N <= 128: the PCG held on the CU in 1024-thread work-groups, two knots per knot group (k_pcg_dz_res)
     const bool wide = N > 64 && N <= 128;
-    const size_t lds_wide = std::max(lds, (size_t)(64 * gato::KG + 16) * sizeof(float));
+    const size_t lds_res = std::max(lds, (size_t)gato::pcg_res_lds_floats<PL>(N) * sizeof(float));
     for (int it = 0; it < iters; it++) {
         hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
         launch_schur<PL>(B, blk, lds, st, a);
         if (wide)
-            hipLaunchKernelGGL((gato::k_pcg_dz_wide<PL, 2>), dim3(B), dim3(1024), lds_wide, st, a, it);
+            hipLaunchKernelGGL((gato::k_pcg_dz_res<PL>), dim3(B), dim3(1024), lds_res, st, a, it);
         else
             launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
         launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 3, it);
@@ -3158,6 +3313,23 @@ int gato_plant_dims(int32_t plant, int32_t* nq, int32_t* nx) {
     return 0;
 }
 
+// k_pcg_dz_res holds a problem's off-diagonal Schur blocks in LDS (more than the default 64 KB of dynamic LDS at
+// N = 128): allow each instantiation its largest size once per process, before any launch or graph capture.
+static int allow_res_lds() {
+    static std::once_flag once;
+    static hipError_t err = hipSuccess;
+    std::call_once(once, [] {
+        err = hipFuncSetAttribute(reinterpret_cast<const void*>(&gato::k_pcg_dz_res<0>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  gato::pcg_res_lds_floats<0>(128) * (int)sizeof(float));
+        if (err == hipSuccess)
+            err = hipFuncSetAttribute(reinterpret_cast<const void*>(&gato::k_pcg_dz_res<1>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      gato::pcg_res_lds_floats<1>(128) * (int)sizeof(float));
+    });
+    return err == hipSuccess ? 0 : fail((int)err, std::string("hipFuncSetAttribute (PCG LDS): ") + hipGetErrorString(err));
+}
+
 int gato_create(gato_solver** out, int32_t plant, int32_t knots, int32_t batch, const gato_params* params) {
     if (!out || !params) return fail(GATO_E_ARG, "null argument");
     const int nq = plant_nq(plant);
@@ -3167,6 +3339,8 @@ int gato_create(gato_solver** out, int32_t plant, int32_t knots, int32_t batch, 
     if (params->max_sqp_iters < 1 || params->max_sqp_iters > 1000) return fail(GATO_E_ARG, "max_sqp_iters must be in [1, 1000]");
     if (params->mode < GATO_MODE_PLAIN || params->mode > GATO_MODE_GRAPH) return fail(GATO_E_ARG, "unknown mode");
     if (params->exit_rule != GATO_EXIT_PCG0 && params->exit_rule != GATO_EXIT_KKT) return fail(GATO_E_ARG, "unknown exit rule");
+    if (knots > 64)
+        if (int rc0 = allow_res_lds()) return rc0;
     gato_solver* s = new gato_solver();
     s->plant = plant;
     s->N = knots;
@@ -3885,6 +4059,8 @@ int gato_solve_noalloc(int32_t plant, int32_t knots, int32_t batch, const gato_p
         return fail(GATO_E_ARG, "gato_solve_noalloc: " + std::to_string(batch) + " problems of " + std::to_string(knots) +
                                     " knots need " + std::to_string(need) + " bytes of workspace, more than the static " +
                                     std::to_string((size_t)GATO_NOALLOC_SCRATCH_BYTES) + "-byte buffer holds");
+    if (knots > 64)
+        if (int rc0 = allow_res_lds()) return rc0;
     void* base = nullptr;
     int rc = scratch_address(&base);
     if (rc) return rc;
