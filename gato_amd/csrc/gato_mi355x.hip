@@ -34,6 +34,7 @@ namespace gato {
 template <typename S>
 struct ArgsT {
     int N, B, traj, max_sqp, max_pcg, exit_rule, adapt_rho;
+    int skip_po;  // the Schur phase leaves P's off-diagonal blocks out (a PCG that applies P in factored form)
     S dt;
     float kkt_tol, solve_ratio;
     S pcg_abs_tol;  // absolute part of the PCG exit test (1e-6 in the solve, pcg.cuh:27; tiny for the adjoint)
@@ -146,6 +147,7 @@ __device__ __forceinline__ void phase_schur(const ArgsT<S>& a, int b, S rho, S* 
         }
     }
     __syncthreads();
+    if (a.skip_po) return;
     for (int k = g; k < N - 1; k += G) {
         const int rr = lane < NX ? lane : 0;
         S phi[NX], prow[NX];
@@ -440,13 +442,16 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
     // of So_{k+1} (k < N-1) with y_{k+1}, starting from acc
     auto offdiag = [&](int k, const float* y, float acc) {
         float L[NX], U[NX];
-        const float* Lr = So + k * NB + l * NX;
+        const float2* Lr = reinterpret_cast<const float2*>(So + k * NB + l * NX);  // 8-byte aligned: NX is even
         const float* Uc = So + (k + 1 < N ? k + 1 : k) * NB + l;
 #pragma unroll
-        for (int c = 0; c < NX; c++) {
-            L[c] = k > 0 ? Lr[c] : 0.f;
-            U[c] = k < N - 1 ? Uc[c * NX] : 0.f;
+        for (int c = 0; c < NX / 2; c++) {
+            const float2 t = Lr[c];
+            L[2 * c] = k > 0 ? t.x : 0.f;
+            L[2 * c + 1] = k > 0 ? t.y : 0.f;
         }
+#pragma unroll
+        for (int c = 0; c < NX; c++) U[c] = k < N - 1 ? Uc[c * NX] : 0.f;
         acc = row_bcast_dot<NX>(acc, y[k * XP + lane], L);
         return row_bcast_dot<NX>(acc, y[(k + 2) * XP + lane], U);
     };
@@ -1088,8 +1093,9 @@ __global__ void GATO_PHASE_BOUNDS(S) k_merit(ArgsT<S> a, int nal, int which, int
     merit_body<PL, S>(a, nal, which, it);
 }
 
-// The same for N * NUM_ALPHAS <= 256: one thread per (step length, knot) item in a 256-thread work-group,
-// which leaves room for 256 VGPRs at two work-groups per CU, so the forward dynamics of an item do not spill.
+// The same in a 256-thread work-group (one thread per (step length, knot) item, looping when N * NUM_ALPHAS > 256),
+// which leaves room for 256 VGPRs at two work-groups per CU, so the forward dynamics of an item do not spill. Each
+// item and each step length's sum over knots are computed as in k_merit: the results are the same bit for bit.
 template <int PL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_merit_n(Args a, int nal, int which,
                                                                                          int it) {
@@ -1208,8 +1214,8 @@ __global__ void GATO_PHASE_BOUNDS(S) k_schur(ArgsT<S> a) {
     phase_schur<PL, S>(a, b, a.rho[b], sm);
 }
 
-// The same in 256-thread work-groups (N <= 32: at most two knots per knot group), with room for 256 VGPRs at two
-// work-groups per CU, so the Schur products do not spill.
+// The same in 256-thread work-groups (16 knot groups, each looping over N / 16 knots), with room for 256 VGPRs at
+// two work-groups per CU, so the Schur products do not spill; every knot's arithmetic is k_schur's.
 template <int PL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_schur_n(Args a) {
     extern __shared__ __align__(16) float sm[];
@@ -1360,6 +1366,18 @@ __global__ void __launch_bounds__(1024) GATO_TWO_WG_PER_CU k_pcg_dz_wide(Args a,
 template <int PL>
 __global__ void __launch_bounds__(1024) GATO_TWO_WG_PER_CU k_pcg_dz_res(Args a, int it) {
     pcg_dz_body<PL, 0>(a, it);
+}
+
+// phase_pcg_res alone (the PCG of k_pcg_dz_res, for the phase-level tests): lambda and the iteration count.
+template <int PL>
+__global__ void __launch_bounds__(1024) GATO_TWO_WG_PER_CU k_pcg_res(Args a, int it) {
+    extern __shared__ __align__(16) float sm[];
+    const int b = blockIdx.x;
+    if (stopped(a)) return;
+    const bool skip = (a.exit_rule == GATO_EXIT_KKT) ? !a.active[b] : a.conv[b];  // quirk Q9 (pcg.cuh:30-33)
+    int n = 0;
+    if (!skip) n = phase_pcg_res<PL>(a, b, a.tol[b], sm);
+    if (threadIdx.x == 0) a.st_pcg[it * a.B + b] = n;
 }
 
 // The convergence bookkeeping (check_problem) for every problem of the batch in one work-group, the converged
@@ -2500,7 +2518,7 @@ int npass(const gato_solver* s) { return (s->N + s->G - 1) / s->G; }
 
 template <int PL>
 void launch_schur(int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a) {
-    if (a.N <= 32 && blk.x >= 256)
+    if (blk.x >= 256)
         hipLaunchKernelGGL((gato::k_schur_n<PL>), dim3(B), dim3(256), lds, st, a);
     else
         hipLaunchKernelGGL((gato::k_schur<PL>), dim3(B), blk, lds, st, a);
@@ -2508,7 +2526,7 @@ void launch_schur(int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args&
 
 template <int PL>
 void launch_merit(int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a, int nal, int which, int it) {
-    if (a.N * gato::NUM_ALPHAS <= 256)
+    if (a.N * gato::NUM_ALPHAS <= 256 || blk.x >= 256)
         hipLaunchKernelGGL((gato::k_merit_n<PL>), dim3(B), dim3(256), lds, st, a, nal, which, it);
     else
         hipLaunchKernelGGL((gato::k_merit<PL>), dim3(B), blk, lds, st, a, nal, which, it);
@@ -2573,6 +2591,7 @@ void fill_args(gato_solver* s, float* d_xu, const float* d_xs, const float* d_re
     a.kkt_tol = s->prm.kkt_tol;
     a.solve_ratio = s->prm.solve_ratio;
     a.pcg_abs_tol = (float)gato::PCG_ABS_TOL;
+    a.skip_po = 0;
     a.cw = gato::CostW{s->prm.q_cost, s->prm.N_cost, s->prm.q_lim_cost, s->prm.vel_lim_cost, s->prm.ctrl_lim_cost,
                        s->prm.terminal_mode, s->prm.hess_mode};
     a.xu = d_xu;
@@ -2623,9 +2642,11 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     // 64 < N <= 128: the PCG held on the CU in 1024-thread work-groups, two knots per knot group (k_pcg_dz_res)
     const bool wide = N > 64 && N <= 128;
     const size_t lds_res = std::max(lds, (size_t)gato::pcg_res_lds_floats<PL>(N) * sizeof(float));
+    gato::Args as = a;  // k_pcg_dz_res applies P in factored form: its off-diagonal blocks are not formed
+    as.skip_po = wide ? 1 : 0;
     for (int it = 0; it < iters; it++) {
         hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
-        launch_schur<PL>(B, blk, lds, st, a);
+        launch_schur<PL>(B, blk, lds, st, as);
         if (wide)
             hipLaunchKernelGGL((gato::k_pcg_dz_res<PL>), dim3(B), dim3(1024), lds_res, st, a, it);
         else
@@ -3319,13 +3340,16 @@ static int allow_res_lds() {
     static std::once_flag once;
     static hipError_t err = hipSuccess;
     std::call_once(once, [] {
-        err = hipFuncSetAttribute(reinterpret_cast<const void*>(&gato::k_pcg_dz_res<0>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  gato::pcg_res_lds_floats<0>(128) * (int)sizeof(float));
-        if (err == hipSuccess)
-            err = hipFuncSetAttribute(reinterpret_cast<const void*>(&gato::k_pcg_dz_res<1>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      gato::pcg_res_lds_floats<1>(128) * (int)sizeof(float));
+        const void* fn[4] = {reinterpret_cast<const void*>(&gato::k_pcg_dz_res<0>),
+                             reinterpret_cast<const void*>(&gato::k_pcg_dz_res<1>),
+                             reinterpret_cast<const void*>(&gato::k_pcg_res<0>),
+                             reinterpret_cast<const void*>(&gato::k_pcg_res<1>)};
+        const int bytes[4] = {gato::pcg_res_lds_floats<0>(128) * (int)sizeof(float),
+                              gato::pcg_res_lds_floats<1>(128) * (int)sizeof(float),
+                              gato::pcg_res_lds_floats<0>(128) * (int)sizeof(float),
+                              gato::pcg_res_lds_floats<1>(128) * (int)sizeof(float)};
+        for (int i = 0; i < 4 && err == hipSuccess; i++)
+            err = hipFuncSetAttribute(fn[i], hipFuncAttributeMaxDynamicSharedMemorySize, bytes[i]);
     });
     return err == hipSuccess ? 0 : fail((int)err, std::string("hipFuncSetAttribute (PCG LDS): ") + hipGetErrorString(err));
 }
@@ -3756,7 +3780,8 @@ int gato_force_est_state(gato_force_est* e, double* h_state, void* stream) {
 int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_xs, const float* d_ref, float dt,
                      void* stream) {
     if (!s || !d_xu || !d_xs || !d_ref) return fail(GATO_E_ARG, "null argument");
-    if (phase < 0 || phase > 4) return fail(GATO_E_ARG, "unknown phase");
+    if (phase < 0 || phase > 5) return fail(GATO_E_ARG, "unknown phase");
+    if (phase == 5 && (s->N <= 64 || s->N > 128)) return fail(GATO_E_ARG, "phase 5 needs 64 < N <= 128");
     hipStream_t st = (hipStream_t)stream;
     fill_args(s, d_xu, d_xs, d_ref, dt);
     const dim3 blk(grid_threads(s));
@@ -3789,6 +3814,13 @@ int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_
             if (ii) launch_merit<0>(s->B, blk, lds, st, a, gato::NUM_ALPHAS, 3, 0);
             else launch_merit<1>(s->B, blk, lds, st, a, gato::NUM_ALPHAS, 3, 0);
             break;
+        case 5: {  // the graph path's PCG for 64 < N <= 128 (held on the CU, P in factored form)
+            const size_t lr = std::max(lds, (size_t)(ii ? gato::pcg_res_lds_floats<0>(s->N)
+                                                        : gato::pcg_res_lds_floats<1>(s->N)) * sizeof(float));
+            if (ii) hipLaunchKernelGGL(gato::k_pcg_res<0>, dim3(s->B), dim3(1024), lr, st, a, 0);
+            else hipLaunchKernelGGL(gato::k_pcg_res<1>, dim3(s->B), dim3(1024), lr, st, a, 0);
+            break;
+        }
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));

@@ -414,7 +414,8 @@ class Solver:
                                       None if d is None else _fptr(d)), "gato_set_state")
 
     def debug_phase(self, phase, xu, xs, ref, dt=None):
-        """Run one plain-path phase (0 KKT, 1 Schur, 2 PCG, 3 dz, 4 merit terms) on the workspace."""
+        """Run one plain-path phase (0 KKT, 1 Schur, 2 PCG, 3 dz, 4 merit terms) on the workspace; 5 is the graph
+        path's PCG for 64 < N <= 128 (held on the CU, preconditioner in factored form)."""
         self._to_dev(xu, self._xu, "xu")
         self._to_dev(xs, self._xs, "xs")
         self._to_dev(ref, self._ref, "ref")

@@ -319,7 +319,8 @@ int gato_last_kernel_time(gato_solver* s, double* avg_us, int32_t* launches);
 /* Extension (phase-level tests, no reference counterpart): run one phase of the plain path on
  * the solver's workspace for all B problems, with the solver's current lambda / rho.
  * phase: 0 KKT setup, 1 Schur complement + preconditioner, 2 PCG, 3 dz + KKT residual,
- *        4 merit terms at the 8 step lengths (no line search). */
+ *        4 merit terms at the 8 step lengths (no line search),
+ *        5 the PCG as the graph path runs it for 64 < N <= 128 (held on the CU, preconditioner in factored form). */
 int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_xs, const float* d_ref, float dt,
                      void* stream);
 /* Extension (phase-level tests): copy workspace array `name` ("A", "B", "c", "Qp", "Qv", "Rg", "q", "r",
