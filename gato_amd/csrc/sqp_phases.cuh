@@ -14,6 +14,8 @@
 // the position block of Q_k (nq x nq) is row-major, the velocity block and R_k are diagonal.
 // Q~_k = Q_k + rho diag(1_nq, 0) (linalg.cuh:85-96): its inverse is blkdiag(Qip, 1/Qv).
 #pragma once
+#include <utility>
+
 #include "dynamics.cuh"
 
 namespace gato {
@@ -36,6 +38,575 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// s + sum_{c < NX} x_c * m[c] in c order, where x_c is x in lane c of this lane's 16-lane row: one
+// v_fmac_f32_dpp with row_newbcast:c per term, so a block row times a vector that is spread one entry per lane over
+// the row reads the vector from the row's lanes instead of from LDS. Bitwise equal to the chain s = fma(m[c], x_c, s).
+// Lanes 0 .. NX-1 of the row must be active. The s_nop before covers the hazard of a DPP read of a VGPR written by a
+// VALU instruction in the previous two wait states, the one after a DPP read of s right after the block.
+#define GATO_FMAC_BCAST(J, C) "v_fmac_f32_dpp %0, %1, %" #J " row_newbcast:" #C " row_mask:0xf bank_mask:0xf\n"
+template <int NX>
+__device__ __forceinline__ float row_bcast_dot(float s, float x, const float* m);
+template <>
+__device__ __forceinline__ float row_bcast_dot<14>(float s, float x, const float* m) {
+    asm volatile("s_nop 1\n" GATO_FMAC_BCAST(2, 0) GATO_FMAC_BCAST(3, 1) GATO_FMAC_BCAST(4, 2) GATO_FMAC_BCAST(5, 3)
+                 GATO_FMAC_BCAST(6, 4) GATO_FMAC_BCAST(7, 5) GATO_FMAC_BCAST(8, 6) GATO_FMAC_BCAST(9, 7)
+                 GATO_FMAC_BCAST(10, 8) GATO_FMAC_BCAST(11, 9) GATO_FMAC_BCAST(12, 10) GATO_FMAC_BCAST(13, 11)
+                 GATO_FMAC_BCAST(14, 12) GATO_FMAC_BCAST(15, 13) "s_nop 1\n"
+                 : "+v"(s)
+                 : "v"(x), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]),
+                   "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]));
+    return s;
+}
+template <>
+__device__ __forceinline__ float row_bcast_dot<12>(float s, float x, const float* m) {
+    asm volatile("s_nop 1\n" GATO_FMAC_BCAST(2, 0) GATO_FMAC_BCAST(3, 1) GATO_FMAC_BCAST(4, 2) GATO_FMAC_BCAST(5, 3)
+                 GATO_FMAC_BCAST(6, 4) GATO_FMAC_BCAST(7, 5) GATO_FMAC_BCAST(8, 6) GATO_FMAC_BCAST(9, 7)
+                 GATO_FMAC_BCAST(10, 8) GATO_FMAC_BCAST(11, 9) GATO_FMAC_BCAST(12, 10) GATO_FMAC_BCAST(13, 11)
+                 "s_nop 1\n"
+                 : "+v"(s)
+                 : "v"(x), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]),
+                   "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]));
+    return s;
+}
+template <>
+__device__ __forceinline__ float row_bcast_dot<7>(float s, float x, const float* m) {
+    asm volatile("s_nop 1\n" GATO_FMAC_BCAST(2, 0) GATO_FMAC_BCAST(3, 1) GATO_FMAC_BCAST(4, 2) GATO_FMAC_BCAST(5, 3)
+                 GATO_FMAC_BCAST(6, 4) GATO_FMAC_BCAST(7, 5) GATO_FMAC_BCAST(8, 6) "s_nop 1\n"
+                 : "+v"(s)
+                 : "v"(x), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]));
+    return s;
+}
+template <>
+__device__ __forceinline__ float row_bcast_dot<6>(float s, float x, const float* m) {
+    asm volatile("s_nop 1\n" GATO_FMAC_BCAST(2, 0) GATO_FMAC_BCAST(3, 1) GATO_FMAC_BCAST(4, 2) GATO_FMAC_BCAST(5, 3)
+                 GATO_FMAC_BCAST(6, 4) GATO_FMAC_BCAST(7, 5) "s_nop 1\n"
+                 : "+v"(s)
+                 : "v"(x), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]));
+    return s;
+}
+// NC independent chains side by side in one asm block, chain i: s[i] + sum_{j < NJ} x_i(lane j) * m[j] in j order.
+// Interleaving hides the dependent-FMA latency that a single chain exposes when few waves share a SIMD; each chain
+// is summed exactly as row_bcast_dot sums it.
+template <int NJ, int NC>
+__device__ __forceinline__ void row_bcast_dots(float* s, const float* x, const float* m);
+template <>
+__device__ __forceinline__ void row_bcast_dots<6, 1>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %1, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %3 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %4 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %5 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %6 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %7 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0])
+                 : "v"(x[0]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<6, 2>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %2, %4 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %4 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %5 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %5 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %6 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %6 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %7 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %7 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %8 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %8 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %9 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %9 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0]), "+v"(s[1])
+                 : "v"(x[0]), "v"(x[1]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<6, 3>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %3, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2])
+                 : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<6, 4>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %4, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3])
+                 : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<7, 1>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %1, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %3 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %4 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %5 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %6 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %7 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %8 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0])
+                 : "v"(x[0]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<7, 2>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %2, %4 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %4 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %5 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %5 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %6 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %6 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %7 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %7 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %8 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %8 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %9 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %9 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %10 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %10 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0]), "+v"(s[1])
+                 : "v"(x[0]), "v"(x[1]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<7, 3>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %3, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2])
+                 : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<7, 4>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %4, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3])
+                 : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<12, 1>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %1, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %3 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %4 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %5 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %6 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %7 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %8 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %9 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %10 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %11 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %12 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %13 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0])
+                 : "v"(x[0]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<12, 2>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %2, %4 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %4 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %5 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %5 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %6 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %6 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %7 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %7 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %8 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %8 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %9 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %9 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %10 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %10 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %11 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %11 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %12 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %12 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %13 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %13 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %14 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %14 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %15 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %15 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0]), "+v"(s[1])
+                 : "v"(x[0]), "v"(x[1]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<12, 3>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %3, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2])
+                 : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<12, 4>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %4, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %15 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %15 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %15 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %15 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %16 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %16 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %16 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %16 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %18 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %18 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %18 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %18 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %19 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %19 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %19 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %19 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3])
+                 : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<14, 1>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %1, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %3 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %4 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %5 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %6 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %7 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %8 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %9 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %10 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %11 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %12 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %13 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %14 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %1, %15 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0])
+                 : "v"(x[0]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<14, 2>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %2, %4 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %4 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %5 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %5 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %6 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %6 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %7 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %7 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %8 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %8 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %9 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %9 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %10 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %10 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %11 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %11 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %12 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %12 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %13 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %13 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %14 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %14 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %15 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %15 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %16 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %16 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %2, %17 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %3, %17 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0]), "+v"(s[1])
+                 : "v"(x[0]), "v"(x[1]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<14, 3>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %3, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %18 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %18 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %18 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %3, %19 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %4, %19 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %5, %19 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2])
+                 : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]));
+}
+template <>
+__device__ __forceinline__ void row_bcast_dots<14, 4>(float* s, const float* x, const float* m) {
+    asm volatile("s_nop 1\n"
+                 "v_fmac_f32_dpp %0, %4, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %9 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %10 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %11 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %12 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %13 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %14 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %15 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %15 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %15 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %15 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %16 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %16 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %16 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %16 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %18 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %18 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %18 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %18 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %19 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %19 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %19 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %19 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %20 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %20 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %20 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %20 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %0, %4, %21 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %1, %5, %21 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %2, %6, %21 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+                 "v_fmac_f32_dpp %3, %7, %21 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+                 "s_nop 1\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3])
+                 : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]));
+}
+// all NR chains of a block product, four at a time
+template <int NJ, int NR>
+__device__ __forceinline__ void row_bcast_dots_all(float* s, const float* x, const float* m) {
+    constexpr int Q = NR / 4, R = NR % 4;
+#pragma unroll
+    for (int i = 0; i < Q; i++) row_bcast_dots<NJ, 4>(s + 4 * i, x + 4 * i, m);
+    if constexpr (R > 0) row_bcast_dots<NJ, R>(s + 4 * Q, x + 4 * Q, m);
+}
+// x in lane P of this lane's 16-lane row (a DPP move the compiler sees, so it inserts the hazard waits itself)
+template <int P>
+__device__ __forceinline__ float row_lane(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x150 + P, 0xF, 0xF,
+                                                                 false));
+}
+#undef GATO_FMAC_BCAST
 
 // max and |x| in the scalar type (fmaxf / fabsf for fp32)
 __device__ __forceinline__ float smax(float a, float b) { return fmaxf(a, b); }
@@ -248,8 +819,36 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const Cos
 // In-place Gauss-Jordan inverse without pivoting of an n x n matrix held one row per lane
 // (lane r < n owns row r in registers); the inverse overwrites the rows. Equivalent in exact
 // arithmetic to the reference's elimination on [A | I] (linalg.cuh:365-519). piv: n floats of LDS.
+// fp32: the pivot row reaches the other lanes of the knot group by DPP row broadcasts (row_lane) instead of an LDS
+// round trip and two wave syncs per pivot. Every lane forms the scaled pivot row entry piv_c = row_p[c] * (1/row_p[p])
+// itself, the same product the pivot lane stores, and updates row[c] = fma(-f, piv_c, row[c]) as the LDS form does:
+// the results are the same bit for bit. Lanes past NX update rows nobody reads (they are never broadcast sources).
+template <int NX, int P>
+__device__ __forceinline__ void gj_pivot_dpp(int lane, float* row) {
+    const float inv = rcp(row_lane<P>(row[P]));  // fast-math reciprocal of the pivot
+    const float f = row[P];
+    const bool pv = lane == P;
+#pragma unroll
+    for (int c = 0; c < NX; c++) {
+        if (c == P) {
+            row[c] = pv ? inv : fmaf(-f, inv, 0.f);
+        } else {
+            const float pc = row_lane<P>(row[c]) * inv;
+            row[c] = pv ? pc : fmaf(-f, pc, row[c]);
+        }
+    }
+}
+template <int NX, int... P>
+__device__ __forceinline__ void gj_dpp(int lane, float* row, std::integer_sequence<int, P...>) {
+    (gj_pivot_dpp<NX, P>(lane, row), ...);
+}
+
 template <int NX, typename S>
 __device__ __forceinline__ void gj_inplace(int lane, S* row, S* piv) {
+    if constexpr (std::is_same_v<S, float>) {
+        gj_dpp<NX>(lane, row, std::make_integer_sequence<int, NX>{});
+        return;
+    }
 #pragma unroll
     for (int p = 0; p < NX; p++) {
         if (lane == p) {
@@ -318,6 +917,70 @@ struct SchurTile {
     static constexpr int SIZE = PIV_OFF + 2 * NX;            // floats per knot group
 };
 
+// schur_row in fp32 with every block product on DPP row broadcasts (row_bcast_dot) instead of the knot group's LDS
+// tile: lane l is the broadcast source of row l of Q~_k^-1 (for phi_k = A Q~_k^-1), of A[m][l] and B[m][l] (for the
+// products with A^T and B^T) and of the Gauss-Jordan pivot rows; each sum runs in the LDS form's order, so the results
+// are the same bit for bit, without the tile's bank conflicts and wave syncs.
+template <int PL>
+__device__ __forceinline__ void schur_row_dpp(int lane, float rho, const float* A, const float* Bm, const float* Qip0,
+                                              const float* Qiv0, const float* Ri0, const float* Qip1,
+                                              const float* Qiv1, const float* q0, const float* q1, const float* r0,
+                                              const float* c1, float* Sd1, float* So1, float* Pd1, float* g1,
+                                              float* phi, float* prow) {
+    using P = Plant<PL>;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
+    const int r = lane < NX ? lane : 0, rq = lane < NQ ? lane : 0, ru = lane < NU ? lane : 0;
+    float Ar[NX], Qr[NQ];
+#pragma unroll
+    for (int l = 0; l < NX; l++) Ar[l] = A[r * NX + l];
+#pragma unroll
+    for (int c = 0; c < NQ; c++) Qr[c] = Qip0[rq * NQ + c];
+#pragma unroll
+    for (int c = 0; c < NQ; c++) {
+        phi[c] = 0.f;
+        phi[NQ + c] = Ar[NQ + c] * Qiv0[c];
+    }
+    row_bcast_dots_all<NQ, NQ>(phi, Qr, Ar);
+    float brr[NU];
+#pragma unroll
+    for (int c = 0; c < NU; c++) brr[c] = Bm[r * NU + c] * Ri0[c];
+    float th[NX];
+    qinv_row<PL, float>(Qip1, Qiv1, r, th);
+    float g = c1[r];
+#pragma unroll
+    for (int m = 0; m < NX; m++) {
+        g -= th[m] * q1[m];
+        g += phi[m] * q0[m];
+    }
+    float xa[NX], xb[NX];  // this lane's entries of A[m][.] and B[m][.] as broadcast sources
+#pragma unroll
+    for (int m = 0; m < NX; m++) {
+        xa[m] = A[m * NX + r];
+        xb[m] = Bm[m * NU + ru];
+    }
+    row_bcast_dots_all<NX, NX>(th, xa, phi);
+    row_bcast_dots_all<NU, NX>(th, xb, brr);
+#pragma unroll
+    for (int l = 0; l < NU; l++) g += brr[l] * r0[l];
+    if (lane < NX) {
+        g1[r] = g;
+#pragma unroll
+        for (int m = 0; m < NX; m++) {
+            Sd1[r * NX + m] = -th[m];
+            So1[r * NX + m] = phi[m];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NX; m++) prow[m] = th[m] + ((m == r && m < NQ) ? rho : 0.f);
+    gj_inplace<NX>(lane, prow, static_cast<float*>(nullptr));
+#pragma unroll
+    for (int m = 0; m < NX; m++) prow[m] = -prow[m];
+    if (lane < NX) {
+#pragma unroll
+        for (int m = 0; m < NX; m++) Pd1[r * NX + m] = prow[m];
+    }
+}
+
 template <int PL, typename S>
 __device__ __forceinline__ void schur_row(int lane, S rho, const S* A, const S* Bm, const S* Qip0,
                                           const S* Qiv0, const S* Ri0, const S* Qip1, const S* Qiv1,
@@ -327,6 +990,10 @@ __device__ __forceinline__ void schur_row(int lane, S rho, const S* A, const S* 
     using P = Plant<PL>;
     using T = SchurTile<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
+    if constexpr (std::is_same_v<S, float>) {
+        schur_row_dpp<PL>(lane, rho, A, Bm, Qip0, Qiv0, Ri0, Qip1, Qiv1, q0, q1, r0, c1, Sd1, So1, Pd1, g1, phi, prow);
+        return;
+    }
     S* As = tile + T::A_OFF;
     S* Bs = tile + T::B_OFF;
     S* Qs = tile + T::Q_OFF;
@@ -423,6 +1090,23 @@ __device__ __forceinline__ void precond_offdiag(int lane, const S* phi, const S*
                                                 S* tile, S* Po1) {
     constexpr int NX = Plant<PL>::NX, AS = SchurTile<PL>::AS;
     const int r = lane < NX ? lane : 0;
+    if constexpr (std::is_same_v<S, float>) {
+        // DPP form, as schur_row_dpp: lane m broadcasts row m of P_{k,k}, then row m of t = phi_k P_{k,k}
+        float pk[NX], t[NX], o[NX];
+#pragma unroll
+        for (int c = 0; c < NX; c++) {
+            pk[c] = Pd0[r * NX + c];
+            t[c] = 0.f;
+            o[c] = 0.f;
+        }
+        row_bcast_dots_all<NX, NX>(t, pk, phi);
+        row_bcast_dots_all<NX, NX>(o, t, prow);
+        if (lane < NX) {
+#pragma unroll
+            for (int c = 0; c < NX; c++) Po1[lane * NX + c] = -o[c];
+        }
+        return;
+    }
     if (lane < NX) {
 #pragma unroll
         for (int c = 0; c < NX; c++) tile[r * AS + c] = Pd0[r * NX + c];
@@ -588,38 +1272,6 @@ constexpr int DPP_QUAD_1032 = 0xB1;   // quad_perm [1,0,3,2]
 constexpr int DPP_QUAD_2301 = 0x4E;   // quad_perm [2,3,0,1]
 constexpr int DPP_ROW_HALF_MIRROR = 0x141;
 constexpr int DPP_ROW_MIRROR = 0x140;
-
-// s + sum_{c < NX} x_c * m[c] in c order, where x_c is x in lane c of this lane's 16-lane row: one
-// v_fmac_f32_dpp with row_newbcast:c per term, so a block row times a vector that is spread one entry per lane over
-// the row reads the vector from the row's lanes instead of from LDS. Bitwise equal to the chain s = fma(m[c], x_c, s).
-// Lanes 0 .. NX-1 of the row must be active. The s_nop before covers the hazard of a DPP read of a VGPR written by a
-// VALU instruction in the previous two wait states, the one after a DPP read of s right after the block.
-#define GATO_FMAC_BCAST(J, C) "v_fmac_f32_dpp %0, %1, %" #J " row_newbcast:" #C " row_mask:0xf bank_mask:0xf\n"
-template <int NX>
-__device__ __forceinline__ float row_bcast_dot(float s, float x, const float* m);
-template <>
-__device__ __forceinline__ float row_bcast_dot<14>(float s, float x, const float* m) {
-    asm volatile("s_nop 1\n" GATO_FMAC_BCAST(2, 0) GATO_FMAC_BCAST(3, 1) GATO_FMAC_BCAST(4, 2) GATO_FMAC_BCAST(5, 3)
-                 GATO_FMAC_BCAST(6, 4) GATO_FMAC_BCAST(7, 5) GATO_FMAC_BCAST(8, 6) GATO_FMAC_BCAST(9, 7)
-                 GATO_FMAC_BCAST(10, 8) GATO_FMAC_BCAST(11, 9) GATO_FMAC_BCAST(12, 10) GATO_FMAC_BCAST(13, 11)
-                 GATO_FMAC_BCAST(14, 12) GATO_FMAC_BCAST(15, 13) "s_nop 1\n"
-                 : "+v"(s)
-                 : "v"(x), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]),
-                   "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]));
-    return s;
-}
-template <>
-__device__ __forceinline__ float row_bcast_dot<12>(float s, float x, const float* m) {
-    asm volatile("s_nop 1\n" GATO_FMAC_BCAST(2, 0) GATO_FMAC_BCAST(3, 1) GATO_FMAC_BCAST(4, 2) GATO_FMAC_BCAST(5, 3)
-                 GATO_FMAC_BCAST(6, 4) GATO_FMAC_BCAST(7, 5) GATO_FMAC_BCAST(8, 6) GATO_FMAC_BCAST(9, 7)
-                 GATO_FMAC_BCAST(10, 8) GATO_FMAC_BCAST(11, 9) GATO_FMAC_BCAST(12, 10) GATO_FMAC_BCAST(13, 11)
-                 "s_nop 1\n"
-                 : "+v"(s)
-                 : "v"(x), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]),
-                   "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]));
-    return s;
-}
-#undef GATO_FMAC_BCAST
 
 template <typename S>
 __device__ __forceinline__ S wave_sum(S v) {
