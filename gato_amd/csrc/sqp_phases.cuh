@@ -917,6 +917,24 @@ struct SchurTile {
     static constexpr int SIZE = PIV_OFF + 2 * NX;            // floats per knot group
 };
 
+// Store an NX x NX block whose row r is held by lane r (r < NX) of the knot group, times sgn, to dst (16-byte aligned,
+// row-major): the rows go through the knot group's LDS stage (NX * NX floats, 16-byte aligned) and leave as contiguous
+// 128-bit stores, instead of NX scattered 32-bit stores per lane at a stride of NX floats.
+template <int NX>
+__device__ __forceinline__ void store_block(int lane, const float* row, float sgn, float* stage, float* dst) {
+    static_assert((NX * NX) % 4 == 0, "whole float4s");
+    if (lane < NX) {
+#pragma unroll
+        for (int c = 0; c < NX; c++) stage[lane * NX + c] = sgn * row[c];
+    }
+    wave_sync();
+    const float4* s4 = reinterpret_cast<const float4*>(stage);
+    float4* d4 = reinterpret_cast<float4*>(dst);
+#pragma unroll
+    for (int i = lane; i < NX * NX / 4; i += 16) d4[i] = s4[i];
+    wave_sync();
+}
+
 // schur_row in fp32 with every block product on DPP row broadcasts (row_bcast_dot) instead of the knot group's LDS
 // tile: lane l is the broadcast source of row l of Q~_k^-1 (for phi_k = A Q~_k^-1), of A[m][l] and B[m][l] (for the
 // products with A^T and B^T) and of the Gauss-Jordan pivot rows; each sum runs in the LDS form's order, so the results
@@ -926,7 +944,7 @@ __device__ __forceinline__ void schur_row_dpp(int lane, float rho, const float* 
                                               const float* Qiv0, const float* Ri0, const float* Qip1,
                                               const float* Qiv1, const float* q0, const float* q1, const float* r0,
                                               const float* c1, float* Sd1, float* So1, float* Pd1, float* g1,
-                                              float* phi, float* prow) {
+                                              float* phi, float* prow, float* stage) {
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
     const int r = lane < NX ? lane : 0, rq = lane < NQ ? lane : 0, ru = lane < NU ? lane : 0;
@@ -962,23 +980,15 @@ __device__ __forceinline__ void schur_row_dpp(int lane, float rho, const float* 
     row_bcast_dots_all<NU, NX>(th, xb, brr);
 #pragma unroll
     for (int l = 0; l < NU; l++) g += brr[l] * r0[l];
-    if (lane < NX) {
-        g1[r] = g;
-#pragma unroll
-        for (int m = 0; m < NX; m++) {
-            Sd1[r * NX + m] = -th[m];
-            So1[r * NX + m] = phi[m];
-        }
-    }
+    if (lane < NX) g1[r] = g;
+    store_block<NX>(lane, th, -1.f, stage, Sd1);
+    store_block<NX>(lane, phi, 1.f, stage, So1);
 #pragma unroll
     for (int m = 0; m < NX; m++) prow[m] = th[m] + ((m == r && m < NQ) ? rho : 0.f);
     gj_inplace<NX>(lane, prow, static_cast<float*>(nullptr));
 #pragma unroll
     for (int m = 0; m < NX; m++) prow[m] = -prow[m];
-    if (lane < NX) {
-#pragma unroll
-        for (int m = 0; m < NX; m++) Pd1[r * NX + m] = prow[m];
-    }
+    store_block<NX>(lane, prow, 1.f, stage, Pd1);
 }
 
 template <int PL, typename S>
@@ -991,7 +1001,8 @@ __device__ __forceinline__ void schur_row(int lane, S rho, const S* A, const S* 
     using T = SchurTile<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
     if constexpr (std::is_same_v<S, float>) {
-        schur_row_dpp<PL>(lane, rho, A, Bm, Qip0, Qiv0, Ri0, Qip1, Qiv1, q0, q1, r0, c1, Sd1, So1, Pd1, g1, phi, prow);
+        schur_row_dpp<PL>(lane, rho, A, Bm, Qip0, Qiv0, Ri0, Qip1, Qiv1, q0, q1, r0, c1, Sd1, So1, Pd1, g1, phi, prow,
+                          tile);
         return;
     }
     S* As = tile + T::A_OFF;
@@ -1101,10 +1112,7 @@ __device__ __forceinline__ void precond_offdiag(int lane, const S* phi, const S*
         }
         row_bcast_dots_all<NX, NX>(t, pk, phi);
         row_bcast_dots_all<NX, NX>(o, t, prow);
-        if (lane < NX) {
-#pragma unroll
-            for (int c = 0; c < NX; c++) Po1[lane * NX + c] = -o[c];
-        }
+        store_block<NX>(lane, o, -1.f, tile, Po1);
         return;
     }
     if (lane < NX) {
