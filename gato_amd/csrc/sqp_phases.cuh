@@ -69,40 +69,11 @@ __device__ __forceinline__ float row_bcast_dot<12>(float s, float x, const float
                    "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]));
     return s;
 }
-template <>
-__device__ __forceinline__ float row_bcast_dot<7>(float s, float x, const float* m) {
-    asm volatile("s_nop 1\n" GATO_FMAC_BCAST(2, 0) GATO_FMAC_BCAST(3, 1) GATO_FMAC_BCAST(4, 2) GATO_FMAC_BCAST(5, 3)
-                 GATO_FMAC_BCAST(6, 4) GATO_FMAC_BCAST(7, 5) GATO_FMAC_BCAST(8, 6) "s_nop 1\n"
-                 : "+v"(s)
-                 : "v"(x), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]));
-    return s;
-}
-template <>
-__device__ __forceinline__ float row_bcast_dot<6>(float s, float x, const float* m) {
-    asm volatile("s_nop 1\n" GATO_FMAC_BCAST(2, 0) GATO_FMAC_BCAST(3, 1) GATO_FMAC_BCAST(4, 2) GATO_FMAC_BCAST(5, 3)
-                 GATO_FMAC_BCAST(6, 4) GATO_FMAC_BCAST(7, 5) "s_nop 1\n"
-                 : "+v"(s)
-                 : "v"(x), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]));
-    return s;
-}
 // NC independent chains side by side in one asm block, chain i: s[i] + sum_{j < NJ} x_i(lane j) * m[j] in j order.
 // Interleaving hides the dependent-FMA latency that a single chain exposes when few waves share a SIMD; each chain
 // is summed exactly as row_bcast_dot sums it.
 template <int NJ, int NC>
 __device__ __forceinline__ void row_bcast_dots(float* s, const float* x, const float* m);
-template <>
-__device__ __forceinline__ void row_bcast_dots<6, 1>(float* s, const float* x, const float* m) {
-    asm volatile("s_nop 1\n"
-                 "v_fmac_f32_dpp %0, %1, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %3 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %4 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %5 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %6 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %7 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "s_nop 1\n"
-                 : "+v"(s[0])
-                 : "v"(x[0]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]));
-}
 template <>
 __device__ __forceinline__ void row_bcast_dots<6, 2>(float* s, const float* x, const float* m) {
     asm volatile("s_nop 1\n"
@@ -121,31 +92,6 @@ __device__ __forceinline__ void row_bcast_dots<6, 2>(float* s, const float* x, c
                  "s_nop 1\n"
                  : "+v"(s[0]), "+v"(s[1])
                  : "v"(x[0]), "v"(x[1]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]));
-}
-template <>
-__device__ __forceinline__ void row_bcast_dots<6, 3>(float* s, const float* x, const float* m) {
-    asm volatile("s_nop 1\n"
-                 "v_fmac_f32_dpp %0, %3, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "s_nop 1\n"
-                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2])
-                 : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]));
 }
 template <>
 __device__ __forceinline__ void row_bcast_dots<6, 4>(float* s, const float* x, const float* m) {
@@ -177,20 +123,6 @@ __device__ __forceinline__ void row_bcast_dots<6, 4>(float* s, const float* x, c
                  "s_nop 1\n"
                  : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3])
                  : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]));
-}
-template <>
-__device__ __forceinline__ void row_bcast_dots<7, 1>(float* s, const float* x, const float* m) {
-    asm volatile("s_nop 1\n"
-                 "v_fmac_f32_dpp %0, %1, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %3 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %4 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %5 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %6 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %7 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %8 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
-                 "s_nop 1\n"
-                 : "+v"(s[0])
-                 : "v"(x[0]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]));
 }
 template <>
 __device__ __forceinline__ void row_bcast_dots<7, 2>(float* s, const float* x, const float* m) {
@@ -277,99 +209,6 @@ __device__ __forceinline__ void row_bcast_dots<7, 4>(float* s, const float* x, c
                  : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]));
 }
 template <>
-__device__ __forceinline__ void row_bcast_dots<12, 1>(float* s, const float* x, const float* m) {
-    asm volatile("s_nop 1\n"
-                 "v_fmac_f32_dpp %0, %1, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %3 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %4 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %5 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %6 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %7 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %8 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %9 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %10 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %11 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %12 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %13 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
-                 "s_nop 1\n"
-                 : "+v"(s[0])
-                 : "v"(x[0]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]));
-}
-template <>
-__device__ __forceinline__ void row_bcast_dots<12, 2>(float* s, const float* x, const float* m) {
-    asm volatile("s_nop 1\n"
-                 "v_fmac_f32_dpp %0, %2, %4 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %3, %4 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %2, %5 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %3, %5 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %2, %6 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %3, %6 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %2, %7 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %3, %7 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %2, %8 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %3, %8 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %2, %9 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %3, %9 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %2, %10 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %3, %10 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %2, %11 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %3, %11 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %2, %12 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %3, %12 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %2, %13 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %3, %13 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %2, %14 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %3, %14 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %2, %15 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %3, %15 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
-                 "s_nop 1\n"
-                 : "+v"(s[0]), "+v"(s[1])
-                 : "v"(x[0]), "v"(x[1]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]));
-}
-template <>
-__device__ __forceinline__ void row_bcast_dots<12, 3>(float* s, const float* x, const float* m) {
-    asm volatile("s_nop 1\n"
-                 "v_fmac_f32_dpp %0, %3, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
-                 "s_nop 1\n"
-                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2])
-                 : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]));
-}
-template <>
 __device__ __forceinline__ void row_bcast_dots<12, 4>(float* s, const float* x, const float* m) {
     asm volatile("s_nop 1\n"
                  "v_fmac_f32_dpp %0, %4, %8 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
@@ -425,27 +264,6 @@ __device__ __forceinline__ void row_bcast_dots<12, 4>(float* s, const float* x, 
                  : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]));
 }
 template <>
-__device__ __forceinline__ void row_bcast_dots<14, 1>(float* s, const float* x, const float* m) {
-    asm volatile("s_nop 1\n"
-                 "v_fmac_f32_dpp %0, %1, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %3 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %4 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %5 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %6 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %7 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %8 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %9 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %10 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %11 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %12 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %13 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %14 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %1, %15 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
-                 "s_nop 1\n"
-                 : "+v"(s[0])
-                 : "v"(x[0]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]));
-}
-template <>
 __device__ __forceinline__ void row_bcast_dots<14, 2>(float* s, const float* x, const float* m) {
     asm volatile("s_nop 1\n"
                  "v_fmac_f32_dpp %0, %2, %4 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
@@ -479,55 +297,6 @@ __device__ __forceinline__ void row_bcast_dots<14, 2>(float* s, const float* x, 
                  "s_nop 1\n"
                  : "+v"(s[0]), "+v"(s[1])
                  : "v"(x[0]), "v"(x[1]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]));
-}
-template <>
-__device__ __forceinline__ void row_bcast_dots<14, 3>(float* s, const float* x, const float* m) {
-    asm volatile("s_nop 1\n"
-                 "v_fmac_f32_dpp %0, %3, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %9 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %10 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %11 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %12 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %13 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %14 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %15 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %16 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %18 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %18 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %18 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %0, %3, %19 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %1, %4, %19 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
-                 "v_fmac_f32_dpp %2, %5, %19 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
-                 "s_nop 1\n"
-                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2])
-                 : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]));
 }
 template <>
 __device__ __forceinline__ void row_bcast_dots<14, 4>(float* s, const float* x, const float* m) {
