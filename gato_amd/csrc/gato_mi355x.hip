@@ -293,7 +293,7 @@ __device__ __forceinline__ float block_sum1(float v, float* red) {
 }
 
 // Same iteration as phase_pcg with the lane's rows of S and P held in registers for the whole
-// solve (one knot per knot group: N <= 32) and four work-group barriers per iteration.
+// solve (one knot per knot group: N <= 32) and two work-group barriers per iteration (the two dot products').
 // The sums run in the same order as btd_row, so the results are identical to phase_pcg.
 template <int PL>
 __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, float* sm) {
@@ -331,8 +331,11 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
     // the knot's own block from the row's lanes by DPP broadcast, the neighbours' entries (one 32-bit LDS load
     // each) likewise; the terms are summed in the order diagonal, left, right block, column by column
     const int ln = lane;  // 0 .. 15; padding lanes read the zero padding of the LDS vectors
-    auto mv = [&](const float* M0, const float* M1, const float* M2, float own, const float* x) {
-        const float xm = x[kk * XP + ln], xp = x[(kk + 2) * XP + ln];
+    float* sz = sr + (N + 2) * XP + 64;  // z, after the two reduction buffers
+    for (int i = tid; i < (N + 2) * XP; i += blockDim.x) sz[i] = 0.f;
+    // block row times a vector held one entry per lane: own entry, and the entries of knots k-1 and k+1 at this
+    // lane's index (xm, xp), which the knot group's lanes broadcast to each other
+    auto mv = [&](const float* M0, const float* M1, const float* M2, float own, float xm, float xp) {
         float s = row_bcast_dot<NX>(0.f, own, M0);
         s = row_bcast_dot<NX>(s, xm, M1);
         return row_bcast_dot<NX>(s, xp, M2);
@@ -341,41 +344,49 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
     __syncthreads();
     if (val) sp[(kk + 1) * XP + l] = x;
     __syncthreads();
-    const float sx = mv(sd, sl, su, x, sp);
+    const float sx = mv(sd, sl, su, x, sp[kk * XP + ln], sp[(kk + 2) * XP + ln]);
     float r = val ? v.gam[kk * NX + l] - sx : 0.f;
     if (val) sr[(kk + 1) * XP + l] = r;
     __syncthreads();
-    const float pr0 = mv(pd, pl, pu, r, sr);
+    // Each lane keeps its own copies of r and p at its index in the neighbour knots (rm, rp, pm, pp) and updates
+    // them with the neighbours' Ap and z, which are published before the two reductions' barriers anyway, by the
+    // same operations the owners apply: the copies equal the owners' values bit for bit, and no barrier is needed
+    // to publish r and p (two barriers per iteration instead of four).
+    float rm = sr[kk * XP + ln], rp = sr[(kk + 2) * XP + ln];
+    const float pr0 = mv(pd, pl, pu, r, rm, rp);
     float z = val ? pr0 : 0.f;
-    float p = z;
+    if (val) sz[(kk + 1) * XP + l] = z;
     int slot = 0;
     float rho = block_sum1(r * z, red + 32 * slot);
     slot ^= 1;
+    float p = z, pm = sz[kk * XP + ln], pp = sz[(kk + 2) * XP + ln];
     if (fabsf(rho) < a.pcg_abs_tol) return 0;
     const float rho0 = fabsf(rho);
     int it = 0;
     for (int i = 0; i < a.max_pcg; i++) {
         it++;
-        if (val) sp[(kk + 1) * XP + l] = p;
-        __syncthreads();
-        const float sp_ = mv(sd, sl, su, p, sp);
+        const float sp_ = mv(sd, sl, su, p, pm, pp);
         const float Ap = val ? sp_ : 0.f;
+        if (val) sp[(kk + 1) * XP + l] = Ap;
         // quotients by reciprocal and product, as the reference's fast-math build divides (-use_fast_math)
         const float alpha = rho * __builtin_amdgcn_rcpf(block_sum1(p * Ap, red + 32 * slot));
         slot ^= 1;
         x += alpha * p;
         r -= alpha * Ap;
-        if (val) sr[(kk + 1) * XP + l] = r;
-        __syncthreads();
+        rm -= alpha * sp[kk * XP + ln];
+        rp -= alpha * sp[(kk + 2) * XP + ln];
         if (it >= a.max_pcg) break;  // x is final at the cap: its last preconditioner application is not needed
-        const float pr = mv(pd, pl, pu, r, sr);
+        const float pr = mv(pd, pl, pu, r, rm, rp);
         z = val ? pr : 0.f;
+        if (val) sz[(kk + 1) * XP + l] = z;
         const float rn = block_sum1(r * z, red + 32 * slot);
         slot ^= 1;
         if (fabsf(rn) < a.pcg_abs_tol + eps * rho0) break;
         const float beta = rn * __builtin_amdgcn_rcpf(rho);
         rho = rn;
         p = z + beta * p;
+        pm = sz[kk * XP + ln] + beta * pm;
+        pp = sz[(kk + 2) * XP + ln] + beta * pp;
     }
     if (val) v.lam[kk * NX + l] = x;
     return it;
