@@ -402,10 +402,10 @@ __device__ __forceinline__ float block_sum64(float v, float* red) {
     return (lane_f(s, 0) + lane_f(s, 16)) + (lane_f(s, 32) + lane_f(s, 48));
 }
 
-// LDS floats of phase_pcg_res: the problem's off-diagonal blocks of S, two padded vectors, two reduction buffers
+// LDS floats of phase_pcg_res: the problem's off-diagonal blocks of S, three padded vectors, two reduction buffers
 template <int PL>
 __host__ __device__ constexpr int pcg_res_lds_floats(int N) {
-    return N * Plant<PL>::NX * Plant<PL>::NX + 2 * (N + 2) * 16 + 128;
+    return N * Plant<PL>::NX * Plant<PL>::NX + 3 * (N + 2) * 16 + 128;
 }
 
 // The iteration of phase_pcg for 64 < N <= 128 with the problem held on its CU for the whole solve: 1024 lanes, knot
@@ -416,22 +416,23 @@ __host__ __device__ constexpr int pcg_res_lds_floats(int N) {
 // form: the preconditioner's off-diagonal blocks are P_{k+1,k} = -P_{k+1,k+1} So_{k+1} P_{k,k} by construction
 // (schur_linsys.cuh:213-260, precond_offdiag), so with w = Pd r (block diagonal),
 //   (P r)_k = Pd_k r_k + P_{k,k-1} r_{k-1} + P_{k+1,k}^T r_{k+1} = Pd_k (r_k - So_k w_{k-1} - So_{k+1}^T w_{k+1}),
-// exactly in real arithmetic (Pd_k is symmetric); in fp32 it differs from the explicit product by rounding. Four
-// work-group barriers per iteration, as phase_pcg_reg.
+// exactly in real arithmetic (Pd_k is symmetric); in fp32 it differs from the explicit product by rounding. Three
+// work-group barriers per iteration: the two dot products' and the exchange of w.
 template <int PL>
 __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, float* sm) {
     constexpr int NX = Plant<PL>::NX, NB = NX * NX, XP = 16, NP = 2, G = 64;
     View<PL> v(a, b);
     const int N = a.N, tid = threadIdx.x, g = tid / KG, lane = tid % KG;
     float* So = sm;
-    float* sp = So + N * NB;
-    float* sw = sp + (N + 2) * XP;
-    float* red = sw + (N + 2) * XP;  // 2 x 64 floats
+    float* sp = So + N * NB;        // x at the start
+    float* sw = sp + (N + 2) * XP;  // w = Pd r
+    float* sz = sw + (N + 2) * XP;  // z
+    float* red = sz + (N + 2) * XP;  // 2 x 64 floats
     {
         const float4* src = reinterpret_cast<const float4*>(v.So);
         float4* dst = reinterpret_cast<float4*>(So);
         for (int i = tid; i < N * NB / 4; i += blockDim.x) dst[i] = src[i];
-        for (int i = tid; i < 2 * (N + 2) * XP; i += blockDim.x) sp[i] = 0.f;
+        for (int i = tid; i < 3 * (N + 2) * XP; i += blockDim.x) sp[i] = 0.f;
     }
     const int l = lane < NX ? lane : 0;
     bool val[NP];
@@ -449,9 +450,9 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
             pd[s][c] = PD[c];
         }
     }
-    // the off-diagonal blocks of block row k times the vector y: row l of So_k (k > 0) with y_{k-1}, then column l
-    // of So_{k+1} (k < N-1) with y_{k+1}, starting from acc
-    auto offdiag = [&](int k, const float* y, float acc) {
+    // the off-diagonal blocks of block row k times a vector whose entries at this lane's index in knots k-1 and k+1
+    // are ym and yp: row l of So_k (k > 0), then column l of So_{k+1} (k < N-1), starting from acc
+    auto offdiag = [&](int k, float ym, float yp, float acc) {
         float L[NX], U[NX];
         const float2* Lr = reinterpret_cast<const float2*>(So + k * NB + l * NX);  // 8-byte aligned: NX is even
         const float* Uc = So + (k + 1 < N ? k + 1 : k) * NB + l;
@@ -463,10 +464,10 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
         }
 #pragma unroll
         for (int c = 0; c < NX; c++) U[c] = k < N - 1 ? Uc[c * NX] : 0.f;
-        acc = row_bcast_dot<NX>(acc, y[k * XP + lane], L);
-        return row_bcast_dot<NX>(acc, y[(k + 2) * XP + lane], U);
+        acc = row_bcast_dot<NX>(acc, ym, L);
+        return row_bcast_dot<NX>(acc, yp, U);
     };
-    float x[NP], r[NP], p[NP], z[NP];
+    float x[NP], r[NP], p[NP], z[NP], pm[NP], pp[NP];
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < NP; s++) {
@@ -476,10 +477,13 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < NP; s++) {
-        const float sx = offdiag(kk[s], sp, row_bcast_dot<NX>(0.f, x[s], sd[s]));
+        const float sx = offdiag(kk[s], sp[kk[s] * XP + lane], sp[(kk[s] + 2) * XP + lane],
+                                 row_bcast_dot<NX>(0.f, x[s], sd[s]));
         r[s] = val[s] ? v.gam[kk[s] * NX + l] - sx : 0.f;
     }
-    // z = P r in factored form; w published for the neighbours, the rest per knot group
+    // z = P r in factored form: w published for the neighbours (one barrier), z published for the neighbours' copies
+    // of p (pm, pp: this lane's index in knots k-1 and k+1, updated by the owners' operations, so no barrier is
+    // needed to publish p)
     auto precond = [&](float* zz) {
 #pragma unroll
         for (int s = 0; s < NP; s++) {
@@ -490,10 +494,11 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
         float loc = 0.f;
 #pragma unroll
         for (int s = 0; s < NP; s++) {
-            const float t = offdiag(kk[s], sw, 0.f);
+            const float t = offdiag(kk[s], sw[kk[s] * XP + lane], sw[(kk[s] + 2) * XP + lane], 0.f);
             const float u = val[s] ? r[s] - t : 0.f;
             const float zs = row_bcast_dot<NX>(0.f, u, pd[s]);
             zz[s] = val[s] ? zs : 0.f;
+            if (val[s]) sz[(kk[s] + 1) * XP + l] = zz[s];
             loc += r[s] * zz[s];
         }
         return loc;
@@ -502,20 +507,20 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
     float rho = block_sum64(precond(z), red + 64 * slot);
     slot ^= 1;
 #pragma unroll
-    for (int s = 0; s < NP; s++) p[s] = z[s];
+    for (int s = 0; s < NP; s++) {
+        p[s] = z[s];
+        pm[s] = sz[kk[s] * XP + lane];
+        pp[s] = sz[(kk[s] + 2) * XP + lane];
+    }
     if (fabsf(rho) < a.pcg_abs_tol) return 0;
     const float rho0 = fabsf(rho);
     int it = 0;
     for (int i = 0; i < a.max_pcg; i++) {
         it++;
-#pragma unroll
-        for (int s = 0; s < NP; s++)
-            if (val[s]) sp[(kk[s] + 1) * XP + l] = p[s];
-        __syncthreads();
         float Ap[NP], loc = 0.f;
 #pragma unroll
         for (int s = 0; s < NP; s++) {
-            const float a_ = offdiag(kk[s], sp, row_bcast_dot<NX>(0.f, p[s], sd[s]));
+            const float a_ = offdiag(kk[s], pm[s], pp[s], row_bcast_dot<NX>(0.f, p[s], sd[s]));
             Ap[s] = val[s] ? a_ : 0.f;
             loc += p[s] * Ap[s];
         }
@@ -534,7 +539,11 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
         const float beta = rn * __builtin_amdgcn_rcpf(rho);
         rho = rn;
 #pragma unroll
-        for (int s = 0; s < NP; s++) p[s] = z[s] + beta * p[s];
+        for (int s = 0; s < NP; s++) {
+            p[s] = z[s] + beta * p[s];
+            pm[s] = sz[kk[s] * XP + lane] + beta * pm[s];
+            pp[s] = sz[(kk[s] + 2) * XP + lane] + beta * pp[s];
+        }
     }
 #pragma unroll
     for (int s = 0; s < NP; s++)
