@@ -450,22 +450,29 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
             pd[s][c] = PD[c];
         }
     }
-    // the off-diagonal blocks of block row k times a vector whose entries at this lane's index in knots k-1 and k+1
-    // are ym and yp: row l of So_k (k > 0), then column l of So_{k+1} (k < N-1), starting from acc
-    auto offdiag = [&](int k, float ym, float yp, float acc) {
-        float L[NX], U[NX];
+    // column l of So_{k+1} (k < N-1) for both knots, in registers for the whole solve
+    float uc[NP][NX];
+#pragma unroll
+    for (int s = 0; s < NP; s++) {
+        const int k = kk[s];
+        const float* Uc = v.So + (size_t)(k + 1 < N ? k + 1 : k) * NB + l;
+#pragma unroll
+        for (int c = 0; c < NX; c++) uc[s][c] = k < N - 1 ? Uc[c * NX] : 0.f;
+    }
+    // the off-diagonal blocks of block row kk[s] times a vector whose entries at this lane's index in knots k-1 and k+1
+    // are ym and yp: row l of So_k (k > 0, from LDS), then column l of So_{k+1} (registers), starting from acc
+    auto offdiag = [&](int s, float ym, float yp, float acc) {
+        const int k = kk[s];
+        float L[NX];
         const float2* Lr = reinterpret_cast<const float2*>(So + k * NB + l * NX);  // 8-byte aligned: NX is even
-        const float* Uc = So + (k + 1 < N ? k + 1 : k) * NB + l;
 #pragma unroll
         for (int c = 0; c < NX / 2; c++) {
             const float2 t = Lr[c];
             L[2 * c] = k > 0 ? t.x : 0.f;
             L[2 * c + 1] = k > 0 ? t.y : 0.f;
         }
-#pragma unroll
-        for (int c = 0; c < NX; c++) U[c] = k < N - 1 ? Uc[c * NX] : 0.f;
         acc = row_bcast_dot<NX>(acc, ym, L);
-        return row_bcast_dot<NX>(acc, yp, U);
+        return row_bcast_dot<NX>(acc, yp, uc[s]);
     };
     float x[NP], r[NP], p[NP], z[NP], pm[NP], pp[NP];
     __syncthreads();
@@ -477,7 +484,7 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < NP; s++) {
-        const float sx = offdiag(kk[s], sp[kk[s] * XP + lane], sp[(kk[s] + 2) * XP + lane],
+        const float sx = offdiag(s, sp[kk[s] * XP + lane], sp[(kk[s] + 2) * XP + lane],
                                  row_bcast_dot<NX>(0.f, x[s], sd[s]));
         r[s] = val[s] ? v.gam[kk[s] * NX + l] - sx : 0.f;
     }
@@ -494,7 +501,7 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
         float loc = 0.f;
 #pragma unroll
         for (int s = 0; s < NP; s++) {
-            const float t = offdiag(kk[s], sw[kk[s] * XP + lane], sw[(kk[s] + 2) * XP + lane], 0.f);
+            const float t = offdiag(s, sw[kk[s] * XP + lane], sw[(kk[s] + 2) * XP + lane], 0.f);
             const float u = val[s] ? r[s] - t : 0.f;
             const float zs = row_bcast_dot<NX>(0.f, u, pd[s]);
             zz[s] = val[s] ? zs : 0.f;
@@ -520,7 +527,7 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
         float Ap[NP], loc = 0.f;
 #pragma unroll
         for (int s = 0; s < NP; s++) {
-            const float a_ = offdiag(kk[s], pm[s], pp[s], row_bcast_dot<NX>(0.f, p[s], sd[s]));
+            const float a_ = offdiag(s, pm[s], pp[s], row_bcast_dot<NX>(0.f, p[s], sd[s]));
             Ap[s] = val[s] ? a_ : 0.f;
             loc += p[s] * Ap[s];
         }
