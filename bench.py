@@ -148,9 +148,10 @@ def time_fwd_bwd(solver, wl, steps, warmup):
     return time.perf_counter() - t0, g
 
 
-def time_mixed(steps):
-    """The uneven batch: 1024 figure-8 problems, half iiwa14 and half indy7, N ~ U{8, 32, 128}, one hipGraph
-    launch per step (launch shapes tuned on the first solve)."""
+def time_mixed(steps, mode="graph", groups=0):
+    """The uneven batch: 1024 figure-8 problems, half iiwa14 and half indy7, N ~ U{8, 32, 128}, one launch per step:
+    the hipGraph with a branch per class (launch shapes tuned on the first solve), or with mode="fused" the
+    persistent kernel over a work list of all 1024 problems (work-groups of `groups` knot groups)."""
     import torch
 
     from gato_amd import workloads as W
@@ -162,7 +163,7 @@ def time_mixed(steps):
     kw = {k: p[k] for k in ("dt", "max_sqp_iters", "kkt_tol", "max_pcg_iters", "pcg_tol", "solve_ratio", "mu",
                             "q_cost", "qd_cost", "u_cost", "N_cost", "q_lim_cost", "vel_lim_cost", "ctrl_lim_cost",
                             "rho")}
-    m = MixedSolver([(k[0], k[1], wls[k].B) for k in keys], **kw)
+    m = MixedSolver([(k[0], k[1], wls[k].B) for k in keys], mode=mode, groups=groups, **kw)
     t = lambda a: torch.tensor(a, dtype=torch.float32, device=m.device).contiguous()
     xu0 = [t(wls[k].xu) for k in keys]
     xs = [t(wls[k].xs) for k in keys]
@@ -176,7 +177,7 @@ def time_mixed(steps):
             m.class_solver(i).reset_rho()
         m.solve_device(xu, xs, ref)
 
-    one()  # tunes the launch shapes and captures the graph
+    one()  # graph: tunes the launch shapes and captures the graph
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     dev_us = 0.0
@@ -185,10 +186,13 @@ def time_mixed(steps):
         dev_us += m.last_time_us()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    return {"problems": 1024, "classes": {f"{k[0]} N={k[1]}": wls[k].B for k in keys},
-            "problem_solves_per_s": 1024 * steps / el, "ms_per_step": 1e3 * el / steps,
-            "graph_launch_us": dev_us / steps,
-            "launch_shapes": {f"{k[0]} N={k[1]}": m.launch_shapes()[i][1] for i, k in enumerate(keys)}}
+    out = {"problems": 1024, "classes": {f"{k[0]} N={k[1]}": wls[k].B for k in keys}, "mode": mode,
+           "problem_solves_per_s": 1024 * steps / el, "ms_per_step": 1e3 * el / steps, "launch_us": dev_us / steps}
+    if mode == "graph":
+        out["launch_shapes"] = {f"{k[0]} N={k[1]}": m.launch_shapes()[i][1] for i, k in enumerate(keys)}
+    else:
+        out["threads_per_work_group"] = 16 * (groups or 32)
+    return out
 
 
 def _weights(p):
@@ -606,6 +610,7 @@ def main():
         out["reference_heatmap_point"] = time_reference_mpc(args.ref_mpc_steps)
     if args.mixed_steps > 0:
         out["mixed_batch"] = time_mixed(args.mixed_steps)
+        out["mixed_batch"]["persistent_kernel"] = time_mixed(args.mixed_steps, mode="fused", groups=32)
     if args.fp64_steps > 0:
         out["fp64_mode"] = time_fp64(args.fp64_steps)
     if args.diff_steps > 0:

@@ -408,9 +408,11 @@ __host__ __device__ constexpr int pcg_res_lds_floats(int N) {
     return N * Plant<PL>::NX * Plant<PL>::NX + 3 * (N + 2) * 16 + 128;
 }
 
-// The iteration of phase_pcg for 64 < N <= 128 with the problem held on its CU for the whole solve: 1024 lanes, knot
-// group g owning knots g and g + 64, the diagonal blocks of S and P in registers (lane l: row l), the off-diagonal
-// blocks of S (So, 100 KB for iiwa14 at N = 128) in LDS, nothing streamed from HBM per iteration. Every block row
+// The iteration of phase_pcg with the problem held on its CU for the whole solve: G knot groups (GC = 64: 1024 lanes,
+// the sums over 64 groups of block_sum64; GC = 0: blockDim.x / 16 <= 32 groups, block_sum1's), knot group g owning
+// knots g, g + G, ..., g + (NP - 1) G (N <= NP G), the diagonal blocks of S and P and each lane's columns of the
+// upper off-diagonal blocks in registers (lane l: row l), the off-diagonal blocks of S (So, 100 KB for iiwa14 at
+// N = 128) in LDS, nothing streamed from HBM per iteration. Every block row
 // times a vector uses DPP broadcasts within the knot group (row_bcast_dot), the neighbour knots' entries read as one
 // 32-bit LDS load per lane. S p sums the diagonal, left and right blocks in btd_row's order. P is applied in factored
 // form: the preconditioner's off-diagonal blocks are P_{k+1,k} = -P_{k+1,k+1} So_{k+1} P_{k,k} by construction
@@ -418,11 +420,12 @@ __host__ __device__ constexpr int pcg_res_lds_floats(int N) {
 //   (P r)_k = Pd_k r_k + P_{k,k-1} r_{k-1} + P_{k+1,k}^T r_{k+1} = Pd_k (r_k - So_k w_{k-1} - So_{k+1}^T w_{k+1}),
 // exactly in real arithmetic (Pd_k is symmetric); in fp32 it differs from the explicit product by rounding. Three
 // work-group barriers per iteration: the two dot products' and the exchange of w.
-template <int PL>
+template <int PL, int NP = 2, int GC = 64>
 __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, float* sm) {
-    constexpr int NX = Plant<PL>::NX, NB = NX * NX, XP = 16, NP = 2, G = 64;
+    constexpr int NX = Plant<PL>::NX, NB = NX * NX, XP = 16;
     View<PL> v(a, b);
-    const int N = a.N, tid = threadIdx.x, g = tid / KG, lane = tid % KG;
+    const int N = a.N, tid = threadIdx.x, g = tid / KG, lane = tid % KG, G = GC ? GC : (int)blockDim.x / KG;
+    auto bsum = [](float x, float* red) { return GC == 64 ? block_sum64(x, red) : block_sum1(x, red); };
     float* So = sm;
     float* sp = So + N * NB;        // x at the start
     float* sw = sp + (N + 2) * XP;  // w = Pd r
@@ -450,7 +453,7 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
             pd[s][c] = PD[c];
         }
     }
-    // column l of So_{k+1} (k < N-1) for both knots, in registers for the whole solve
+    // column l of So_{k+1} (k < N-1) for every knot of the group, in registers for the whole solve
     float uc[NP][NX];
 #pragma unroll
     for (int s = 0; s < NP; s++) {
@@ -511,7 +514,7 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
         return loc;
     };
     int slot = 0;
-    float rho = block_sum64(precond(z), red + 64 * slot);
+    float rho = bsum(precond(z), red + 64 * slot);
     slot ^= 1;
 #pragma unroll
     for (int s = 0; s < NP; s++) {
@@ -532,7 +535,7 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
             loc += p[s] * Ap[s];
         }
         // quotients by reciprocal and product, as the reference's fast-math build divides (-use_fast_math)
-        const float alpha = rho * __builtin_amdgcn_rcpf(block_sum64(loc, red + 64 * slot));
+        const float alpha = rho * __builtin_amdgcn_rcpf(bsum(loc, red + 64 * slot));
         slot ^= 1;
 #pragma unroll
         for (int s = 0; s < NP; s++) {
@@ -540,7 +543,7 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
             r[s] -= alpha * Ap[s];
         }
         if (it >= a.max_pcg) break;  // x is final at the cap: its last preconditioner application is not needed
-        const float rn = block_sum64(precond(z), red + 64 * slot);
+        const float rn = bsum(precond(z), red + 64 * slot);
         slot ^= 1;
         if (fabsf(rn) < a.pcg_abs_tol + eps * rho0) break;
         const float beta = rn * __builtin_amdgcn_rcpf(rho);
@@ -1013,6 +1016,15 @@ __device__ __forceinline__ int pcg_dispatch(const Args& a, int b, float eps, flo
     if constexpr (NPASS == 1) return phase_pcg_reg<PL>(a, b, eps, sm);
     else if constexpr (NPASS == 0) return phase_pcg_res<PL>(a, b, eps, sm);
     else return phase_pcg<PL, NPASS>(a, b, eps, sm);
+}
+
+// The persistent kernels' PCG: as pcg_dispatch, but with two or four knots per knot group the problem stays resident
+// (phase_pcg_res; the caller provides pcg_res_lds_floats(N) of LDS); eight knots per group would not fit in
+// registers and stream as phase_pcg.
+template <int PL, int NPASS>
+__device__ __forceinline__ int pcg_dispatch_fused(const Args& a, int b, float eps, float* sm) {
+    if constexpr (NPASS == 2 || NPASS == 4) return phase_pcg_res<PL, NPASS, 0>(a, b, eps, sm);
+    else return pcg_dispatch<PL, NPASS>(a, b, eps, sm);
 }
 
 template <int PL, typename S = float>
@@ -1515,15 +1527,14 @@ __global__ void __launch_bounds__(64) k_status(ArgsT<S> a, int* status) {
 }
 
 // ------------------------------------------------------------------ fused persistent kernel
-// One work-group per problem; problems are independent (a converged problem stops iterating;
-// the reference instead keeps stepping converged problems until the whole batch has converged).
+// One whole solve of problem b by the calling work-group: initial merit, the SQP iterations (linearization, Schur
+// complement, PCG, dz, merit at every step size, line search) until the problem's own exit rule fires or the
+// iteration cap, final merit. Problems are independent (a converged problem stops iterating; the reference instead
+// keeps stepping converged problems until the whole batch has converged). shs: 16 floats, shi: 4 ints of LDS.
 template <int PL, int NPASS>
-__global__ void __launch_bounds__(512) k_fused(Args a) {
-    extern __shared__ __align__(16) float sm[];
-    __shared__ float shs[16];
-    __shared__ int shi[4];
-    constexpr int NX = Plant<PL>::NX, NU = Plant<PL>::NU;
-    for (int b = blockIdx.x; b < a.B; b += gridDim.x) {
+__device__ __forceinline__ void fused_problem(const Args& a, int b, float* sm, float* shs, int* shi) {
+    constexpr int NU = Plant<PL>::NU;
+    {
         View<PL> v(a, b);
         const float mu = a.mu[b], eps = a.tol[b];
         for (int i = threadIdx.x; i < a.traj + NU; i += blockDim.x) v.dz[i] = 0.f;
@@ -1541,7 +1552,7 @@ __global__ void __launch_bounds__(512) k_fused(Args a) {
             __syncthreads();
             phase_schur<PL>(a, b, a.rho[b], sm);
             __syncthreads();
-            const int n = pcg_dispatch<PL, NPASS>(a, b, eps, sm);
+            const int n = pcg_dispatch_fused<PL, NPASS>(a, b, eps, sm);
             __syncthreads();
             const float kr = phase_dz<PL>(a, b, sm);
             if (threadIdx.x == 0) {
@@ -1577,7 +1588,60 @@ __global__ void __launch_bounds__(512) k_fused(Args a) {
         }
         __syncthreads();
     }
-    (void)NX;
+}
+
+// One work-group per problem, grid-strided over the batch.
+template <int PL, int NPASS>
+__global__ void __launch_bounds__(512) k_fused(Args a) {
+    extern __shared__ __align__(16) float sm[];
+    __shared__ float shs[16];
+    __shared__ int shi[4];
+    for (int b = blockIdx.x; b < a.B; b += gridDim.x) fused_problem<PL, NPASS>(a, b, sm, shs, shi);
+}
+
+// Problems of several (plant, horizon) classes in one persistent launch. items lists (class | tag << 8, problem) in
+// the order they are to start (longest first), tag = 16 plant + knot passes of the class; cls[class] its arguments.
+// Work-groups take the next item from ctr[0] as they finish the last, so the load balances itself; each item is
+// the class's begin-of-solve reset of the problem and its whole solve, the same work as k_begin and k_fused for
+// it. The last work-group to exit (ctr[1]) sets both counters back to zero for the next launch, so a replay needs
+// no clearing of them (they are zero when allocated).
+__global__ void __launch_bounds__(512) k_fused_mixed(const Args* __restrict__ cls, const int2* __restrict__ items,
+                                                     int nitems, int* __restrict__ ctr) {
+    extern __shared__ __align__(16) float sm[];
+    __shared__ float shs[16];
+    __shared__ int shi[4];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(ctr, 1);
+        __syncthreads();
+        const int i = __builtin_amdgcn_readfirstlane(next);
+        __syncthreads();
+        if (i >= nitems) break;
+        const int2 item = items[i];
+        const int c = __builtin_amdgcn_readfirstlane(item.x & 255), tag = __builtin_amdgcn_readfirstlane(item.x >> 8);
+        const int b = __builtin_amdgcn_readfirstlane(item.y);
+        const Args& a = cls[c];
+        if (threadIdx.x == 0) begin_problem(a, b);
+        __syncthreads();
+        switch (tag) {
+            case 0x01: fused_problem<0, 1>(a, b, sm, shs, shi); break;
+            case 0x02: fused_problem<0, 2>(a, b, sm, shs, shi); break;
+            case 0x04: fused_problem<0, 4>(a, b, sm, shs, shi); break;
+            case 0x08: fused_problem<0, 8>(a, b, sm, shs, shi); break;
+            case 0x11: fused_problem<1, 1>(a, b, sm, shs, shi); break;
+            case 0x12: fused_problem<1, 2>(a, b, sm, shs, shi); break;
+            case 0x14: fused_problem<1, 4>(a, b, sm, shs, shi); break;
+            case 0x18: fused_problem<1, 8>(a, b, sm, shs, shi); break;
+            default: break;
+        }
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(ctr + 1, 1) == (int)gridDim.x - 1) {
+            ctr[0] = 0;
+            ctr[1] = 0;
+        }
+    }
 }
 
 
@@ -2593,8 +2657,9 @@ void launch_fused(int np, int B, dim3 blk, size_t lds, hipStream_t st, const gat
     else hipLaunchKernelGGL((gato::k_fused<PL, 8>), dim3(B), blk, lds, st, a);
 }
 
-size_t lds_bytes(const gato_solver* s) {
-    const int G = s->G, nx = s->nx, nq = s->nq;
+// dynamic LDS of the per-phase kernels and k_fused for G knot groups per work-group
+size_t lds_bytes_g(const gato_solver* s, int G) {
+    const int nx = s->nx, nq = s->nq;
     size_t f = 0;
     f = std::max<size_t>(f, (size_t)G * (24 * nq + 9 * (nq * (nq + 1) / 2)));  // lin
     const size_t tile = s->plant == GATO_PLANT_IIWA14 ? gato::SchurTile<0>::SIZE : gato::SchurTile<1>::SIZE;
@@ -2603,6 +2668,20 @@ size_t lds_bytes(const gato_solver* s) {
     f = std::max<size_t>(f, (size_t)G * gato::KG + 16);             // dz
     f = std::max<size_t>(f, (size_t)2 * gato::NUM_ALPHAS + 16);     // merit + ls
     return f * sizeof(float);
+}
+
+size_t lds_bytes(const gato_solver* s) { return lds_bytes_g(s, s->G); }
+
+// LDS of k_fused and k_fused_mixed for G knot groups: the phases', or with two or four knots per group the resident
+// PCG's (pcg_dispatch_fused) where that is larger
+size_t fused_lds_bytes(const gato_solver* s, int G) {
+    size_t f = lds_bytes_g(s, G);
+    const int np = (s->N + G - 1) / G;
+    if (np == 2 || np == 4) {
+        const int r = s->plant == GATO_PLANT_IIWA14 ? gato::pcg_res_lds_floats<0>(s->N) : gato::pcg_res_lds_floats<1>(s->N);
+        f = std::max(f, (size_t)r * sizeof(float));
+    }
+    return f;
 }
 
 void fill_args(gato_solver* s, float* d_xu, const float* d_xs, const float* d_ref, float dt) {
@@ -2735,7 +2814,7 @@ int launch_solve(gato_solver* s, hipStream_t st) {
     const int np = npass(s);
     if (s->prm.mode == GATO_MODE_FUSED) {
         HIP_TRY(hipEventRecord(s->ev0, st));
-        launch_fused<PL>(np, B, blk, lds, st, a);
+        launch_fused<PL>(np, B, blk, fused_lds_bytes(s, s->G), st, a);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(s->ev1, st));
         HIP_TRY(hipStreamSynchronize(st));
@@ -3367,15 +3446,19 @@ static int allow_res_lds() {
     static std::once_flag once;
     static hipError_t err = hipSuccess;
     std::call_once(once, [] {
-        const void* fn[4] = {reinterpret_cast<const void*>(&gato::k_pcg_dz_res<0>),
+        const int b0 = gato::pcg_res_lds_floats<0>(128) * (int)sizeof(float);
+        const int b1 = gato::pcg_res_lds_floats<1>(128) * (int)sizeof(float);
+        const void* fn[9] = {reinterpret_cast<const void*>(&gato::k_pcg_dz_res<0>),
                              reinterpret_cast<const void*>(&gato::k_pcg_dz_res<1>),
                              reinterpret_cast<const void*>(&gato::k_pcg_res<0>),
-                             reinterpret_cast<const void*>(&gato::k_pcg_res<1>)};
-        const int bytes[4] = {gato::pcg_res_lds_floats<0>(128) * (int)sizeof(float),
-                              gato::pcg_res_lds_floats<1>(128) * (int)sizeof(float),
-                              gato::pcg_res_lds_floats<0>(128) * (int)sizeof(float),
-                              gato::pcg_res_lds_floats<1>(128) * (int)sizeof(float)};
-        for (int i = 0; i < 4 && err == hipSuccess; i++)
+                             reinterpret_cast<const void*>(&gato::k_pcg_res<1>),
+                             reinterpret_cast<const void*>(&gato::k_fused<0, 2>),
+                             reinterpret_cast<const void*>(&gato::k_fused<0, 4>),
+                             reinterpret_cast<const void*>(&gato::k_fused<1, 2>),
+                             reinterpret_cast<const void*>(&gato::k_fused<1, 4>),
+                             reinterpret_cast<const void*>(&gato::k_fused_mixed)};
+        const int bytes[9] = {b0, b1, b0, b1, b0, b0, b1, b1, b0};
+        for (int i = 0; i < 9 && err == hipSuccess; i++)
             err = hipFuncSetAttribute(fn[i], hipFuncAttributeMaxDynamicSharedMemorySize, bytes[i]);
     });
     return err == hipSuccess ? 0 : fail((int)err, std::string("hipFuncSetAttribute (PCG LDS): ") + hipGetErrorString(err));
@@ -3953,6 +4036,13 @@ struct gato_mixed {
     int tune = 1;
     bool tuned = false;
     double last_us = 0.0;
+    // GATO_MODE_FUSED: k_fused_mixed over a work list of every class's problems, longest class first
+    int mode = GATO_MODE_GRAPH, fgroups = 32;
+    std::vector<int2> items;  // (class, problem) in start order; the class's tag is added by gato_mixed_set_mode
+    std::vector<gato::Args> h_cls;
+    gato::Args* d_cls = nullptr;
+    int2* d_items = nullptr;
+    int* d_ctr = nullptr;
 };
 
 static void mixed_drop_graph(gato_mixed* m) {
@@ -3972,6 +4062,9 @@ int gato_mixed_destroy(gato_mixed* m) {
     if (m->fork) hipEventDestroy(m->fork);
     if (m->ev0) hipEventDestroy(m->ev0);
     if (m->ev1) hipEventDestroy(m->ev1);
+    if (m->d_cls) hipFree(m->d_cls);
+    if (m->d_items) hipFree(m->d_items);
+    if (m->d_ctr) hipFree(m->d_ctr);
     delete m;
     return 0;
 }
@@ -4006,7 +4099,78 @@ int gato_mixed_create(gato_mixed** out, int32_t nclass, const int32_t* plants, c
     }
     m->keys.resize(nclass);
     m->key_groups.assign(nclass, 0);
+    // work list of the persistent mode: classes in decreasing order of per-iteration work (N nx^2), each class's
+    // problems in order
+    std::vector<int> order(nclass);
+    for (int i = 0; i < nclass; i++) order[i] = i;
+    auto work = [&](int i) { return (double)knots[i] * (2.0 * plant_nq(plants[i])) * (2.0 * plant_nq(plants[i])); };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    for (int i : order)
+        for (int b = 0; b < batches[i]; b++) m->items.push_back(int2{i, b});
+    m->h_cls.resize(nclass);
+    if ((e = hipMalloc(&m->d_cls, nclass * sizeof(gato::Args))) != hipSuccess ||
+        (e = hipMalloc(&m->d_items, m->items.size() * sizeof(int2))) != hipSuccess ||
+        (e = hipMalloc(&m->d_ctr, 2 * sizeof(int))) != hipSuccess ||
+        (e = hipMemset(m->d_ctr, 0, 2 * sizeof(int))) != hipSuccess) {
+        gato_mixed_destroy(m);
+        return fail((int)e, std::string("mixed work list: ") + hipGetErrorString(e));
+    }
     *out = m;
+    return 0;
+}
+
+int gato_mixed_set_mode(gato_mixed* m, int32_t mode, int32_t groups) {
+    if (!m) return fail(GATO_E_ARG, "null solver");
+    if (mode != GATO_MODE_GRAPH && mode != GATO_MODE_FUSED)
+        return fail(GATO_E_ARG, "mixed mode must be GATO_MODE_GRAPH or GATO_MODE_FUSED");
+    if (mode == GATO_MODE_FUSED) {
+        const int G = groups ? groups : 32;
+        if (G < 4 || G > 32 || G % 4) return fail(GATO_E_ARG, "groups must be a multiple of 4 in [4, 32] (0: 32)");
+        const int nc = (int)m->cls.size();
+        std::vector<int> tags(nc);
+        for (int i = 0; i < nc; i++) {
+            const int np = (m->cls[i]->N + G - 1) / G;
+            if (np != 1 && np != 2 && np != 4 && np != 8)
+                return fail(GATO_E_ARG, "class " + std::to_string(i) + ": " + std::to_string(m->cls[i]->N) +
+                                            " knots over " + std::to_string(G) + " knot groups is not 1, 2, 4 or 8 passes");
+            tags[i] = (m->cls[i]->plant == GATO_PLANT_IIWA14 ? 0 : 16) + np;
+        }
+        if (int rc = allow_res_lds()) return rc;
+        std::vector<int2> items(m->items);
+        for (auto& it : items) it.x |= tags[it.x] << 8;
+        HIP_TRY(hipMemcpy(m->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+        m->fgroups = G;
+    }
+    m->mode = mode;
+    return 0;
+}
+
+// One launch of k_fused_mixed over every class's problems (fresh solves from the classes' persistent state, as
+// GATO_MODE_FUSED solves each class), on as many work-groups as fit on the device at once.
+static int mixed_solve_fused(gato_mixed* m, hipStream_t st) {
+    const int nc = (int)m->cls.size(), threads = gato::KG * m->fgroups;
+    size_t lds = 0;
+    for (auto* s : m->cls) lds = std::max(lds, fused_lds_bytes(s, m->fgroups));
+    int dev = 0, cus = 0, per = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gato::k_fused_mixed, threads, lds));
+    const int nitems = (int)m->items.size();
+    const int grid = std::max(1, std::min(nitems, std::max(1, per) * cus));
+    bool same = true;
+    for (int i = 0; i < nc; i++) same = same && std::memcmp(&m->h_cls[i], &m->cls[i]->a, sizeof(gato::Args)) == 0;
+    if (!same) {  // the classes' arguments changed (new tensors, dt): upload them before the launch
+        for (int i = 0; i < nc; i++) m->h_cls[i] = m->cls[i]->a;
+        HIP_TRY(hipMemcpyAsync(m->d_cls, m->h_cls.data(), nc * sizeof(gato::Args), hipMemcpyHostToDevice, st));
+    }
+    HIP_TRY(hipEventRecord(m->ev0, st));
+    hipLaunchKernelGGL(gato::k_fused_mixed, dim3(grid), dim3(threads), lds, st, m->d_cls, m->d_items, nitems, m->d_ctr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(m->ev1, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    m->last_us = ms * 1e3;
     return 0;
 }
 
@@ -4031,6 +4195,17 @@ int gato_mixed_solve(gato_mixed* m, float* const* d_xu, const float* const* d_xs
         if (!d_xu[i] || !d_xs[i] || !d_ref[i]) return fail(GATO_E_ARG, "null class pointer");
     hipStream_t st = (hipStream_t)stream;
     const auto t0 = std::chrono::high_resolution_clock::now();
+    if (m->mode == GATO_MODE_FUSED) {
+        for (int i = 0; i < nc; i++) fill_args(m->cls[i], d_xu[i], d_xs[i], d_ref[i], dt);
+        const int rc = mixed_solve_fused(m, st);
+        if (rc) return rc;
+        const double us = std::chrono::duration<double, std::micro>(std::chrono::high_resolution_clock::now() - t0).count();
+        for (int i = 0; i < nc && stats; i++) {
+            const int rs = collect_stats(m->cls[i], st, &stats[i], us);
+            if (rs) return rs;
+        }
+        return 0;
+    }
     if (m->tune && !m->tuned) {
         for (int i = 0; i < nc; i++) {
             const int rc = autotune(m->cls[i], d_xu[i], d_xs[i], d_ref[i], dt, st, nullptr);

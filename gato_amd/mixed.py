@@ -1,11 +1,13 @@
-"""Uneven batching: problems of several horizons and both plants solved by one hipGraph launch.
+"""Uneven batching: problems of several horizons and both plants solved by one launch.
 
 ``MixedSolver(classes, *scalars, **settings)`` takes classes ``[(plant, N, B), ...]`` that share the solver
 scalars (the reference constructor's 15, ``solver.DEFAULTS``). ``solve_device`` solves every class in place
-from per-class device tensors; each class is one branch of a single captured graph, so all horizons and
-plants run concurrently on the GPU. The exit rule (solve_ratio) applies per class. ``class_solver(i)``
-returns a ``Solver`` view of class i for the per-class setters. No reference counterpart: the reference
-builds one module per (N, plant) and solves one batch per call.
+from per-class device tensors. With ``mode="graph"`` (the default) each class is one branch of a single captured
+graph, so all horizons and plants run concurrently on the GPU, and the exit rule (solve_ratio) applies per class.
+With ``mode="fused"`` one persistent kernel solves every problem of every class, work-groups of ``groups`` knot
+groups taking problems from a work list, larger classes first; each problem runs to its own exit, as
+``Solver(mode="fused")`` solves a class. ``class_solver(i)`` returns a ``Solver`` view of class i for the per-class
+setters. No reference counterpart: the reference builds one module per (N, plant) and solves one batch per call.
 """
 import ctypes as C
 
@@ -14,12 +16,12 @@ import torch
 
 from . import _lib
 from ._lib import check
-from .solver import EXITS, Solver, _params, _settings
+from .solver import EXITS, MODES, Solver, _params, _settings
 
 
 class MixedSolver:
     def __init__(self, classes, *args, exit_rule="pcg0", terminal_mode=0, hess_mode=0, autotune=True, device=None,
-                 **kw):
+                 mode="graph", groups=0, **kw):
         self.cfg = _settings(args, kw)
         self.classes = [(str(p), int(n), int(b)) for p, n, b in classes]
         if not self.classes:
@@ -39,6 +41,7 @@ class MixedSolver:
             check(self.lib.gato_mixed_create(C.byref(self._h), n, plants, knots, batches, C.byref(prm)),
                   "gato_mixed_create")
         check(self.lib.gato_mixed_set_autotune(self._h, int(bool(autotune))), "gato_mixed_set_autotune")
+        self.set_mode(mode, groups)
         self._views = []
         for i, (p, k, b) in enumerate(self.classes):
             h = C.c_void_p()
@@ -56,6 +59,14 @@ class MixedSolver:
             except Exception:
                 pass
             self._h = None
+
+    def set_mode(self, mode, groups=0):
+        """``"graph"``: one hipGraph with a branch per class; ``"fused"``: one persistent kernel whose work-groups
+        have ``groups`` knot groups (0: 32)."""
+        if mode not in ("graph", "fused"):
+            raise ValueError(f"mode must be 'graph' or 'fused', not {mode!r}")
+        check(self.lib.gato_mixed_set_mode(self._h, MODES[mode], int(groups)), "gato_mixed_set_mode")
+        self.mode = mode
 
     def class_solver(self, i):
         return self._views[i]

@@ -224,8 +224,8 @@ int gato_autotune(gato_solver* s, float* d_xu, const float* d_xs, const float* d
 
 /* Uneven batching (no reference counterpart; the reference compiles one N and one plant per module).
  * A mixed solver holds `nclass` classes of problems, class i = (plants[i], knots[i], batches[i]), all with
- * `params` (run in GATO_MODE_GRAPH). gato_mixed_solve captures every class's solve as a parallel branch of
- * ONE hipGraph and launches it once; the exit rule applies per class. On the first solve each class's launch
+ * `params` (params->mode is ignored: gato_mixed_set_mode chooses). By default gato_mixed_solve captures every
+ * class's solve as a parallel branch of ONE hipGraph and launches it once; the exit rule applies per class. On the first solve each class's launch
  * shape is tuned on its inputs (gato_autotune) unless disabled. Per-class arrays: d_xu[i], d_xs[i], d_ref[i]
  * (device pointers, layouts as gato_solve); stats: NULL or an array of nclass gato_stats. */
 typedef struct gato_mixed gato_mixed;
@@ -235,9 +235,14 @@ int gato_mixed_destroy(gato_mixed* m);
 /* The class's solver handle, for the per-class setters (f_ext, rho, mu, joint costs, state). Owned by m. */
 int gato_mixed_class(gato_mixed* m, int32_t i, gato_solver** out);
 int gato_mixed_set_autotune(gato_mixed* m, int32_t enabled);
+/* How gato_mixed_solve runs. GATO_MODE_GRAPH (the default): the hipGraph above. GATO_MODE_FUSED: ONE persistent
+ * kernel for all classes; its work-groups (`groups` knot groups of 16 lanes each, 0: 32) take problems from a
+ * work list, larger classes (N nx^2) first, and solve each to completion as GATO_MODE_FUSED does for one class,
+ * with the class's own exit rule per problem. Every class's N / groups, rounded up, must be 1, 2, 4 or 8. */
+int gato_mixed_set_mode(gato_mixed* m, int32_t mode, int32_t groups);
 int gato_mixed_solve(gato_mixed* m, float* const* d_xu, const float* const* d_xs, const float* const* d_ref, float dt,
                      void* stream, gato_stats* stats);
-/* Device time of the last mixed graph launch (hipEvents on the caller's stream), microseconds. */
+/* Device time of the last mixed launch (graph or persistent kernel; hipEvents on the caller's stream), microseconds. */
 int gato_mixed_last_time(gato_mixed* m, double* us);
 
 /* End-effector positions of a batch of trajectories d_xu [B * traj] (device): d_pos [B * N * 3] receives p(q_k), the

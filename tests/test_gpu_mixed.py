@@ -2,6 +2,8 @@
 
 A mixed solve runs each (plant, N) class as a branch of one hipGraph, with the same kernels and launch
 shape as a stand-alone solver of that class, so its results must equal the stand-alone solves bit for bit.
+The persistent mode runs every problem of every class through the same per-problem code as a stand-alone fused
+solve, so it too equals those solves bit for bit at the same knot-group count.
 Other launch shapes change only the grouping of partial sums (PCG dot products), so they agree to rounding.
 """
 import numpy as np
@@ -60,6 +62,59 @@ def test_mixed_solve_equals_standalone_solves(gpu, tune):
     for a, b in zip(ins, ins2):
         assert torch.equal(a[0], b[0])
     assert m.last_time_us() > 0
+
+
+@pytest.mark.parametrize("groups", [32, 16])
+def test_persistent_mixed_kernel_equals_standalone_fused_solves(gpu, groups):
+    # One k_fused_mixed launch over six classes (both plants; N = 8 to 128; one to eight knots per knot group),
+    # against stand-alone fused solves of each class. Where the class's own default knot-group count equals the
+    # persistent kernel's (N = 32 and 128 at 32 groups, N = 16 at 16) the results are equal bit for bit; N = 8 runs
+    # with more groups than knots there (idle groups add zeros to the sums), checked to rounding against its
+    # default shape. A second launch after resetting the classes' state repeats the first exactly (the work-list
+    # counters reset themselves).
+    from gato_amd.mixed import MixedSolver
+    from gato_amd.solver import Solver
+
+    spec = [("iiwa14", 32, 6), ("indy7", 128, 3), ("indy7", 32, 5), ("iiwa14", 128, 2), ("iiwa14", 8, 4),
+            ("indy7", 16, 3)]
+    wls = [W.drop_in_figure8(p, N=n, B=b, seed=50 + i) for i, (p, n, b) in enumerate(spec)]
+    for wl in wls:
+        wl.params["max_sqp_iters"] = 4
+    kw = {k: wls[0].params[k] for k in SCALARS}
+    m = MixedSolver(spec, mode="fused", groups=groups, **kw)
+    dev = m.device
+    ins = [_dev(wl, dev) for wl in wls]
+    st = m.solve_device([x[0] for x in ins], [x[1] for x in ins], [x[2] for x in ins], stats=True)
+    for i, ((p, n, b), wl) in enumerate(zip(spec, wls)):
+        s = Solver(p, n, b, mode="fused", **kw)
+        default = s.launch_shape()[0]
+        exact = default == groups
+        if not exact and groups < default:
+            s.set_launch_shape(groups)
+            exact = True
+        xu, xs, ref = _dev(wl, dev)
+        ref_st = s.solve_device(xu, xs, ref, stats=True)
+        if exact:
+            assert torch.equal(ins[i][0], xu), (i, groups)
+            for key in ("final_merit", "initial_merit", "pcg_iters", "ls_step_size", "sqp_iters", "kkt_residual"):
+                np.testing.assert_array_equal(st[i][key], ref_st[key], err_msg=f"class {i} {key}")
+            np.testing.assert_array_equal(m.class_solver(i).get_state()[0], s.get_state()[0])
+        else:
+            a, r = ins[i][0].cpu().numpy(), xu.cpu().numpy()
+            d = np.linalg.norm(a - r, axis=1) / np.linalg.norm(r, axis=1)
+            assert d.max() < 1e-5, (i, d.max())
+            assert np.all(np.abs(st[i]["pcg_iters"] - ref_st["pcg_iters"]) <= 1), i
+            np.testing.assert_array_equal(st[i]["sqp_iters"], ref_st["sqp_iters"])
+    for i in range(len(spec)):
+        m.class_solver(i).reset_dual()
+        m.class_solver(i).reset_rho()
+    ins2 = [_dev(wl, dev) for wl in wls]
+    m.solve_device([x[0] for x in ins2], [x[1] for x in ins2], [x[2] for x in ins2])
+    for a, b in zip(ins, ins2):
+        assert torch.equal(a[0], b[0])
+    assert m.last_time_us() > 0
+    with pytest.raises(Exception):
+        m.set_mode("fused", groups=8)  # N = 128 over 8 groups is 16 passes
 
 
 def test_launch_shapes_agree_to_rounding(gpu):
