@@ -151,7 +151,8 @@ def time_fwd_bwd(solver, wl, steps, warmup):
 def time_mixed(steps, mode="graph", groups=0):
     """The uneven batch: 1024 figure-8 problems, half iiwa14 and half indy7, N ~ U{8, 32, 128}, one launch per step:
     the hipGraph with a branch per class (launch shapes tuned on the first solve), or with mode="fused" the
-    persistent kernel over a work list of all 1024 problems (work-groups of `groups` knot groups)."""
+    persistent kernel over a work list of all 1024 problems (work-groups of `groups` knot groups; 0: tuned on the
+    first solve)."""
     import torch
 
     from gato_amd import workloads as W
@@ -191,7 +192,7 @@ def time_mixed(steps, mode="graph", groups=0):
     if mode == "graph":
         out["launch_shapes"] = {f"{k[0]} N={k[1]}": m.launch_shapes()[i][1] for i, k in enumerate(keys)}
     else:
-        out["threads_per_work_group"] = 16 * (groups or 32)
+        out["threads_per_work_group"] = 16 * m.fused_groups()
     return out
 
 
@@ -610,7 +611,7 @@ def main():
         out["reference_heatmap_point"] = time_reference_mpc(args.ref_mpc_steps)
     if args.mixed_steps > 0:
         out["mixed_batch"] = time_mixed(args.mixed_steps)
-        out["mixed_batch"]["persistent_kernel"] = time_mixed(args.mixed_steps, mode="fused", groups=32)
+        out["mixed_batch"]["persistent_kernel"] = time_mixed(args.mixed_steps, mode="fused")
     if args.fp64_steps > 0:
         out["fp64_mode"] = time_fp64(args.fp64_steps)
     if args.diff_steps > 0:

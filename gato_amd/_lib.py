@@ -100,6 +100,7 @@ SIGNATURES = {
     "gato_mixed_class": (C.c_int, [_VP, C.c_int32, C.POINTER(C.c_void_p)]),
     "gato_mixed_set_autotune": (C.c_int, [_VP, C.c_int32]),
     "gato_mixed_set_mode": (C.c_int, [_VP, C.c_int32, C.c_int32]),
+    "gato_mixed_get_mode": (C.c_int, [_VP, _ip, _ip]),
     "gato_mixed_solve": (C.c_int, [_VP, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_float,
                                    _VP, C.POINTER(GatoStats)]),
     "gato_mixed_last_time": (C.c_int, [_VP, C.POINTER(C.c_double)]),

@@ -4038,6 +4038,7 @@ struct gato_mixed {
     double last_us = 0.0;
     // GATO_MODE_FUSED: k_fused_mixed over a work list of every class's problems, longest class first
     int mode = GATO_MODE_GRAPH, fgroups = 32;
+    bool fgroups_fixed = false, ftuned = false;  // persistent mode: caller's knot-group count, or tuned once
     std::vector<int2> items;  // (class, problem) in start order; the class's tag is added by gato_mixed_set_mode
     std::vector<gato::Args> h_cls;
     gato::Args* d_cls = nullptr;
@@ -4119,29 +4120,51 @@ int gato_mixed_create(gato_mixed** out, int32_t nclass, const int32_t* plants, c
     return 0;
 }
 
+// Whether the persistent kernel can take every class at G knot groups per work-group: N / G rounded up is 1, 2, 4
+// or 8 for each (its knot passes), G a multiple of 4 in [4, 32].
+static bool mixed_groups_ok(const gato_mixed* m, int G) {
+    if (G < 4 || G > 32 || G % 4) return false;
+    for (auto* c : m->cls) {
+        const int np = (c->N + G - 1) / G;
+        if (np != 1 && np != 2 && np != 4 && np != 8) return false;
+    }
+    return true;
+}
+
+// The work list with every item's tag (16 plant + knot passes) for G knot groups, uploaded; G becomes the kernel's.
+static int mixed_use_groups(gato_mixed* m, int G) {
+    std::vector<int2> items(m->items);
+    for (auto& it : items) {
+        const gato_solver* c = m->cls[it.x];
+        it.x |= ((c->plant == GATO_PLANT_IIWA14 ? 0 : 16) + (c->N + G - 1) / G) << 8;
+    }
+    HIP_TRY(hipMemcpy(m->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+    m->fgroups = G;
+    return 0;
+}
+
 int gato_mixed_set_mode(gato_mixed* m, int32_t mode, int32_t groups) {
     if (!m) return fail(GATO_E_ARG, "null solver");
     if (mode != GATO_MODE_GRAPH && mode != GATO_MODE_FUSED)
         return fail(GATO_E_ARG, "mixed mode must be GATO_MODE_GRAPH or GATO_MODE_FUSED");
     if (mode == GATO_MODE_FUSED) {
         const int G = groups ? groups : 32;
-        if (G < 4 || G > 32 || G % 4) return fail(GATO_E_ARG, "groups must be a multiple of 4 in [4, 32] (0: 32)");
-        const int nc = (int)m->cls.size();
-        std::vector<int> tags(nc);
-        for (int i = 0; i < nc; i++) {
-            const int np = (m->cls[i]->N + G - 1) / G;
-            if (np != 1 && np != 2 && np != 4 && np != 8)
-                return fail(GATO_E_ARG, "class " + std::to_string(i) + ": " + std::to_string(m->cls[i]->N) +
-                                            " knots over " + std::to_string(G) + " knot groups is not 1, 2, 4 or 8 passes");
-            tags[i] = (m->cls[i]->plant == GATO_PLANT_IIWA14 ? 0 : 16) + np;
-        }
+        if (!mixed_groups_ok(m, G))
+            return fail(GATO_E_ARG, "groups " + std::to_string(G) + ": need a multiple of 4 in [4, 32] (0: tuned, else "
+                                    "32) that gives every class 1, 2, 4 or 8 knots per knot group");
         if (int rc = allow_res_lds()) return rc;
-        std::vector<int2> items(m->items);
-        for (auto& it : items) it.x |= tags[it.x] << 8;
-        HIP_TRY(hipMemcpy(m->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
-        m->fgroups = G;
+        if (int rc = mixed_use_groups(m, G)) return rc;
+        m->fgroups_fixed = groups != 0;
+        m->ftuned = false;
     }
     m->mode = mode;
+    return 0;
+}
+
+int gato_mixed_get_mode(gato_mixed* m, int32_t* mode, int32_t* groups) {
+    if (!m || !mode || !groups) return fail(GATO_E_ARG, "null argument");
+    *mode = m->mode;
+    *groups = m->fgroups;
     return 0;
 }
 
@@ -4174,6 +4197,54 @@ static int mixed_solve_fused(gato_mixed* m, hipStream_t st) {
     return 0;
 }
 
+// Knot groups per work-group of the persistent kernel, tuned on the caller's inputs: every valid count of 8, 16 and
+// 32 runs the whole mixed solve twice (the second timed) on scratch copies of the trajectories, each run from the
+// classes' saved multipliers and rho; the fastest is kept, and the classes' state is restored.
+static int mixed_tune_fused(gato_mixed* m, float* const* d_xu, const float* const* d_xs, const float* const* d_ref,
+                            float dt, hipStream_t st) {
+    const int nc = (int)m->cls.size();
+    std::vector<std::vector<float>> rho(nc), drho(nc);
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int i = 0; i < nc; i++) {
+        gato_solver* c = m->cls[i];
+        rho[i].resize(c->B);
+        drho[i].resize(c->B);
+        HIP_TRY(hipMemcpy(c->lam_save, c->a.lam, (size_t)c->B * c->N * c->nx * 4, hipMemcpyDeviceToDevice));
+        HIP_TRY(hipMemcpy(rho[i].data(), c->d_rho, c->B * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(drho[i].data(), c->d_drho, c->B * 4, hipMemcpyDeviceToHost));
+    }
+    auto restore = [&]() -> int {
+        for (int i = 0; i < nc; i++) {
+            gato_solver* c = m->cls[i];
+            HIP_TRY(hipMemcpy(c->a.lam, c->lam_save, (size_t)c->B * c->N * c->nx * 4, hipMemcpyDeviceToDevice));
+            HIP_TRY(hipMemcpy(c->d_rho, rho[i].data(), c->B * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(c->d_drho, drho[i].data(), c->B * 4, hipMemcpyHostToDevice));
+        }
+        return 0;
+    };
+    int best = m->fgroups;
+    double best_us = 1e300;
+    for (int G = 8; G <= 32; G *= 2) {
+        if (!mixed_groups_ok(m, G)) continue;
+        int rc = mixed_use_groups(m, G);
+        for (int rep = 0; rep < 2 && !rc; rep++) {
+            rc = restore();
+            for (int i = 0; i < nc && !rc; i++) {
+                gato_solver* c = m->cls[i];
+                const hipError_t e = hipMemcpyAsync(c->xu_tune, d_xu[i], (size_t)c->B * c->traj * 4,
+                                                    hipMemcpyDeviceToDevice, st);
+                if (e != hipSuccess) rc = fail((int)e, "mixed tuner copy");
+                fill_args(c, c->xu_tune, d_xs[i], d_ref[i], dt);
+            }
+            if (!rc) rc = mixed_solve_fused(m, st);
+        }
+        if (rc) { restore(); return rc; }
+        if (m->last_us < best_us) { best_us = m->last_us; best = G; }
+    }
+    if (int rc = mixed_use_groups(m, best)) return rc;
+    return restore();
+}
+
 int gato_mixed_class(gato_mixed* m, int32_t i, gato_solver** out) {
     if (!m || !out || i < 0 || i >= (int)m->cls.size()) return fail(GATO_E_ARG, "bad class index");
     *out = m->cls[i];
@@ -4196,6 +4267,11 @@ int gato_mixed_solve(gato_mixed* m, float* const* d_xu, const float* const* d_xs
     hipStream_t st = (hipStream_t)stream;
     const auto t0 = std::chrono::high_resolution_clock::now();
     if (m->mode == GATO_MODE_FUSED) {
+        if (m->tune && !m->fgroups_fixed && !m->ftuned) {
+            const int rt = mixed_tune_fused(m, d_xu, d_xs, d_ref, dt, st);
+            if (rt) return rt;
+            m->ftuned = true;
+        }
         for (int i = 0; i < nc; i++) fill_args(m->cls[i], d_xu[i], d_xs[i], d_ref[i], dt);
         const int rc = mixed_solve_fused(m, st);
         if (rc) return rc;

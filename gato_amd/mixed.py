@@ -62,11 +62,18 @@ class MixedSolver:
 
     def set_mode(self, mode, groups=0):
         """``"graph"``: one hipGraph with a branch per class; ``"fused"``: one persistent kernel whose work-groups
-        have ``groups`` knot groups (0: 32)."""
+        have ``groups`` knot groups (0: tuned on the first solve among 8, 16 and 32 when autotuning is on, else
+        32)."""
         if mode not in ("graph", "fused"):
             raise ValueError(f"mode must be 'graph' or 'fused', not {mode!r}")
         check(self.lib.gato_mixed_set_mode(self._h, MODES[mode], int(groups)), "gato_mixed_set_mode")
         self.mode = mode
+
+    def fused_groups(self):
+        """Knot groups per work-group of the persistent kernel (after tuning, the chosen count)."""
+        mode, g = C.c_int32(), C.c_int32()
+        check(self.lib.gato_mixed_get_mode(self._h, C.byref(mode), C.byref(g)), "gato_mixed_get_mode")
+        return g.value
 
     def class_solver(self, i):
         return self._views[i]

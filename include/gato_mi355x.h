@@ -236,10 +236,14 @@ int gato_mixed_destroy(gato_mixed* m);
 int gato_mixed_class(gato_mixed* m, int32_t i, gato_solver** out);
 int gato_mixed_set_autotune(gato_mixed* m, int32_t enabled);
 /* How gato_mixed_solve runs. GATO_MODE_GRAPH (the default): the hipGraph above. GATO_MODE_FUSED: ONE persistent
- * kernel for all classes; its work-groups (`groups` knot groups of 16 lanes each, 0: 32) take problems from a
- * work list, larger classes (N nx^2) first, and solve each to completion as GATO_MODE_FUSED does for one class,
- * with the class's own exit rule per problem. Every class's N / groups, rounded up, must be 1, 2, 4 or 8. */
+ * kernel for all classes; its work-groups (`groups` knot groups of 16 lanes each) take problems from a work list,
+ * larger classes (N nx^2) first, and solve each to completion as GATO_MODE_FUSED does for one class, with the
+ * class's own exit rule per problem. Every class's N / groups, rounded up, must be 1, 2, 4 or 8. groups = 0: 32,
+ * and unless autotuning is off, the first solve times every valid count of 8, 16 and 32 on its inputs (state
+ * restored) and keeps the fastest. */
 int gato_mixed_set_mode(gato_mixed* m, int32_t mode, int32_t groups);
+/* The current mode and the persistent kernel's knot groups per work-group (tuned or set). */
+int gato_mixed_get_mode(gato_mixed* m, int32_t* mode, int32_t* groups);
 int gato_mixed_solve(gato_mixed* m, float* const* d_xu, const float* const* d_xs, const float* const* d_ref, float dt,
                      void* stream, gato_stats* stats);
 /* Device time of the last mixed launch (graph or persistent kernel; hipEvents on the caller's stream), microseconds. */

@@ -117,6 +117,35 @@ def test_persistent_mixed_kernel_equals_standalone_fused_solves(gpu, groups):
         m.set_mode("fused", groups=8)  # N = 128 over 8 groups is 16 passes
 
 
+def test_persistent_kernel_tuner_keeps_state_and_inputs(gpu):
+    # groups = 0 with autotuning on: the first solve times every valid knot-group count (8, 16 and 32 here) on
+    # scratch copies of the trajectories from the classes' saved state, keeps the fastest and restores the state, so
+    # its results equal those of a solver fixed at the chosen count from the start, bit for bit.
+    from gato_amd.mixed import MixedSolver
+
+    spec = [("iiwa14", 32, 6), ("indy7", 64, 3), ("iiwa14", 16, 4)]
+    wls = [W.drop_in_figure8(p, N=n, B=b, seed=70 + i) for i, (p, n, b) in enumerate(spec)]
+    for wl in wls:
+        wl.params["max_sqp_iters"] = 3
+    kw = {k: wls[0].params[k] for k in SCALARS}
+    m1 = MixedSolver(spec, mode="fused", **kw)
+    ins1 = [_dev(wl, m1.device) for wl in wls]
+    keep = [x[0].clone() for x in ins1]
+    st1 = m1.solve_device([x[0] for x in ins1], [x[1] for x in ins1], [x[2] for x in ins1], stats=True)
+    G = m1.fused_groups()
+    assert G in (8, 16, 32)
+    m2 = MixedSolver(spec, mode="fused", groups=G, **kw)
+    ins2 = [_dev(wl, m2.device) for wl in wls]
+    for a, b in zip(keep, ins2):
+        assert torch.equal(a, b[0])  # the tuner left the inputs as they were before the solve
+    st2 = m2.solve_device([x[0] for x in ins2], [x[1] for x in ins2], [x[2] for x in ins2], stats=True)
+    for i in range(len(spec)):
+        assert torch.equal(ins1[i][0], ins2[i][0]), (i, G)
+        np.testing.assert_array_equal(st1[i]["pcg_iters"], st2[i]["pcg_iters"])
+        np.testing.assert_array_equal(m1.class_solver(i).get_state()[0], m2.class_solver(i).get_state()[0])
+        np.testing.assert_array_equal(m1.class_solver(i).get_state()[1], m2.class_solver(i).get_state()[1])
+
+
 def test_launch_shapes_agree_to_rounding(gpu):
     from gato_amd.solver import Solver
 
