@@ -418,8 +418,9 @@ __host__ __device__ constexpr int pcg_res_lds_floats(int N) {
 // form: the preconditioner's off-diagonal blocks are P_{k+1,k} = -P_{k+1,k+1} So_{k+1} P_{k,k} by construction
 // (schur_linsys.cuh:213-260, precond_offdiag), so with w = Pd r (block diagonal),
 //   (P r)_k = Pd_k r_k + P_{k,k-1} r_{k-1} + P_{k+1,k}^T r_{k+1} = Pd_k (r_k - So_k w_{k-1} - So_{k+1}^T w_{k+1}),
-// exactly in real arithmetic (Pd_k is symmetric); in fp32 it differs from the explicit product by rounding. Three
-// work-group barriers per iteration: the two dot products' and the exchange of w.
+// exactly in real arithmetic (Pd_k is symmetric); in fp32 it differs from the explicit product by rounding. The
+// neighbours' w are carried like r (w -= alpha Pd Ap), so the iteration has two work-group barriers, its two dot
+// products'.
 template <int PL, int NP = 2, int GC = 64>
 __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, float* sm) {
     constexpr int NX = Plant<PL>::NX, NB = NX * NX, XP = 16;
@@ -491,20 +492,27 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
                                  row_bcast_dot<NX>(0.f, x[s], sd[s]));
         r[s] = val[s] ? v.gam[kk[s] * NX + l] - sx : 0.f;
     }
-    // z = P r in factored form: w published for the neighbours (one barrier), z published for the neighbours' copies
-    // of p (pm, pp: this lane's index in knots k-1 and k+1, updated by the owners' operations, so no barrier is
-    // needed to publish p)
-    auto precond = [&](float* zz) {
+    // w = Pd r at this lane's index in knots k-1 and k+1 (wm, wp): published once here (one barrier), then kept up to
+    // date as r is, w -= alpha Pd Ap, with Pd Ap published by its owners before the barrier of p.Ap, so that
+    // z = Pd (r - So_k wm - So_{k+1}^T wp) needs no exchange of its own; z is published for the neighbours' copies
+    // of p (pm, pp), which are updated by the owners' operations, so no barrier publishes p either
+    float wm[NP], wp[NP];
 #pragma unroll
-        for (int s = 0; s < NP; s++) {
-            const float w = row_bcast_dot<NX>(0.f, r[s], pd[s]);
-            if (val[s]) sw[(kk[s] + 1) * XP + l] = w;
-        }
-        __syncthreads();
+    for (int s = 0; s < NP; s++) {
+        const float w = row_bcast_dot<NX>(0.f, r[s], pd[s]);
+        if (val[s]) sw[(kk[s] + 1) * XP + l] = w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NP; s++) {
+        wm[s] = sw[kk[s] * XP + lane];
+        wp[s] = sw[(kk[s] + 2) * XP + lane];
+    }
+    auto precond = [&](float* zz) {
         float loc = 0.f;
 #pragma unroll
         for (int s = 0; s < NP; s++) {
-            const float t = offdiag(s, sw[kk[s] * XP + lane], sw[(kk[s] + 2) * XP + lane], 0.f);
+            const float t = offdiag(s, wm[s], wp[s], 0.f);
             const float u = val[s] ? r[s] - t : 0.f;
             const float zs = row_bcast_dot<NX>(0.f, u, pd[s]);
             zz[s] = val[s] ? zs : 0.f;
@@ -533,6 +541,8 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
             const float a_ = offdiag(s, pm[s], pp[s], row_bcast_dot<NX>(0.f, p[s], sd[s]));
             Ap[s] = val[s] ? a_ : 0.f;
             loc += p[s] * Ap[s];
+            const float q = row_bcast_dot<NX>(0.f, Ap[s], pd[s]);  // Pd Ap, for the neighbours' copies of w
+            if (val[s]) sw[(kk[s] + 1) * XP + l] = q;
         }
         // quotients by reciprocal and product, as the reference's fast-math build divides (-use_fast_math)
         const float alpha = rho * __builtin_amdgcn_rcpf(bsum(loc, red + 64 * slot));
@@ -543,6 +553,11 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
             r[s] -= alpha * Ap[s];
         }
         if (it >= a.max_pcg) break;  // x is final at the cap: its last preconditioner application is not needed
+#pragma unroll
+        for (int s = 0; s < NP; s++) {
+            wm[s] -= alpha * sw[kk[s] * XP + lane];
+            wp[s] -= alpha * sw[(kk[s] + 2) * XP + lane];
+        }
         const float rn = bsum(precond(z), red + 64 * slot);
         slot ^= 1;
         if (fabsf(rn) < a.pcg_abs_tol + eps * rho0) break;
