@@ -173,13 +173,14 @@ def test_pcg_fixed_iteration_count_matches_oracle(gpu, O, plant, iters, tol):
 @pytest.mark.parametrize("plant,N,iters", [("iiwa14", 128, 5), ("iiwa14", 128, 20), ("indy7", 96, 5)])
 def test_graph_long_horizon_pcg_fixed_iteration_count(gpu, O, plant, N, iters):
     # The graph path's PCG for 64 < N <= 128 keeps the problem on its CU and applies P in factored form,
-    # Pd_k (r_k - So_k w_{k-1} - So_{k+1}^T w_{k+1}) with w = Pd r, which equals the explicit P of the Schur phase in
-    # real arithmetic. With the relative tolerance at 0, it and the FP64 oracle's PCG (explicit P, on the GPU's own S,
-    # P and gamma) run exactly `iters` iterations from lambda = 0; so does the plain path's streamed fp32 PCG, which
-    # bounds what fp32 rounding alone moves. Measured: the factored form's iterate is 0.8-4.6x as far from the oracle's
-    # as the plain path's (at most 1.7e-4 relative, iiwa14 N = 128 and indy7 N = 96, 5 and 20 iterations): it rounds
-    # differently (three block products instead of the explicit off-diagonal blocks, P's diagonal blocks taken as
-    # symmetric), not worse in kind. Bound: 10x the plain path's deviation, and 1e-3, the N = 32 bound at 20 iterations.
+    # Pd_k (r_k - So_k w_{k-1} - So_{k+1}^T w_{k+1}) with w = Pd r (the neighbours' w carried by w -= alpha Pd Ap),
+    # which equals the explicit P of the Schur phase in real arithmetic. With the relative tolerance at 0, it and the
+    # FP64 oracle's PCG (explicit P, on the GPU's own S, P and gamma) run exactly `iters` iterations from lambda = 0;
+    # so does the plain path's streamed fp32 PCG, which bounds what fp32 rounding alone moves. Measured: the factored
+    # form's iterate is 0.4-4.7x as far from the oracle's as the plain path's (at most 1.3e-4 relative, iiwa14
+    # N = 128 at 5 and 20 iterations, indy7 N = 96 at 5): it rounds differently (three block products instead of the
+    # explicit off-diagonal blocks, P's diagonal blocks taken as symmetric, w by recurrence), not worse in kind.
+    # Bound: 10x the plain path's deviation, and 1e-3, the N = 32 bound at 20 iterations.
     from gato_amd.solver import Solver
 
     B = 4
@@ -204,4 +205,5 @@ def test_graph_long_horizon_pcg_fixed_iteration_count(gpu, O, plant, N, iters):
         l64, it = O.pcg(nx, N, gs, np.zeros(N * nx), 0.0, iters)
         assert it == iters and int(out[2][1][b]) == iters and int(out[5][1][b]) == iters
         e_plain, e_res = rel_err(out[2][0][b], l64), rel_err(out[5][0][b], l64)
+        print(f"{plant} N={N} iters={iters} problem {b}: plain {e_plain:.3g} resident {e_res:.3g}")
         assert e_res < 10 * max(e_plain, 1e-5) and e_res < 1e-3, (b, e_plain, e_res)
