@@ -240,9 +240,9 @@ def test_graph_equals_plain_when_problems_freeze_and_the_batch_stops(gpu, N):
 
 
 def test_graph_equals_plain_to_rounding_at_long_horizons(gpu):
-    # For 64 < N <= 128 the graph path runs its PCG in 1024-thread work-groups (two knots per knot group), the plain
-    # path in 512-thread ones (four), so the PCG's dot products are summed in a different order and the results
-    # agree to rounding, not bit for bit. Joint-space tracking (well conditioned, the PCG converges before its cap)
+    # For 64 < N <= 128 the graph path runs its PCG held on the CU in 1024-thread work-groups (two knots per knot
+    # group, P applied in factored form, the neighbours' w carried by recurrence), the plain path streams S and P in
+    # 512-thread ones (four), so the results agree to rounding, not bit for bit. Joint-space tracking (well conditioned, the PCG converges before its cap)
     # keeps the step decisions away from ties: the exit decisions, SQP counts and line-search steps must be equal and
     # PCG counts within one. The PCG stops at its cap with cond(S) ~ 1e8 here, which amplifies the different
     # summation order to 1e-4 - 3.3e-4 of the trajectory (measured), so trajectories and merits are held to the fp32
