@@ -433,9 +433,11 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
     float* sz = sw + (N + 2) * XP;  // z
     float* red = sz + (N + 2) * XP;  // 2 x 64 floats
     {
+        // knot 0 has no left neighbour: its block is stored as zeros, so block row 0 needs no test of k > 0 (taken
+        // in the 1024-lane form, GC = 64, where the tests measured faster; the same values either way)
         const float4* src = reinterpret_cast<const float4*>(v.So);
         float4* dst = reinterpret_cast<float4*>(So);
-        for (int i = tid; i < N * NB / 4; i += blockDim.x) dst[i] = src[i];
+        for (int i = tid; i < N * NB / 4; i += blockDim.x) dst[i] = i < NB / 4 ? make_float4(0.f, 0.f, 0.f, 0.f) : src[i];
         for (int i = tid; i < 3 * (N + 2) * XP; i += blockDim.x) sp[i] = 0.f;
     }
     const int l = lane < NX ? lane : 0;
@@ -464,7 +466,7 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
         for (int c = 0; c < NX; c++) uc[s][c] = k < N - 1 ? Uc[c * NX] : 0.f;
     }
     // the off-diagonal blocks of block row kk[s] times a vector whose entries at this lane's index in knots k-1 and k+1
-    // are ym and yp: row l of So_k (k > 0, from LDS), then column l of So_{k+1} (registers), starting from acc
+    // are ym and yp: row l of So_k (from LDS; zeros for k = 0), then column l of So_{k+1} (registers), from acc
     auto offdiag = [&](int s, float ym, float yp, float acc) {
         const int k = kk[s];
         float L[NX];
@@ -472,8 +474,8 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
 #pragma unroll
         for (int c = 0; c < NX / 2; c++) {
             const float2 t = Lr[c];
-            L[2 * c] = k > 0 ? t.x : 0.f;
-            L[2 * c + 1] = k > 0 ? t.y : 0.f;
+            L[2 * c] = GC == 64 && k == 0 ? 0.f : t.x;
+            L[2 * c + 1] = GC == 64 && k == 0 ? 0.f : t.y;
         }
         acc = row_bcast_dot<NX>(acc, ym, L);
         return row_bcast_dot<NX>(acc, yp, uc[s]);
