@@ -1408,15 +1408,6 @@ __global__ void __launch_bounds__(512) GATO_TWO_WG_PER_CU k_pcg_dz(Args a, int i
     pcg_dz_body<PL, NPASS>(a, it);
 }
 
-// The same in 1024-thread work-groups (64 knot groups) for the streamed PCG (NPASS >= 2, 64 < N <= 128): half the
-// knots per lane and twice the waves per problem of the 512-thread launch, at the same 128-VGPR budget (one
-// work-group per CU). For batches too small to fill the GPU with 512-thread work-groups.
-template <int PL, int NPASS>
-__global__ void __launch_bounds__(1024) GATO_TWO_WG_PER_CU k_pcg_dz_wide(Args a, int it) {
-    static_assert(NPASS >= 2, "the register-resident PCG's reductions hold at most 32 knot groups");
-    pcg_dz_body<PL, NPASS>(a, it);
-}
-
 // The graph path's step for 64 < N <= 128 with the PCG held on the CU (phase_pcg_res): 1024 lanes, the problem's
 // off-diagonal blocks of S in LDS (pcg_res_lds_floats), one work-group per CU.
 template <int PL>
