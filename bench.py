@@ -452,6 +452,16 @@ def algorithmic_bytes(wl):
     return 4 * per * wl.B
 
 
+def algorithmic_bytes_backward(wl):
+    """Compulsory HBM bytes of one batched backward pass: XU, x_s, reference and dL/dXU in, the gradients with
+    respect to x_s, the reference, diag Q and diag R out."""
+    nq = 7 if wl.plant == "iiwa14" else 6
+    nx, nu, N = 2 * nq, nq, wl.N
+    traj = (nx + nu) * N - nu
+    per = 2 * traj + nx + 6 * N + (nx + 6 * N + nx + nu)
+    return 4 * per * wl.B
+
+
 def algorithmic_flops(st, N):
     """Nominal FLOPs of the reference iteration per problem-iteration (estimate at iiwa14 N=32,
     scaled linearly in N): 8.5 MFLOP + 0.08 MFLOP per PCG iteration."""
@@ -606,6 +616,14 @@ def main():
         fe, g = time_fwd_bwd(solver, wl, args.fwd_bwd_steps, 1)
         out["fwd_bwd_solves_per_s"] = B * args.fwd_bwd_steps / fe
         out["fwd_bwd_ms_per_step"] = 1e3 * fe / args.fwd_bwd_steps
+        fb_bytes = algorithmic_bytes(wl) + algorithmic_bytes_backward(wl)
+        fb_gbs = fb_bytes * args.fwd_bwd_steps / fe / 1e9
+        out["fwd_bwd_hbm"] = {"compulsory_bytes_per_step": fb_bytes, "achieved_GBs": fb_gbs,
+                              "frac_of_peak": fb_gbs / HBM_PEAK_GBS,
+                              "note": "compulsory bytes only (inputs and outputs of the solve and the backward "
+                                      "pass per step); the KKT, Schur and preconditioner blocks written and re-read "
+                                      "between phases are not counted (measured per kernel with HBM counters); "
+                                      "far below HBM peak: the path is bound by vector-ALU work and barrier latency"}
         out["adjoint_pcg_iters_mean"] = float(g["adj_pcg_iters"].mean())
     if args.ref_mpc_steps > 0:
         out["reference_heatmap_point"] = time_reference_mpc(args.ref_mpc_steps)
