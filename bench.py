@@ -568,6 +568,8 @@ def main():
         del src, dst
     wl = workload(args.knots, args.batch)
     solver = make_solver(wl, args.mode)
+    if args.mode == "fused":  # its best knot-group count depends on the batch (two problems per CU at 16 groups)
+        solver.autotune(wl.xu, wl.xs, wl.ref)
     elapsed, kern, st = time_gpu(solver, wl, args.steps, args.warmup)
     B = wl.B
     value = B * args.steps / elapsed
@@ -595,7 +597,8 @@ def main():
         "config": {"workload": f"{wl.plant} N={wl.N} B={B} figure-8 end-effector tracking, <= "
                                f"{wl.params['max_sqp_iters']} SQP iterations per solve",
                    "plant": wl.plant, "knots": wl.N, "batch": B, "max_sqp_iters": wl.params["max_sqp_iters"],
-                   "mode": args.mode, "parallelism": "single GPU, one work-group per problem"},
+                   "mode": args.mode, "threads_per_problem": solver.launch_shape()[1],
+                   "parallelism": "single GPU, one work-group per problem"},
         "value_counts": "every problem-solve of <= 10 SQP iterations; the KKT <= 1e-4 qualifier holds for "
                         "kkt_converged_fraction of them (in exact arithmetic too, the reference iteration's steps on "
                         "this workload are 10-200x longer than the line search can use, leaving the residual near "
