@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--fwd-bwd-steps", type=int, default=3, help="steps of forward solve + backward pass timed (0: skip)")
     ap.add_argument("--scale-steps", type=int, default=200,
                     help="Adam steps of the scale workload (iiwa14 N=128 B=4096, shared diag Q/R learned through "
-                         "the backward pass) timed (0: skip)")
+                         "closed-loop rollouts of 4 MPC steps and their backward pass) timed (0: skip)")
     ap.add_argument("--fp64-steps", type=int, default=3,
                     help="steps of FP64 mode on the drop-in workload at B=128 timed (0: skip)")
     ap.add_argument("--diff-steps", type=int, default=3,
@@ -202,14 +202,19 @@ def _weights(p):
                               "rho")}
 
 
-def time_scale(steps, check=16):
+def time_scale(steps, check=16, rollout=4, lr=3e-3):
     """iiwa14 N=128 B=4096 figure-8s of seeded radius and phase. Shared log-weights theta = log diag(Q, R) are learned
-    with Adam (lr 1e-2) so that the solutions' end-effector paths track the figure-8s: loss = mean over problems of
-    sum_k |p_k - ref_k|^2 + 1e-4 |u_0|^2 (planned, open-loop paths). One step is a forward solve of all 4096 problems,
-    the end-effector positions, the backward pass and the sum over problems of the per-problem weight gradients (the
-    one reduction the path has). The backward runs with its defaults (the one-step derivative; two refinement rounds,
-    one PCG iteration per round with the factorization of S as preconditioner). After the timed steps the
-    per-problem gradients of `check` problems are compared with the FP64 oracle's at the GPU's solution."""
+    with Adam (lr 3e-3; at 1e-2 the loss turns back up after about 100 steps) so that closed-loop rollouts track the
+    figure-8s. A rollout is `rollout` MPC steps from the
+    problems' initial states: step t solves all 4096 problems from the current states x_t over the reference window
+    starting at knot t, applies the plan on the solver's model (x_{t+1} is the plan's state at knot 1, the model's
+    response to the first control; differentiable through the solve's backward, which gives dL/dx_s), and adds
+    |p(x_{t+1}) - ref_{t+1}|^2 + 1e-4 |u_0|^2, averaged over problems, to the loss. The next solve warm-starts from
+    the plan and from the multipliers and rho the solve left. One Adam step is the rollout's `rollout` forward solves,
+    the end-effector positions, and the backward pass through all of them (each solve's adjoint with its defaults:
+    the one-step derivative; two refinement rounds, one PCG iteration per round with the factorization of S as
+    preconditioner), chained through the states. After the timed steps the per-problem gradients of one solve at the
+    final weights are compared, for `check` problems, with the FP64 oracle's at the GPU's solution."""
     import numpy as np
     import torch
 
@@ -219,17 +224,21 @@ def time_scale(steps, check=16):
     from oracle import oracle as O
 
     wl = W.scale_figure8(N=128, B=4096, seed=4)
+    wl_long = W.scale_figure8(N=128 + rollout, B=4096, seed=4)  # the same problems, the path `rollout` knots longer
     p = wl.params
     s = Solver(wl.plant, wl.N, wl.B, mode="graph", **_weights(p))
     dev = s.device
     t = lambda a: torch.tensor(a, dtype=torch.float32, device=dev).contiguous()
     xs, ref, xu0 = t(wl.xs), t(wl.ref), t(wl.xu)
     B, N, nx, nu = wl.B, wl.N, s.nx, s.nu
+    ref_long = t(wl_long.ref).view(B, N + rollout, 6)
+    windows = [ref_long[:, k:k + N].reshape(B, 6 * N).contiguous() for k in range(rollout)]
+    target = ref_long[:, :, :3]
     ref3 = ref.view(B, N, 6)[:, :, :3]
     theta = torch.zeros(nx + nu, dtype=torch.float32, device=dev, requires_grad=True)
-    opt = torch.optim.Adam([theta], lr=1e-2)
+    opt = torch.optim.Adam([theta], lr=lr)
 
-    def loss_of(XU):
+    def loss_of(XU):  # the open-loop planned path's tracking error, for the gradient check below
         pos = ee_positions(s, XU)
         return (((pos - ref3) ** 2).sum((1, 2)) + 1e-4 * (XU[:, nx:nx + nu] ** 2).sum(1)).mean()
 
@@ -238,9 +247,16 @@ def time_scale(steps, check=16):
     def step():
         opt.zero_grad()
         w = torch.exp(theta)
+        Qd, Rd = w[:nx].expand(B, nx).contiguous(), w[nx:].expand(B, nu).contiguous()
         s.reset_dual()
         s.reset_rho()
-        loss = loss_of(solve(s, xs, ref, w[:nx].expand(B, nx).contiguous(), w[nx:].expand(B, nu).contiguous(), xu0))
+        x, warm, loss = xs, xu0, 0.0
+        for k in range(rollout):
+            XU = solve(s, x, windows[k], Qd, Rd, warm)
+            x = XU[:, nx + nu:2 * nx + nu]  # the plan's state at knot 1
+            pos = ee_positions(s, XU)[:, 1]
+            loss = loss + (((pos - target[:, k + 1]) ** 2).sum(1) + 1e-4 * (XU[:, nx:nx + nu] ** 2).sum(1)).mean()
+            warm = XU.detach()
         loss.backward()
         opt.step()
         losses.append(float(loss.detach()))
@@ -254,13 +270,15 @@ def time_scale(steps, check=16):
             print(f"scale leg: {i + 1} of {steps} Adam steps, loss {losses[-1]:.4f}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    out = {"workload": "iiwa14 N=128 B=4096 figure-8s (seeded radius/phase), shared diag Q/R learned by Adam on the "
-                       "solutions' end-effector tracking error (planned paths, open loop)",
-           "adam_steps": steps, "backward": "defaults: one-step derivative, 2 refinement rounds, 1 PCG iteration per "
-                                            "round (factorization preconditioner)",
-           "fwd_bwd_solves_per_s": B * steps / el, "ms_per_step": 1e3 * el / steps,
+    out = {"workload": "iiwa14 N=128 B=4096 figure-8s (seeded radius/phase), shared diag Q/R learned by Adam so that "
+                       f"closed-loop rollouts of {rollout} MPC steps on the solver's model track them",
+           "adam_steps": steps, "rollout_steps": rollout,
+           "backward": "through every solve of the rollout, chained through the states; each with the defaults: "
+                       "one-step derivative, 2 refinement rounds, 1 PCG iteration per round (factorization "
+                       "preconditioner)",
+           "fwd_bwd_solves_per_s": B * rollout * steps / el, "ms_per_step": 1e3 * el / steps,
            "loss_first": losses[0], "loss_last": losses[-1],
-           "compulsory_hbm_GBs": 4 * (nx + 6 * wl.N + 2 * s.traj + 2 * nx * wl.N + 7) * B * steps / el / 1e9}
+           "compulsory_hbm_GBs": 4 * (nx + 6 * wl.N + 2 * s.traj + 2 * nx * wl.N + 7) * B * rollout * steps / el / 1e9}
     if check > 0:  # per-problem gradients at the final weights against the FP64 oracle's step gradient
         w = torch.exp(theta.detach())
         Qd, Rd = w[:nx].expand(B, nx).contiguous(), w[nx:].expand(B, nu).contiguous()
