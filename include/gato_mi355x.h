@@ -240,7 +240,8 @@ int gato_mixed_set_autotune(gato_mixed* m, int32_t enabled);
  * larger classes (N nx^2) first, and solve each to completion as GATO_MODE_FUSED does for one class, with the
  * class's own exit rule per problem. Every class's N / groups, rounded up, must be 1, 2, 4 or 8. groups = 0: 32,
  * and unless autotuning is off, the first solve times every valid count of 8, 16 and 32 on its inputs (state
- * restored) and keeps the fastest. */
+ * restored) and keeps the fastest. The work list's counters belong to the mixed solver: its solves must not
+ * overlap (gato_mixed_solve returns after the solve, so calls from one thread never do). */
 int gato_mixed_set_mode(gato_mixed* m, int32_t mode, int32_t groups);
 /* The current mode and the persistent kernel's knot groups per work-group (tuned or set). */
 int gato_mixed_get_mode(gato_mixed* m, int32_t* mode, int32_t* groups);
