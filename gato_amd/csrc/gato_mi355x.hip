@@ -1027,11 +1027,14 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
     *n2 = it2;
 }
 
-// register-resident PCG when every knot group owns a single knot (N <= 32), else S and P stream
+// register-resident PCG when every knot group owns a single knot (N <= 32); with two knots per group the problem is
+// held on the CU (phase_pcg_res over blockDim.x / 16 <= 32 groups; the caller provides pcg_res_lds_floats(N) of LDS,
+// at most 64 KB for N <= 64); else S and P stream
 template <int PL, int NPASS>
 __device__ __forceinline__ int pcg_dispatch(const Args& a, int b, float eps, float* sm) {
     if constexpr (NPASS == 1) return phase_pcg_reg<PL>(a, b, eps, sm);
     else if constexpr (NPASS == 0) return phase_pcg_res<PL>(a, b, eps, sm);
+    else if constexpr (NPASS == 2) return phase_pcg_res<PL, 2, 0>(a, b, eps, sm);
     else return phase_pcg<PL, NPASS>(a, b, eps, sm);
 }
 
@@ -2680,6 +2683,16 @@ size_t lds_bytes_g(const gato_solver* s, int G) {
 
 size_t lds_bytes(const gato_solver* s) { return lds_bytes_g(s, s->G); }
 
+// LDS of the plain and graph paths' PCG launches (k_pcg, k_pcg_dz): with two knots per knot group the PCG is held on
+// the CU (pcg_dispatch), which needs pcg_res_lds_floats(N), at most 64 KB for N <= 64
+size_t pcg_lds_bytes(const gato_solver* s) {
+    size_t f = lds_bytes(s);
+    if ((s->N + s->G - 1) / s->G == 2)
+        f = std::max(f, (size_t)(s->plant == GATO_PLANT_IIWA14 ? gato::pcg_res_lds_floats<0>(s->N)
+                                                             : gato::pcg_res_lds_floats<1>(s->N)) * sizeof(float));
+    return f;
+}
+
 // LDS of k_fused and k_fused_mixed for G knot groups: the phases', or with two or four knots per group the resident
 // PCG's (pcg_dispatch_fused) where that is larger
 size_t fused_lds_bytes(const gato_solver* s, int G) {
@@ -2742,7 +2755,7 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     if (iters >= 1 && N <= 32 && N * gato::NUM_ALPHAS <= 256 && blk.x >= 256) {
         hipLaunchKernelGGL((gato::k_first_n<PL>), dim3(B), dim3(256), lds, st, a);
         for (int it = 0; it < iters; it++) {
-            launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
+            launch_pcg_dz<PL>(np, B, blk, pcg_lds_bytes(s), st, a, it);
             if (it + 1 < iters)
                 hipLaunchKernelGGL((gato::k_ls_lin_schur_n<PL>), dim3(B), dim3(256), lds, st, a, it + 1);
             else
@@ -2764,7 +2777,7 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
         if (wide)
             hipLaunchKernelGGL((gato::k_pcg_dz_res<PL>), dim3(B), dim3(1024), lds_res, st, a, it);
         else
-            launch_pcg_dz<PL>(np, B, blk, lds, st, a, it);
+            launch_pcg_dz<PL>(np, B, blk, pcg_lds_bytes(s), st, a, it);
         launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 3, it);
     }
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
@@ -2846,7 +2859,7 @@ int launch_solve(gato_solver* s, hipStream_t st) {
         hipLaunchKernelGGL((gato::k_lin<PL>), dim3(B), blk, lds, st, a);
         launch_schur<PL>(B, blk, lds, st, a);
         HIP_TRY(hipEventRecord(s->ev0, st));
-        launch_pcg<PL>(np, B, blk, lds, st, a, it);
+        launch_pcg<PL>(np, B, blk, pcg_lds_bytes(s), st, a, it);
         HIP_TRY(hipEventRecord(s->ev1, st));
         hipLaunchKernelGGL((gato::k_dz<PL>), dim3(B), blk, lds, st, a, it);
         hipLaunchKernelGGL(gato::k_check, dim3(1), dim3(check_threads(B)), 0, st, a, it);
@@ -3920,8 +3933,8 @@ int gato_debug_phase(gato_solver* s, int32_t phase, float* d_xu, const float* d_
             else launch_schur<1>(s->B, blk, lds, st, a);
             break;
         case 2:
-            if (ii) launch_pcg<0>(np, s->B, blk, lds, st, a, 0);
-            else launch_pcg<1>(np, s->B, blk, lds, st, a, 0);
+            if (ii) launch_pcg<0>(np, s->B, blk, pcg_lds_bytes(s), st, a, 0);
+            else launch_pcg<1>(np, s->B, blk, pcg_lds_bytes(s), st, a, 0);
             break;
         case 3:
             if (ii) hipLaunchKernelGGL(gato::k_dz<0>, dim3(s->B), blk, lds, st, a, 0);
