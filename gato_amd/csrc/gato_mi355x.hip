@@ -2607,16 +2607,21 @@ size_t layout(gato_solver* s, bool assign, bool forward_only = false) {
 
 // Launch shape: G 16-lane knot groups per work-group (a multiple of 4: whole wavefronts, since the DPP
 // reductions read all 64 lanes of every wave; surplus groups own no knot). Each group owns knots
-// g, g + G, ...: npass = ceil(N / G) in {1, 2, 4, 8} (PCG keeps npass rows per lane).
+// g, g + G, ...: ceil(N / G) of them, at most 8, and the kernels are instantiated for 1, 2, 4 and 8 passes over
+// them (PCG keeps a row per pass and lane), so ceil(N / G) is rounded up to the next of those (a pass whose knot
+// is beyond N owns nothing).
+int passes(int N, int G) {
+    const int np = (N + G - 1) / G;
+    return np <= 2 ? np : np <= 4 ? 4 : 8;
+}
 int default_groups(int N) { return 4 * ((std::min(N, 32) + 3) / 4); }
 bool valid_groups(int N, int G) {
     if (G < 4 || G > 32 || G % 4) return false;
-    const int np = (N + G - 1) / G;
-    return (np == 1 || np == 2 || np == 4 || np == 8) && G <= default_groups(N);
+    return (N + G - 1) / G <= 8 && G <= default_groups(N);
 }
 int grid_threads(const gato_solver* s) { return gato::KG * s->G; }
 int check_threads(int B) { return std::min(1024, 64 * ((B + 63) / 64)); }
-int npass(const gato_solver* s) { return (s->N + s->G - 1) / s->G; }
+int npass(const gato_solver* s) { return passes(s->N, s->G); }
 
 template <int PL>
 void launch_schur(int B, dim3 blk, size_t lds, hipStream_t st, const gato::Args& a) {
@@ -2687,7 +2692,7 @@ size_t lds_bytes(const gato_solver* s) { return lds_bytes_g(s, s->G); }
 // the CU (pcg_dispatch), which needs pcg_res_lds_floats(N), at most 64 KB for N <= 64
 size_t pcg_lds_bytes(const gato_solver* s) {
     size_t f = lds_bytes(s);
-    if ((s->N + s->G - 1) / s->G == 2)
+    if (npass(s) == 2)
         f = std::max(f, (size_t)(s->plant == GATO_PLANT_IIWA14 ? gato::pcg_res_lds_floats<0>(s->N)
                                                              : gato::pcg_res_lds_floats<1>(s->N)) * sizeof(float));
     return f;
@@ -2697,7 +2702,7 @@ size_t pcg_lds_bytes(const gato_solver* s) {
 // PCG's (pcg_dispatch_fused) where that is larger
 size_t fused_lds_bytes(const gato_solver* s, int G) {
     size_t f = lds_bytes_g(s, G);
-    const int np = (s->N + G - 1) / G;
+    const int np = passes(s->N, G);
     if (np == 2 || np == 4) {
         const int r = s->plant == GATO_PLANT_IIWA14 ? gato::pcg_res_lds_floats<0>(s->N) : gato::pcg_res_lds_floats<1>(s->N);
         f = std::max(f, (size_t)r * sizeof(float));
@@ -4026,7 +4031,7 @@ int gato_backward(gato_solver* s, const float* d_xu, const float* d_xs, const fl
 int gato_set_launch_shape(gato_solver* s, int32_t groups) {
     if (!s) return fail(GATO_E_ARG, "null solver");
     if (!valid_groups(s->N, groups))
-        return fail(GATO_E_ARG, "groups must be a multiple of 4 in [4, 32], at most the default, with ceil(N / groups) in {1, 2, 4, 8}");
+        return fail(GATO_E_ARG, "groups must be a multiple of 4 in [4, 32], at most the default, with ceil(N / groups) <= 8");
     s->G = groups;
     return 0;
 }
@@ -4141,14 +4146,12 @@ int gato_mixed_create(gato_mixed** out, int32_t nclass, const int32_t* plants, c
     return 0;
 }
 
-// Whether the persistent kernel can take every class at G knot groups per work-group: N / G rounded up is 1, 2, 4
-// or 8 for each (its knot passes), G a multiple of 4 in [4, 32].
+// Whether the persistent kernel can take every class at G knot groups per work-group: at most 8 knots per group for
+// each (N / G rounded up), G a multiple of 4 in [4, 32].
 static bool mixed_groups_ok(const gato_mixed* m, int G) {
     if (G < 4 || G > 32 || G % 4) return false;
-    for (auto* c : m->cls) {
-        const int np = (c->N + G - 1) / G;
-        if (np != 1 && np != 2 && np != 4 && np != 8) return false;
-    }
+    for (auto* c : m->cls)
+        if ((c->N + G - 1) / G > 8) return false;
     return true;
 }
 
@@ -4157,7 +4160,7 @@ static int mixed_use_groups(gato_mixed* m, int G) {
     std::vector<int2> items(m->items);
     for (auto& it : items) {
         const gato_solver* c = m->cls[it.x];
-        it.x |= ((c->plant == GATO_PLANT_IIWA14 ? 0 : 16) + (c->N + G - 1) / G) << 8;
+        it.x |= ((c->plant == GATO_PLANT_IIWA14 ? 0 : 16) + passes(c->N, G)) << 8;
     }
     HIP_TRY(hipMemcpy(m->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
     m->fgroups = G;
@@ -4172,7 +4175,7 @@ int gato_mixed_set_mode(gato_mixed* m, int32_t mode, int32_t groups) {
         const int G = groups ? groups : 32;
         if (!mixed_groups_ok(m, G))
             return fail(GATO_E_ARG, "groups " + std::to_string(G) + ": need a multiple of 4 in [4, 32] (0: tuned, else "
-                                    "32) that gives every class 1, 2, 4 or 8 knots per knot group");
+                                    "32) that gives every class at most 8 knots per knot group");
         if (int rc = allow_res_lds()) return rc;
         if (int rc = mixed_use_groups(m, G)) return rc;
         m->fgroups_fixed = groups != 0;

@@ -214,7 +214,7 @@ int gato_set_backward_params(gato_solver* s, int32_t max_pcg_iters, float pcg_to
 int gato_set_backward_implicit(gato_solver* s, int32_t rounds, float fd_step);
 
 /* Extension: launch shape. `groups` 16-lane knot groups per work-group (a multiple of 4 in [4, 32], at most
- * the default 4*ceil(min(N,32)/4), with ceil(N / groups) in {1, 2, 4, 8}); threads = 16 * groups. */
+ * the default 4*ceil(min(N,32)/4), with ceil(N / groups) <= 8); threads = 16 * groups. */
 int gato_set_launch_shape(gato_solver* s, int32_t groups);
 int gato_get_launch_shape(gato_solver* s, int32_t* groups, int32_t* threads);
 /* Extension: launch-shape tuner. Times one whole solve per valid shape on these inputs (the trajectory is
@@ -238,7 +238,7 @@ int gato_mixed_set_autotune(gato_mixed* m, int32_t enabled);
 /* How gato_mixed_solve runs. GATO_MODE_GRAPH (the default): the hipGraph above. GATO_MODE_FUSED: ONE persistent
  * kernel for all classes; its work-groups (`groups` knot groups of 16 lanes each) take problems from a work list,
  * larger classes (N nx^2) first, and solve each to completion as GATO_MODE_FUSED does for one class, with the
- * class's own exit rule per problem. Every class's N / groups, rounded up, must be 1, 2, 4 or 8. groups = 0: 32,
+ * class's own exit rule per problem. Every class's N / groups, rounded up, must be at most 8. groups = 0: 32,
  * and unless autotuning is off, the first solve times every valid count of 8, 16 and 32 on its inputs (state
  * restored) and keeps the fastest. The work list's counters belong to the mixed solver: its solves must not
  * overlap (gato_mixed_solve returns after the solve, so calls from one thread never do). */

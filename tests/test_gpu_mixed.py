@@ -66,17 +66,17 @@ def test_mixed_solve_equals_standalone_solves(gpu, tune):
 
 @pytest.mark.parametrize("groups", [32, 16])
 def test_persistent_mixed_kernel_equals_standalone_fused_solves(gpu, groups):
-    # One k_fused_mixed launch over seven classes (both plants; N = 2 to 128, the shortest with a single problem; one
-    # to eight knots per knot group), against stand-alone fused solves of each class. Where the class's own default
-    # knot-group count equals the persistent kernel's (N = 32 and 128 at 32 groups, N = 16 at 16) the results are
-    # equal bit for bit; N = 8 and N = 2 run with more groups than knots there (idle groups add zeros to the sums),
+    # One k_fused_mixed launch over eight classes (both plants; N = 2 to 128, the shortest with a single problem; one
+    # to eight knots per knot group, three of them at N = 96 and 32 groups, run as four passes), against stand-alone
+    # fused solves of each class. Where the class's own default knot-group count equals the persistent kernel's
+    # (N = 32, 96 and 128 at 32 groups, N = 16 at 16) the results are equal bit for bit; N = 8 and N = 2 run with more groups than knots there (idle groups add zeros to the sums),
     # checked to rounding against their default shapes. A second launch after resetting the classes' state repeats the first exactly (the work-list
     # counters reset themselves).
     from gato_amd.mixed import MixedSolver
     from gato_amd.solver import Solver
 
     spec = [("iiwa14", 32, 6), ("indy7", 128, 3), ("indy7", 32, 5), ("iiwa14", 128, 2), ("iiwa14", 8, 4),
-            ("indy7", 16, 3), ("indy7", 2, 1)]
+            ("indy7", 16, 3), ("indy7", 2, 1), ("iiwa14", 96, 2)]
     wls = [W.drop_in_figure8(p, N=n, B=b, seed=50 + i) for i, (p, n, b) in enumerate(spec)]
     for wl in wls:
         wl.params["max_sqp_iters"] = 4
