@@ -622,20 +622,9 @@ __global__ void __launch_bounds__(128) k_btd_factor(Args a, float* Gm, float* Di
     float* Dp = Di + base;
     float* t1 = tiles[side][0];
     float* t2 = tiles[side][1];
-    // in-place Gauss-Jordan inverse, lane = row, the pivot row broadcast from lane p with readlane
-    auto invert = [&](float* row) {
-#pragma unroll
-        for (int p = 0; p < NX; p++) {
-            const float inv = __builtin_amdgcn_rcpf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(row[p]), p)));
-            float pr[NX];
-#pragma unroll
-            for (int c = 0; c < NX; c++)
-                pr[c] = c == p ? inv : __int_as_float(__builtin_amdgcn_readlane(__float_as_int(row[c]), p)) * inv;
-            const float f = row[p];
-#pragma unroll
-            for (int c = 0; c < NX; c++) row[c] = lane == p ? pr[c] : (c == p ? 0.f : row[c]) - f * pr[c];
-        }
-    };
+    // in-place Gauss-Jordan inverse, lane = row (lanes 0-15 of the wave: one DPP row), the pivot row broadcast by
+    // DPP (gj_pivot_dpp)
+    auto invert = [&](float* row) { gj_dpp<NX>(lane, row, std::make_integer_sequence<int, NX>{}); };
     auto store = [&](float* dst, const float* v) {
         if (lane < NX) {
 #pragma unroll
@@ -732,17 +721,15 @@ __global__ void __launch_bounds__(128) k_btd_factor(Args a, float* Gm, float* Di
 // btd_ends (both ends towards m: top y_k = r_k - G_k y_{k-1}, w_k = D_k^-1 y_k for k < m; bottom u_k = r_k -
 // H_k u_{k+1}, v_k = E_k^-1 u_k for k > m; y_m and u_m to LDS), btd_middle (z_m = C_m^-1 (y_m + u_m - r_m)) and
 // btd_out (from m outwards: z_k = w_k - G_{k+1}^T z_{k+1} above, z_k = v_k - H_{k-1}^T z_{k-1} below). Each side
-// is one 16-lane knot group (lane = row) starting at lane `base` of its wave; each step's vector is broadcast
-// from the group's lanes with readlane and the next step's rows are read a step ahead. rv, zv: padded LDS
+// is one 16-lane knot group (lane = row; knot groups are the waves' DPP rows); each step's block row times the vector takes the vector's entries from the group's lanes by DPP row broadcasts
+// (row_bcast_dot, summing in column order) and the next step's rows are read a step ahead. rv, zv: padded LDS
 // vectors (knot k at (k + 1) * 16); Gp, Dp: the problem's factor blocks (in LDS in the paired PCG); pv: 16
 // floats of LDS for the side's vector at m.
 template <int PL>
-__device__ __forceinline__ void btd_ends(bool top, int lane, int base, int N, const float* Gp, const float* Dp,
+__device__ __forceinline__ void btd_ends(bool top, int lane, int N, const float* Gp, const float* Dp,
                                          const float* rv, float* zv, float* pv) {
     constexpr int NX = Plant<PL>::NX, XP = 16;
     const int blk = NX * NX, r = lane < NX ? lane : 0, m = N / 2;
-    base = __builtin_amdgcn_readfirstlane(base);
-    auto bcast = [&](float v, int c) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), base + c)); };
     const int steps = top ? m + 1 : N - m;
     auto knot = [&](int s) { return top ? s : N - 1 - s; };
     auto mult = [&](int k) { const int g = top ? k : k + 1; return g < N ? g : N - 1; };  // G_k or H_k
@@ -763,14 +750,10 @@ __device__ __forceinline__ void btd_ends(bool top, int lane, int base, int N, co
             dn[c] = Dp[kn * blk + r * NX + c];
         }
         const float rn = rv[(kn + 1) * XP + r];
-        float t = 0.f;
-#pragma unroll
-        for (int c = 0; c < NX; c++) t += gc[c] * bcast(y, c);  // gc = 0 at the first step
+        const float t = row_bcast_dot<NX>(0.f, y, gc);  // gc = 0 at the first step
         y = rc - t;
         if (k != m) {
-            float w = 0.f;
-#pragma unroll
-            for (int c = 0; c < NX; c++) w += dc[c] * bcast(y, c);
+            const float w = row_bcast_dot<NX>(0.f, y, dc);
             if (lane < NX) zv[(k + 1) * XP + lane] = w;
         } else if (lane < NX) {
             pv[lane] = y;
@@ -785,25 +768,22 @@ __device__ __forceinline__ void btd_ends(bool top, int lane, int base, int N, co
 }
 
 template <int PL>
-__device__ __forceinline__ void btd_middle(int lane, int base, int N, const float* Dp, const float* rv, float* zv,
+__device__ __forceinline__ void btd_middle(int lane, int N, const float* Dp, const float* rv, float* zv,
                                            const float* pt, const float* pb) {
     constexpr int NX = Plant<PL>::NX, XP = 16;
     const int r = lane < NX ? lane : 0, m = N / 2;
-    base = __builtin_amdgcn_readfirstlane(base);
     const float t = pt[r] + pb[r] - rv[(m + 1) * XP + r];
-    float z = 0.f;
+    float dm[NX];
 #pragma unroll
-    for (int c = 0; c < NX; c++)
-        z += Dp[m * NX * NX + r * NX + c] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t), base + c));
+    for (int c = 0; c < NX; c++) dm[c] = Dp[m * NX * NX + r * NX + c];
+    const float z = row_bcast_dot<NX>(0.f, t, dm);
     if (lane < NX) zv[(m + 1) * XP + lane] = z;
 }
 
 template <int PL>
-__device__ __forceinline__ void btd_out(bool top, int lane, int base, int N, const float* Gp, float* zv) {
+__device__ __forceinline__ void btd_out(bool top, int lane, int N, const float* Gp, float* zv) {
     constexpr int NX = Plant<PL>::NX, XP = 16;
     const int blk = NX * NX, r = lane < NX ? lane : 0, m = N / 2;
-    base = __builtin_amdgcn_readfirstlane(base);
-    auto bcast = [&](float v, int c) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), base + c)); };
     const int steps = top ? m : N - 1 - m;
     if (steps <= 0) return;
     auto knot = [&](int s) { return top ? m - 1 - s : m + 1 + s; };
@@ -816,9 +796,7 @@ __device__ __forceinline__ void btd_out(bool top, int lane, int base, int N, con
         const int k = knot(s), kn = knot(s + 1 < steps ? s + 1 : s);
 #pragma unroll
         for (int j = 0; j < NX; j++) gn[j] = Gp[col(kn) * blk + j * NX + r];
-        float t = 0.f;
-#pragma unroll
-        for (int j = 0; j < NX; j++) t += gc[j] * bcast(x, j);
+        const float t = row_bcast_dot<NX>(0.f, x, gc);
         x = zv[(k + 1) * XP + r] - t;
         if (lane < NX) zv[(k + 1) * XP + lane] = x;
 #pragma unroll
@@ -832,13 +810,13 @@ __device__ __forceinline__ void btd_out(bool top, int lane, int base, int N, con
 template <int PL>
 __device__ void btd_apply(int N, const float* Gp, const float* Dp, const float* rv, float* zv, float* pv) {
     const int g = threadIdx.x / KG, lane = threadIdx.x % KG, gb = blockDim.x / KG >= 8 ? 4 : 1;
-    if (g == 0) btd_ends<PL>(true, lane, 0, N, Gp, Dp, rv, zv, pv);
-    if (g == gb) btd_ends<PL>(false, lane, (gb & 3) * KG, N, Gp, Dp, rv, zv, pv + 16);
+    if (g == 0) btd_ends<PL>(true, lane, N, Gp, Dp, rv, zv, pv);
+    if (g == gb) btd_ends<PL>(false, lane, N, Gp, Dp, rv, zv, pv + 16);
     __syncthreads();
-    if (g == 0) btd_middle<PL>(lane, 0, N, Dp, rv, zv, pv, pv + 16);
+    if (g == 0) btd_middle<PL>(lane, N, Dp, rv, zv, pv, pv + 16);
     __syncthreads();
-    if (g == 0) btd_out<PL>(true, lane, 0, N, Gp, zv);
-    if (g == gb) btd_out<PL>(false, lane, (gb & 3) * KG, N, Gp, zv);
+    if (g == 0) btd_out<PL>(true, lane, N, Gp, zv);
+    if (g == gb) btd_out<PL>(false, lane, N, Gp, zv);
     __syncthreads();
 }
 
@@ -935,24 +913,15 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
     const bool rtop = (role & 1) == 0;
     const float* rrv = role < 2 ? sr1 : sr2;
     float* rzv = role < 2 ? sz1 : sz2;
-    // With 16 groups or more each sweep side has a wave of its own (lanes 0..15). Groups that share a wave (fewer
-    // groups) take turns, one wave slot q at a time: the sweeps broadcast with readlane from lanes base + c, so
-    // only one group of a wave may run them at once.
-    const int nslot = G >= 16 ? 1 : 4;
+    // With 16 groups or more each sweep side has a wave of its own. The sweeps' block products broadcast within the
+    // knot group's DPP row, so sides whose groups share a wave (fewer groups) run side by side too.
     auto precond = [&](bool d1, bool d2, float& z1o, float& z2o) {
         const bool mine = role >= 0 && (role < 2 ? d1 : d2);
-#pragma unroll 1
-        for (int q = 0; q < nslot; q++)
-            if (mine && (k & 3) == q) btd_ends<PL>(rtop, lane, q * KG, N, sG, sD, rrv, rzv, pv + role * XP);
+        if (mine) btd_ends<PL>(rtop, lane, N, sG, sD, rrv, rzv, pv + role * XP);
         __syncthreads();
-#pragma unroll 1
-        for (int q = 0; q < nslot; q++)
-            if (mine && rtop && (k & 3) == q)
-                btd_middle<PL>(lane, q * KG, N, sD, rrv, rzv, pv + role * XP, pv + (role + 1) * XP);
+        if (mine && rtop) btd_middle<PL>(lane, N, sD, rrv, rzv, pv + role * XP, pv + (role + 1) * XP);
         __syncthreads();
-#pragma unroll 1
-        for (int q = 0; q < nslot; q++)
-            if (mine && (k & 3) == q) btd_out<PL>(rtop, lane, q * KG, N, sG, rzv);
+        if (mine) btd_out<PL>(rtop, lane, N, sG, rzv);
         __syncthreads();
         z1o = (val && d1) ? sz1[o] : 0.f;
         z2o = (val && d2) ? sz2[o] : 0.f;
