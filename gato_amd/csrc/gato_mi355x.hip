@@ -611,7 +611,7 @@ __device__ __forceinline__ float2 block_sum2(float v1, float v2, float* red) {
 template <int PL>
 __global__ void __launch_bounds__(128) k_btd_factor(Args a, float* Gm, float* Di) {
     constexpr int NX = Plant<PL>::NX, TS = 16;
-    __shared__ float tiles[2][2][NX * TS];
+    __shared__ float tiles[2][NX * TS];  // the bottom side's transpose of So_{k+1}
     __shared__ float pivot[2][NX * TS];  // D_m (top) and E_m (bottom)
     const int b = blockIdx.x, side = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int N = a.N, m = N / 2, r = lane < NX ? lane : 0;
@@ -620,8 +620,7 @@ __global__ void __launch_bounds__(128) k_btd_factor(Args a, float* Gm, float* Di
     const float* So = a.So + base;
     float* Gp = Gm + base;
     float* Dp = Di + base;
-    float* t1 = tiles[side][0];
-    float* t2 = tiles[side][1];
+    float* t2 = tiles[side];
     // in-place Gauss-Jordan inverse, lane = row (lanes 0-15 of the wave: one DPP row), the pivot row broadcast by
     // DPP (gj_pivot_dpp)
     auto invert = [&](float* row) { gj_dpp<NX>(lane, row, std::make_integer_sequence<int, NX>{}); };
@@ -663,44 +662,33 @@ __global__ void __launch_bounds__(128) k_btd_factor(Args a, float* Gm, float* Di
             }
             invert(row);  // D_{k-1}^-1 (top) or E_{k+1}^-1 (bottom)
             store(Dp + (side == 0 ? k - 1 : k + 1) * blk, row);
-            wave_sync();
-            if (lane < NX) {
+            // the block products by DPP row broadcasts from the group's lanes (lane j holds row j of the inverse and
+            // of So), each sum over j in order as the products were formed before
+            float t[NX];
 #pragma unroll
-                for (int c = 0; c < NX; c++) {
-                    t1[lane * TS + c] = row[c];
-                    t2[lane * TS + c] = so[c];
-                }
+            for (int c = 0; c < NX; c++) {
+                g[c] = 0.f;
+                t[c] = 0.f;
             }
-            wave_sync();
+            if (side == 0) {
+                row_bcast_dots_all<NX, NX>(g, row, so);  // G_k = So_k D_{k-1}^-1: g[c] = sum_j So_k[r][j] D^-1[j][c]
+                lane_dots_all<NX>(t, so, g);             // (G_k So_k^T)[r][c] = sum_j G_k[r][j] So_k[c][j]
+            } else {
+                // H_k = So_{k+1}^T E_{k+1}^-1 needs column r of So_{k+1}: a transpose through the wave's LDS tile
+                if (lane < NX) {
 #pragma unroll
-            for (int c = 0; c < NX; c++) g[c] = 0.f;
-            if (side == 0) {  // G_k = So_k D_{k-1}^-1: row r = sum_j So_k[r][j] D^-1[j][.]
-#pragma unroll
-                for (int j = 0; j < NX; j++) {
-#pragma unroll
-                    for (int c = 0; c < NX; c++) g[c] += so[j] * t1[j * TS + c];
+                    for (int c = 0; c < NX; c++) t2[lane * TS + c] = so[c];
                 }
-            } else {  // H_k = So_{k+1}^T E_{k+1}^-1: row r = sum_j So_{k+1}[j][r] E^-1[j][.]
+                wave_sync();
+                float sc[NX];
 #pragma unroll
-                for (int j = 0; j < NX; j++) {
-                    const float sj = t2[j * TS + r];
-#pragma unroll
-                    for (int c = 0; c < NX; c++) g[c] += sj * t1[j * TS + c];
-                }
+                for (int j = 0; j < NX; j++) sc[j] = t2[j * TS + r];
+                row_bcast_dots_all<NX, NX>(g, row, sc);  // g[c] = sum_j So_{k+1}[j][r] E^-1[j][c]
+                row_bcast_dots_all<NX, NX>(t, so, g);    // (H_k So_{k+1})[r][c] = sum_j H_k[r][j] So_{k+1}[j][c]
             }
             store(Gp + (size_t)(side == 0 ? k : k + 1) * blk, g);
 #pragma unroll
-            for (int c = 0; c < NX; c++) {
-                float t = 0.f;
-                if (side == 0) {  // (G_k So_k^T)[r][c] = sum_j G_k[r][j] So_k[c][j]
-#pragma unroll
-                    for (int j = 0; j < NX; j++) t += g[j] * t2[c * TS + j];
-                } else {  // (H_k So_{k+1})[r][c] = sum_j H_k[r][j] So_{k+1}[j][c]
-#pragma unroll
-                    for (int j = 0; j < NX; j++) t += g[j] * t2[j * TS + c];
-                }
-                row[c] = sd[c] - t;
-            }
+            for (int c = 0; c < NX; c++) row[c] = sd[c] - t[c];
             wave_sync();
         }
         if (lane < NX) {

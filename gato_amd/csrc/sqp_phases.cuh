@@ -14,6 +14,7 @@
 // the position block of Q_k (nq x nq) is row-major, the velocity block and R_k are diagonal.
 // Q~_k = Q_k + rho diag(1_nq, 0) (linalg.cuh:85-96): its inverse is blkdiag(Qip, 1/Qv).
 #pragma once
+#include <type_traits>
 #include <utility>
 
 #include "dynamics.cuh"
@@ -374,6 +375,180 @@ template <int P>
 __device__ __forceinline__ float row_lane(float x) {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x150 + P, 0xF, 0xF,
                                                                  false));
+}
+// s[k] += sum_j x[j] of lane C_k of this lane's 16-lane row times m[j], j = 0 .. NX - 1 in order: the broadcast lane is
+// fixed per chain and the broadcast register runs along it (the row of M times the row of another lane: M X^T with
+// both matrices held a row per lane), four chains per asm block (two for the last two chains of NX = 14).
+template <int NX, int C0, int C1, int C2, int C3>
+__device__ __forceinline__ std::enable_if_t<NX == 14> lane_dots4(float* s, const float* x, const float* m) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, %4, %18 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %4, %18 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %4, %18 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %4, %18 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %5, %19 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %5, %19 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %5, %19 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %5, %19 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %6, %20 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %6, %20 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %6, %20 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %6, %20 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %7, %21 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %7, %21 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %7, %21 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %7, %21 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %8, %22 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %8, %22 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %8, %22 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %8, %22 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %9, %23 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %9, %23 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %9, %23 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %9, %23 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %10, %24 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %10, %24 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %10, %24 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %10, %24 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %11, %25 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %11, %25 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %11, %25 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %11, %25 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %12, %26 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %12, %26 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %12, %26 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %12, %26 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %13, %27 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %13, %27 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %13, %27 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %13, %27 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %14, %28 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %14, %28 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %14, %28 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %14, %28 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %15, %29 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %15, %29 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %15, %29 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %15, %29 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %16, %30 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %16, %30 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %16, %30 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %16, %30 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %17, %31 row_newbcast:%32 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %17, %31 row_newbcast:%33 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %17, %31 row_newbcast:%34 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %17, %31 row_newbcast:%35 row_mask:0xf bank_mask:0xf\n"
+        "s_nop 1\n"
+        : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3])
+        : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7]), "v"(x[8]), "v"(x[9]), "v"(x[10]), "v"(x[11]), "v"(x[12]), "v"(x[13]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]), "i"(C0), "i"(C1), "i"(C2), "i"(C3));
+}
+template <int NX, int C0, int C1>
+__device__ __forceinline__ std::enable_if_t<NX == 14> lane_dots2(float* s, const float* x, const float* m) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, %2, %16 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %2, %16 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %3, %17 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %3, %17 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %4, %18 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %4, %18 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %5, %19 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %5, %19 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %6, %20 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %6, %20 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %7, %21 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %7, %21 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %8, %22 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %8, %22 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %9, %23 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %9, %23 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %10, %24 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %10, %24 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %11, %25 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %11, %25 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %12, %26 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %12, %26 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %13, %27 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %13, %27 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %14, %28 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %14, %28 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %15, %29 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %15, %29 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "s_nop 1\n"
+        : "+v"(s[0]), "+v"(s[1])
+        : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7]), "v"(x[8]), "v"(x[9]), "v"(x[10]), "v"(x[11]), "v"(x[12]), "v"(x[13]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "v"(m[12]), "v"(m[13]), "i"(C0), "i"(C1));
+}
+template <int NX, int C0, int C1, int C2, int C3>
+__device__ __forceinline__ std::enable_if_t<NX == 12> lane_dots4(float* s, const float* x, const float* m) {
+    asm volatile(
+        "s_nop 1\n"
+        "v_fmac_f32_dpp %0, %4, %16 row_newbcast:%28 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %4, %16 row_newbcast:%29 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %4, %16 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %4, %16 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %5, %17 row_newbcast:%28 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %5, %17 row_newbcast:%29 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %5, %17 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %5, %17 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %6, %18 row_newbcast:%28 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %6, %18 row_newbcast:%29 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %6, %18 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %6, %18 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %7, %19 row_newbcast:%28 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %7, %19 row_newbcast:%29 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %7, %19 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %7, %19 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %8, %20 row_newbcast:%28 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %8, %20 row_newbcast:%29 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %8, %20 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %8, %20 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %9, %21 row_newbcast:%28 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %9, %21 row_newbcast:%29 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %9, %21 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %9, %21 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %10, %22 row_newbcast:%28 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %10, %22 row_newbcast:%29 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %10, %22 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %10, %22 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %11, %23 row_newbcast:%28 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %11, %23 row_newbcast:%29 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %11, %23 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %11, %23 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %12, %24 row_newbcast:%28 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %12, %24 row_newbcast:%29 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %12, %24 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %12, %24 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %13, %25 row_newbcast:%28 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %13, %25 row_newbcast:%29 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %13, %25 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %13, %25 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %14, %26 row_newbcast:%28 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %14, %26 row_newbcast:%29 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %14, %26 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %14, %26 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %0, %15, %27 row_newbcast:%28 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %1, %15, %27 row_newbcast:%29 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %2, %15, %27 row_newbcast:%30 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f32_dpp %3, %15, %27 row_newbcast:%31 row_mask:0xf bank_mask:0xf\n"
+        "s_nop 1\n"
+        : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3])
+        : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7]), "v"(x[8]), "v"(x[9]), "v"(x[10]), "v"(x[11]), "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11]), "i"(C0), "i"(C1), "i"(C2), "i"(C3));
+}
+template <int NX, int... C>
+__device__ __forceinline__ void lane_dots_seq(float* s, const float* x, const float* m, std::integer_sequence<int, C...>) {
+    constexpr int c[] = {C...};
+    constexpr int n = sizeof...(C);
+    if constexpr (n == 4) lane_dots4<NX, c[0], c[1], c[2], c[3]>(s, x, m);
+    else lane_dots2<NX, c[0], c[1]>(s, x, m);
+}
+// all NX chains: s[c] += sum_j x[j] of lane c times m[j] (c = 0 .. NX - 1)
+template <int NX>
+__device__ __forceinline__ void lane_dots_all(float* s, const float* x, const float* m) {
+    lane_dots_seq<NX>(s, x, m, std::integer_sequence<int, 0, 1, 2, 3>{});
+    lane_dots_seq<NX>(s + 4, x, m, std::integer_sequence<int, 4, 5, 6, 7>{});
+    lane_dots_seq<NX>(s + 8, x, m, std::integer_sequence<int, 8, 9, 10, 11>{});
+    if constexpr (NX == 14) lane_dots_seq<NX>(s + 12, x, m, std::integer_sequence<int, 12, 13>{});
 }
 #undef GATO_FMAC_BCAST
 
