@@ -77,6 +77,7 @@ SIGNATURES = {
     "gato_last_kernel_time": (C.c_int, [_VP, C.POINTER(C.c_double), _ip]),
     "gato_backward": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _VP, _VP, _VP, _VP, _VP, _ip]),
     "gato_set_backward_params": (C.c_int, [_VP, C.c_int32, C.c_float, C.c_int32]),
+    "gato_get_backward_params": (C.c_int, [_VP, _ip, C.POINTER(C.c_float), _ip]),
     "gato_set_backward_implicit": (C.c_int, [_VP, C.c_int32, C.c_float]),
     "gato_ee_positions": (C.c_int, [_VP, _VP, _VP, _VP, _VP]),
     "gato_select_hypothesis": (C.c_int, [_VP, _VP, _VP, _VP, C.c_float, _VP, _VP, _ip, _VP]),

@@ -309,9 +309,17 @@ class Solver:
         the one-step derivative (rounds = 0)."""
         check(self.lib.gato_set_backward_implicit(self._h, int(rounds), float(fd_step)), "gato_set_backward_implicit")
 
-    def set_backward_params(self, max_pcg_iters=-1, pcg_tol=1e-6, refine=2):
+    def set_backward_params(self, max_pcg_iters=-1, pcg_tol=1e-6, refine=-1):
+        """Backward solves: PCG iteration cap per round (-1: one, with the factorization preconditioner), relative
+        tolerance, refinement rounds (-1: the default for this horizon, 2 up to N = 64 and 4 beyond)."""
         check(self.lib.gato_set_backward_params(self._h, int(max_pcg_iters), float(pcg_tol), int(refine)),
               "gato_set_backward_params")
+
+    def backward_params(self):
+        it, tol, rounds = C.c_int32(), C.c_float(), C.c_int32()
+        check(self.lib.gato_get_backward_params(self._h, C.byref(it), C.byref(tol), C.byref(rounds)),
+              "gato_get_backward_params")
+        return {"max_pcg_iters": it.value, "pcg_tol": tol.value, "refine": rounds.value}
 
     def select_hypothesis(self, x_last, u_last, x_curr, dt, xu):
         """MPC step helper: simulate every wrench hypothesis from (x_last, u_last), pick the one closest to the

@@ -197,10 +197,13 @@ int gato_backward(gato_solver* s, const float* d_xu, const float* d_xs, const fl
                   void* stream, int32_t* h_adj_pcg_iters);
 /* Backward solves: PCG iteration cap per round (-1, the default: 1), relative tolerance on the preconditioned
  * residual r^T M^-1 r (default 1e-6) and rounds of iterative refinement of the step and adjoint solutions
- * (default 2, at most 8). M is the twisted block-tridiagonal factorization of the Schur complement, equal to it up
+ * (at most 8; -1, the default: 2 for N <= 64 and 4 beyond, where two rounds leave about 1e-2 of relative gradient
+ * error against an FP64 solve and four about 2e-4). M is the twisted block-tridiagonal factorization of the Schur complement, equal to it up
  * to fp32 rounding, so one PCG iteration per round is a solve to the fp32 floor and the refinement rounds, whose
  * residuals come from the KKT blocks, correct it. */
 int gato_set_backward_params(gato_solver* s, int32_t max_pcg_iters, float pcg_tol, int32_t refine);
+/* The backward solves' settings in force (refine resolved to its count). */
+int gato_get_backward_params(gato_solver* s, int32_t* max_pcg_iters, float* pcg_tol, int32_t* refine);
 
 /* Backward, derivative of the solution (no reference counterpart). With rounds > 0 the backward returns the
  * implicit derivative of the solution z*(theta), defined by F(z, lambda; theta) = [g + C^T lambda; c] = 0, instead of

@@ -70,6 +70,20 @@ def test_backward_matches_oracle_step_gradient(gpu, O, plant, hess, term, N, gro
         assert _rel(g["Rd"][b].cpu().numpy(), o["Rd"]) < 1e-3, b
 
 
+def test_backward_refinement_rounds_default_by_horizon(gpu):
+    # two rounds reach the fp32 floor up to N = 64; at N = 128 the default is four (each round cuts the error by
+    # about 8x there, and two leave about 1e-2); -1 restores the horizon's default after an explicit setting
+    from gato_amd.solver import Solver
+
+    for N, rounds in ((32, 2), (64, 2), (128, 4)):
+        s = Solver("iiwa14", N, 2)
+        assert s.backward_params()["refine"] == rounds
+        s.set_backward_params(refine=6)
+        assert s.backward_params()["refine"] == 6
+        s.set_backward_params()
+        assert s.backward_params() == {"max_pcg_iters": -1, "pcg_tol": pytest.approx(1e-6), "refine": rounds}
+
+
 def test_backward_leaves_the_solver_state_alone(gpu):
     s, wl, Qd, Rd, _ = _problem("iiwa14", 8, 2, seed=23)
     dev = s.device
