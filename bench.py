@@ -33,18 +33,18 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=3,
                     help="steps of the plain path timed for the speed-up and the k_pcg roofline (0: skip)")
     ap.add_argument("--cpu-runs", type=int, default=5, help="timed runs of the CPU baseline (median; 0: skip)")
-    ap.add_argument("--fwd-bwd-steps", type=int, default=3, help="steps of forward solve + backward pass timed (0: skip)")
+    ap.add_argument("--fwd-bwd-steps", type=int, default=10, help="steps of forward solve + backward pass timed (0: skip)")
     ap.add_argument("--scale-steps", type=int, default=200,
                     help="Adam steps of the scale workload (iiwa14 N=128 B=4096, shared diag Q/R learned through "
                          "closed-loop rollouts of 4 MPC steps and their backward pass) timed (0: skip)")
-    ap.add_argument("--fp64-steps", type=int, default=3,
+    ap.add_argument("--fp64-steps", type=int, default=5,
                     help="steps of FP64 mode on the drop-in workload at B=128 timed (0: skip)")
-    ap.add_argument("--diff-steps", type=int, default=3,
+    ap.add_argument("--diff-steps", type=int, default=20,
                     help="steps of the differentiation workload (iiwa14 N=32 B=256, forward + backward) timed (0: skip)")
     ap.add_argument("--ref-mpc-steps", type=int, default=40,
                     help="MPC steps of the reference's published configuration (indy7 N=32 B=512, 1 SQP "
                          "iteration per solve) timed (0: skip)")
-    ap.add_argument("--mixed-steps", type=int, default=3,
+    ap.add_argument("--mixed-steps", type=int, default=10,
                     help="steps of the uneven batch (1024 problems, both plants, N in {8, 32, 128}) timed (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run three device copies of MIB MiB (a known-size reference for HBM counters)")
