@@ -694,8 +694,10 @@ def main():
                            "avg_launch_us": us, "flops_per_launch": fl, "algorithmic_bytes_per_launch": by,
                            "hbm_achieved_GBs": by / (us * 1e-6) / 1e9,
                            "hbm_frac": by / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                           "frac_of_unpacked_fp32_peak": ach / (FP32_PEAK_TFLOPS / 2),
                            "note": "fp32 vector-ALU work (no MFMA): bound by the vector ALU and barriers; "
-                                   "peak = MI355X fp32 157.3 TFLOP/s"}
+                                   "peak = MI355X fp32 157.3 TFLOP/s, which counts packed two-wide FMAs; the "
+                                   "build issues one fp32 FMA per lane and instruction (78.65 TFLOP/s)"}
     if args.mode in ("fused", "graph"):
         by = algorithmic_bytes(wl)
         fl = algorithmic_flops(st, wl.N)
