@@ -103,22 +103,33 @@ template <typename S>
 __device__ __forceinline__ bool stopped(const ArgsT<S>& a) { return a.stop && *a.stop; }
 
 // ------------------------------------------------------------------ phases on one problem
+// The KKT set-up of every knot (setup_kkt.cuh:14-108) in chunks of ts knots (ts <= 64, ts LDS tables of
+// lin_lds_floats): the dynamics that all columns of a knot share (forward dynamics, RNEA table, transported axes) once
+// per knot by one lane of the first wave (lin_dyn), then the knot's derivative columns and cost model one lane per
+// column in its knot group (lin_knot). Computing the shared part per knot group instead would repeat it in all 16
+// lanes of the group, and it is about 60% of a lane's instructions.
 template <int PL, typename S = float>
-__device__ __forceinline__ void phase_lin(const ArgsT<S>& a, int b, S* sm) {
+__device__ __forceinline__ void phase_lin(const ArgsT<S>& a, int b, S* sm, int ts) {
     using P = Plant<PL>;
-    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
+    constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU, TL = lin_lds_floats<PL>();
     View<PL, S> v(a, b);
-    const int N = a.N, g = threadIdx.x / KG, lane = threadIdx.x % KG, G = blockDim.x / KG;
-    S* shg = sm + g * lin_lds_floats<PL>();
+    const int N = a.N, tid = threadIdx.x, g = tid / KG, lane = tid % KG, G = blockDim.x / KG;
     const S* xt = v.xu + (size_t)(a.cw.terminal_mode ? N - 1 : N - 2) * (NX + NU);
-    for (int k = g; k < N; k += G) {
-        // the lane index is re-laundered per knot so that lane-only constants of the knot routine are
-        // recomputed inside the loop rather than hoisted out of it and held in registers throughout
-        const int ln = opaque(lane);
-        const S* xk = v.xu + (size_t)(k < N - 1 ? k : 0) * (NX + NU);
-        lin_knot<PL, S>(k, N, ln, a.dt, a.cw, xk, xt, v.ref + 6 * k, v.xref ? v.xref + k * NX : nullptr, v.Qd, v.Rd,
-                     v.fext, v.xu, v.xs, v.A + (size_t)k * NX * NX, v.Bm + (size_t)k * NX * NU, v.c + (k + 1) * NX,
-                     v.c, v.Qp + k * NQ * NQ, v.Qv + k * NQ, v.Rg + k * NU, v.q + k * NX, v.r + k * NU, shg);
+    for (int k0 = 0; k0 < N; k0 += ts) {
+        const int kn = min(N, k0 + ts);
+        if (k0 > 0) __syncthreads();  // the previous chunk's tables have been read
+        if (tid < kn - k0 && k0 + tid < N - 1) lin_dyn<PL, S>(v.xu + (size_t)(k0 + tid) * (NX + NU), v.fext, sm + tid * TL);
+        __syncthreads();
+        for (int k = k0 + g; k < kn; k += G) {
+            // the lane index is re-laundered per knot so that lane-only constants of the knot routine are
+            // recomputed inside the loop rather than hoisted out of it and held in registers throughout
+            const int ln = opaque(lane);
+            const S* xk = v.xu + (size_t)(k < N - 1 ? k : 0) * (NX + NU);
+            lin_knot<PL, S, false>(k, N, ln, a.dt, a.cw, xk, xt, v.ref + 6 * k, v.xref ? v.xref + k * NX : nullptr,
+                                   v.Qd, v.Rd, v.fext, v.xu, v.xs, v.A + (size_t)k * NX * NX,
+                                   v.Bm + (size_t)k * NX * NU, v.c + (k + 1) * NX, v.c, v.Qp + k * NQ * NQ,
+                                   v.Qv + k * NQ, v.Rg + k * NU, v.q + k * NX, v.r + k * NU, sm + (k - k0) * TL);
+        }
     }
 }
 
@@ -1147,7 +1158,7 @@ __device__ __forceinline__ void begin_problem(const ArgsT<S>& a, int b) {
 // Solve start (k_begin, the clear of dz and the initial merit of bsqp.cuh:111-118 for problem b), then the
 // linearization and Schur phase of SQP iteration 0.
 template <int PL>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_first_n(Args a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_first_n(Args a, int ts) {
     extern __shared__ __align__(16) float sm[];
     constexpr int NU = Plant<PL>::NU;
     const int b = blockIdx.x;
@@ -1164,7 +1175,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         a.st_init[b] = sm[0];
     }
     __syncthreads();
-    phase_lin<PL>(a, b, sm);
+    phase_lin<PL>(a, b, sm, ts);
     __syncthreads();
     phase_schur<PL>(a, b, a.rho[b], sm);
 }
@@ -1172,7 +1183,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 // The batch-level exit test and line search of SQP iteration it - 1 (merit_body, which == 3), then the
 // linearization and Schur phase of iteration it on the trajectory the line search has just updated.
 template <int PL>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_ls_lin_schur_n(Args a, int it) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_ls_lin_schur_n(Args a, int it, int ts) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
     if (stopped(a)) return;
@@ -1184,7 +1195,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     phase_merit<PL>(a, b, a.mu[b], NUM_ALPHAS, sm);
     phase_ls<PL>(a, b, it - 1, sm, sm + NUM_ALPHAS);
     __syncthreads();  // the updated trajectory and rho are settled, and the LDS is free for the linearization
-    phase_lin<PL>(a, b, sm);
+    phase_lin<PL>(a, b, sm, ts);
     __syncthreads();
     phase_schur<PL>(a, b, a.rho[b], sm);
 }
@@ -1219,7 +1230,7 @@ __global__ void GATO_PHASE_BOUNDS(S) k_lin(ArgsT<S> a) {
     S* sm = dyn_lds<S>();
     const int b = blockIdx.x;
     if (stopped(a) || (a.exit_rule == GATO_EXIT_KKT && !a.active[b])) return;
-    phase_lin<PL, S>(a, b, sm);
+    phase_lin<PL, S>(a, b, sm, blockDim.x / KG);
 }
 
 template <int PL, typename S = float>
@@ -1516,7 +1527,7 @@ __device__ __forceinline__ void fused_problem(const Args& a, int b, float* sm, f
         __syncthreads();
         int it = 0;
         for (; it < a.max_sqp; it++) {
-            phase_lin<PL>(a, b, sm);
+            phase_lin<PL>(a, b, sm, blockDim.x / KG);
             __syncthreads();
             phase_schur<PL>(a, b, a.rho[b], sm);
             __syncthreads();
@@ -2638,7 +2649,7 @@ void launch_fused(int np, int B, dim3 blk, size_t lds, hipStream_t st, const gat
 size_t lds_bytes_g(const gato_solver* s, int G) {
     const int nx = s->nx, nq = s->nq;
     size_t f = 0;
-    f = std::max<size_t>(f, (size_t)G * (24 * nq + 9 * (nq * (nq + 1) / 2)));  // lin
+    f = std::max<size_t>(f, (size_t)G * (24 * nq + 9 * (nq * (nq + 1) / 2) + 8));  // lin (lin_lds_floats)
     const size_t tile = s->plant == GATO_PLANT_IIWA14 ? gato::SchurTile<0>::SIZE : gato::SchurTile<1>::SIZE;
     f = std::max<size_t>(f, (size_t)G * tile);                                  // schur (SchurTile)
     f = std::max<size_t>(f, (size_t)2 * (s->N + 2) * 16 + 64);      // pcg
@@ -2719,11 +2730,12 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
     const size_t nz = (size_t)B * (s->traj + s->nu);
     const int iters = (int)s->prm.max_sqp_iters;
     if (iters >= 1 && N <= 32 && N * gato::NUM_ALPHAS <= 256 && blk.x >= 256) {
-        hipLaunchKernelGGL((gato::k_first_n<PL>), dim3(B), dim3(256), lds, st, a);
+        const int ts = std::min(N, s->G);  // LDS tables of the linearization: lds_bytes holds s->G of them
+        hipLaunchKernelGGL((gato::k_first_n<PL>), dim3(B), dim3(256), lds, st, a, ts);
         for (int it = 0; it < iters; it++) {
             launch_pcg_dz<PL>(np, B, blk, pcg_lds_bytes(s), st, a, it);
             if (it + 1 < iters)
-                hipLaunchKernelGGL((gato::k_ls_lin_schur_n<PL>), dim3(B), dim3(256), lds, st, a, it + 1);
+                hipLaunchKernelGGL((gato::k_ls_lin_schur_n<PL>), dim3(B), dim3(256), lds, st, a, it + 1, ts);
             else
                 hipLaunchKernelGGL((gato::k_last_n<PL>), dim3(B), dim3(256), lds, st, a, it);
         }

@@ -615,20 +615,50 @@ __device__ __forceinline__ S plim(int which, int i) {
     return r;
 }
 
-// LDS floats per knot group used by lin_knot: RNEA table (24 nq), Cholesky factor (nq(nq+1)/2) and the
-// transported-axes table (8 nq(nq+1)/2), which also hosts the RNEA force buffer before it is built and
-// the gradient / Jacobian scratch after the tangents are done.
+// LDS floats per knot used by lin_knot: RNEA table (24 nq), Cholesky factor (nq(nq+1)/2), the transported-axes
+// table (8 nq(nq+1)/2), which also hosts the RNEA force buffer before it is built and the gradient / Jacobian
+// scratch after the tangents are done, and the knot's joint accelerations (8).
 template <int PL>
 __host__ __device__ constexpr int lin_lds_floats() {
     constexpr int NQ = Plant<PL>::NQ;
-    return 24 * NQ + 9 * (NQ * (NQ + 1) / 2);
+    return 24 * NQ + 9 * (NQ * (NQ + 1) / 2) + 8;
+}
+
+// The dynamics of knot k <= N-2 that every column of its linearization shares, by ONE lane: qdd = M^-1 (u - c) with
+// the Cholesky factor of M, the RNEA table at (q, qd, qdd) and the transported axes, written to the knot's LDS table
+// shg (lin_lds_floats). xk points at [x_k u_k].
+template <int PL, typename S = float>
+__device__ __forceinline__ void lin_dyn(const S* xk, const S* fext, S* shg) {
+    constexpr int NQ = Plant<PL>::NQ, NX = Plant<PL>::NX;
+    S q[NQ], qd[NQ], u[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; i++) {
+        q[i] = xk[i];
+        qd[i] = xk[NQ + i];
+        u[i] = xk[NX + i];
+    }
+    Kin<PL, S> K;
+    K.set(q);
+    S* tab = shg;
+    S* shL = tab + RneaTab<PL>::SIZE;
+    S* ax = shL + NQ * (NQ + 1) / 2;
+    S* sqdd = ax + 8 * (NQ * (NQ + 1) / 2);
+    S qdd[NQ];
+    FLdsT<S> fs{ax, 1};
+    forward_dynamics_fs<PL>(K, qd, u, fext, qdd, shL, fs);
+    __builtin_amdgcn_sched_barrier(0);
+    rnea_publish<PL>(K, qd, qdd, fext, tab);
+    publish_axes<PL>(K, ax);
+#pragma unroll
+    for (int i = 0; i < NQ; i++) sqdd[i] = qdd[i];
 }
 
 // ---------------------------------------------------------------- KKT setup, one knot
 // k <= N-2: A_k, B_k, c_{k+1} and the cost model at (x_k, u_k, ref_k);
 // k == N-1: terminal cost model (quirk Q4: at x_{N-2} with ref_{N-1} and q_cost) and c_0 = x_0 - x_s.
-// xk points at [x_k u_k x_{k+1}] (k <= N-2) or is unused; xt is the terminal-block state.
-template <int PL, typename S = float>
+// xk points at [x_k u_k x_{k+1}] (k <= N-2) or is unused; xt is the terminal-block state. DYN: the knot group computes
+// the shared dynamics itself (every lane, identical values); otherwise lin_dyn has written them to shg beforehand.
+template <int PL, typename S = float, bool DYN = true>
 __device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const CostW& cw, const S* xk,
                                          const S* xt, const S* refk, const S* xrefk, const S* Qd,
                                          const S* Rd, const S* fext, const S* x0, const S* xs,
@@ -654,12 +684,18 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const Cos
     S* ax = shL + NQ * (NQ + 1) / 2;
     if (!term) {
         S qdd[NQ];
-        FLdsT<S> fs{ax, 1};
-        forward_dynamics_fs<PL>(K, qd, u, fext, qdd, shL, fs);
-        __builtin_amdgcn_sched_barrier(0);
-        rnea_publish<PL>(K, qd, qdd, fext, tab);
-        publish_axes<PL>(K, ax);
-        wave_sync();
+        if constexpr (DYN) {
+            FLdsT<S> fs{ax, 1};
+            forward_dynamics_fs<PL>(K, qd, u, fext, qdd, shL, fs);
+            __builtin_amdgcn_sched_barrier(0);
+            rnea_publish<PL>(K, qd, qdd, fext, tab);
+            publish_axes<PL>(K, ax);
+            wave_sync();
+        } else {
+            const S* sqdd = ax + 8 * (NQ * (NQ + 1) / 2);
+#pragma unroll
+            for (int i = 0; i < NQ; i++) qdd[i] = sqdd[i];
+        }
         const int col = lane < NX ? lane : 0;
         const int j = col < NQ ? col : col - NQ;
         const int dir = col < NQ ? 0 : 1;
