@@ -1035,9 +1035,10 @@ __device__ __forceinline__ S phase_dz(const ArgsT<S>& a, int b, S* sm) {
 
 // merit at nal step lengths alpha_j = 2^-j (merit.cuh:16-92): per-(alpha, knot) partial terms,
 // then fixed-order sums over knots. out[j] (LDS) receives phi(alpha_j). The reference's
-// initial / final merit (dz = 0, one alpha) is nal = 1.
+// initial / final merit (dz = 0, one alpha) is nal = 1. The partial terms go to part (LDS, nal * N values), from
+// which the sums read them, and to mpart (global memory, kept for inspection).
 template <int PL, typename S = float>
-__device__ __forceinline__ void phase_merit(const ArgsT<S>& a, int b, S mu, int nal, S* out) {
+__device__ __forceinline__ void phase_merit(const ArgsT<S>& a, int b, S mu, int nal, S* out, S* part) {
     using P = Plant<PL>;
     constexpr int NX = P::NX, NU = P::NU;
     View<PL, S> v(a, b);
@@ -1046,14 +1047,15 @@ __device__ __forceinline__ void phase_merit(const ArgsT<S>& a, int b, S mu, int 
         const int k = it % N, j = it / N;
         const S alpha = S(1) / (S)(1 << j);
         const size_t off = (size_t)k * (NX + NU);
-        v.mpart[j * N + k] = merit_item<PL, S>(k, N, alpha, a.dt, a.cw, mu, v.xu + off, v.dz + off, v.ref + 6 * k,
-                                            v.xref ? v.xref + k * NX : nullptr, v.Qd, v.Rd, v.fext, v.xu, v.dz,
-                                            v.xs);
+        const S m = merit_item<PL, S>(k, N, alpha, a.dt, a.cw, mu, v.xu + off, v.dz + off, v.ref + 6 * k,
+                                      v.xref ? v.xref + k * NX : nullptr, v.Qd, v.Rd, v.fext, v.xu, v.dz, v.xs);
+        v.mpart[j * N + k] = m;
+        part[j * N + k] = m;
     }
     __syncthreads();
     if (threadIdx.x < (unsigned)nal) {
         S s = S(0);
-        for (int k = 0; k < N; k++) s += v.mpart[threadIdx.x * N + k];
+        for (int k = 0; k < N; k++) s += part[threadIdx.x * N + k];
         out[threadIdx.x] = s;
     }
     __syncthreads();
@@ -1107,7 +1109,7 @@ __device__ __forceinline__ void merit_body(const ArgsT<S>& a, int nal, int which
         return;
     }
     if (which >= 2 && a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
-    phase_merit<PL, S>(a, b, a.mu[b], nal, sm);
+    phase_merit<PL, S>(a, b, a.mu[b], nal, sm, sm + 2 * NUM_ALPHAS);
     if (threadIdx.x == 0 && which < 2) {
         a.phi[b] = sm[0];
         (which == 0 ? a.st_init : a.st_final)[b] = sm[0];
@@ -1169,7 +1171,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     }
     for (int i = threadIdx.x; i < a.traj + NU; i += blockDim.x) v.dz[i] = 0.f;
     __syncthreads();
-    phase_merit<PL>(a, b, a.mu[b], 1, sm);
+    phase_merit<PL>(a, b, a.mu[b], 1, sm, sm + 2 * NUM_ALPHAS);
     if (threadIdx.x == 0) {
         a.phi[b] = sm[0];
         a.st_init[b] = sm[0];
@@ -1192,7 +1194,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         return;
     }
     if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
-    phase_merit<PL>(a, b, a.mu[b], NUM_ALPHAS, sm);
+    phase_merit<PL>(a, b, a.mu[b], NUM_ALPHAS, sm, sm + 2 * NUM_ALPHAS);
     phase_ls<PL>(a, b, it - 1, sm, sm + NUM_ALPHAS);
     __syncthreads();  // the updated trajectory and rho are settled, and the LDS is free for the linearization
     phase_lin<PL>(a, b, sm, ts);
@@ -1211,13 +1213,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     View<PL> v(a, b);
     if (!stopped(a) && (double)a.solved[it] < (double)a.B * (double)a.solve_ratio &&
         !(a.exit_rule == GATO_EXIT_KKT && !a.active[b])) {
-        phase_merit<PL>(a, b, a.mu[b], NUM_ALPHAS, sm);
+        phase_merit<PL>(a, b, a.mu[b], NUM_ALPHAS, sm, sm + 2 * NUM_ALPHAS);
         phase_ls<PL>(a, b, it, sm, sm + NUM_ALPHAS);
         __syncthreads();  // the line search has read dz
     }
     for (int i = threadIdx.x; i < a.traj + NU; i += blockDim.x) v.dz[i] = 0.f;
     __syncthreads();
-    phase_merit<PL>(a, b, a.mu[b], 1, sm);
+    phase_merit<PL>(a, b, a.mu[b], 1, sm, sm + 2 * NUM_ALPHAS);
     if (threadIdx.x == 0) {
         a.phi[b] = sm[0];
         a.st_final[b] = sm[0];
@@ -1518,7 +1520,7 @@ __device__ __forceinline__ void fused_problem(const Args& a, int b, float* sm, f
         const float mu = a.mu[b], eps = a.tol[b];
         for (int i = threadIdx.x; i < a.traj + NU; i += blockDim.x) v.dz[i] = 0.f;
         __syncthreads();
-        phase_merit<PL>(a, b, mu, 1, shs);
+        phase_merit<PL>(a, b, mu, 1, shs, sm);
         if (threadIdx.x == 0) {
             a.phi[b] = shs[0];
             a.st_init[b] = shs[0];
@@ -1544,7 +1546,7 @@ __device__ __forceinline__ void fused_problem(const Args& a, int b, float* sm, f
             }
             __syncthreads();
             if (shi[0]) break;
-            phase_merit<PL>(a, b, mu, NUM_ALPHAS, shs);
+            phase_merit<PL>(a, b, mu, NUM_ALPHAS, shs, sm);
             phase_ls<PL>(a, b, it, shs, shs + NUM_ALPHAS);
             __syncthreads();
         }
@@ -1560,7 +1562,7 @@ __device__ __forceinline__ void fused_problem(const Args& a, int b, float* sm, f
         }
         for (int i = threadIdx.x; i < a.traj + NU; i += blockDim.x) v.dz[i] = 0.f;
         __syncthreads();
-        phase_merit<PL>(a, b, mu, 1, shs);
+        phase_merit<PL>(a, b, mu, 1, shs, sm);
         if (threadIdx.x == 0) {
             a.st_final[b] = shs[0];
             a.drho[b] = a.drho_init[b];
@@ -2654,7 +2656,7 @@ size_t lds_bytes_g(const gato_solver* s, int G) {
     f = std::max<size_t>(f, (size_t)G * tile);                                  // schur (SchurTile)
     f = std::max<size_t>(f, (size_t)2 * (s->N + 2) * 16 + 64);      // pcg
     f = std::max<size_t>(f, (size_t)G * gato::KG + 16);             // dz
-    f = std::max<size_t>(f, (size_t)2 * gato::NUM_ALPHAS + 16);     // merit + ls
+    f = std::max<size_t>(f, (size_t)(2 + s->N) * gato::NUM_ALPHAS);  // merit + ls (out, shs, partial terms)
     return f * sizeof(float);
 }
 
@@ -3194,7 +3196,7 @@ int launch_solve64(gato_solver64* s, hipStream_t st) {
     f = std::max<size_t>(f, (size_t)s->G * tile);
     f = std::max<size_t>(f, (size_t)2 * (N + 2) * 16 + 64);
     f = std::max<size_t>(f, (size_t)s->G * gato::KG + 16);
-    f = std::max<size_t>(f, (size_t)2 * gato::NUM_ALPHAS + 16);
+    f = std::max<size_t>(f, (size_t)(2 + N) * gato::NUM_ALPHAS);
     const size_t lds = f * sizeof(double);
     gato::ArgsT<double>& a = s->a;
     const int sb = (B + 255) / 256;
