@@ -328,14 +328,18 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
         const float* PL_ = v.Po + (size_t)kk * NX * NX + l * NX;
         const float* PU = v.Po + (size_t)(kk + 1 < N ? kk + 1 : kk) * NX * NX + l;
         const bool hasl = kk > 0, hasu = kk < N - 1;
+        // every address is inside the batch's blocks (kk, l clamped), so all 84 loads are issued unconditionally and
+        // the missing neighbours zeroed by selects: a load under the condition becomes a branch per column, each
+        // waiting for its loads
 #pragma unroll
         for (int c = 0; c < NX; c++) {
+            const float l0 = L[c], l1 = PL_[c], u0 = U[c * NX], u1 = PU[c * NX];
             sd[c] = D[c];
             pd[c] = PD[c];
-            sl[c] = hasl ? L[c] : 0.f;
-            pl[c] = hasl ? PL_[c] : 0.f;
-            su[c] = hasu ? U[c * NX] : 0.f;
-            pu[c] = hasu ? PU[c * NX] : 0.f;
+            sl[c] = hasl ? l0 : 0.f;
+            pl[c] = hasl ? l1 : 0.f;
+            su[c] = hasu ? u0 : 0.f;
+            pu[c] = hasu ? u1 : 0.f;
         }
     }
     // block-tridiagonal row times a vector held one entry per lane (own: this knot's entry, 0 on padding lanes):
@@ -351,12 +355,13 @@ __device__ __forceinline__ int phase_pcg_reg(const Args& a, int b, float eps, fl
         s = row_bcast_dot<NX>(s, xm, M1);
         return row_bcast_dot<NX>(s, xp, M2);
     };
-    float x = val ? v.lam[kk * NX + l] : 0.f;
+    const float lam0 = v.lam[kk * NX + l], gam0 = v.gam[kk * NX + l];  // issued with the rows, before the barriers
+    float x = val ? lam0 : 0.f;
     __syncthreads();
     if (val) sp[(kk + 1) * XP + l] = x;
     __syncthreads();
     const float sx = mv(sd, sl, su, x, sp[kk * XP + ln], sp[(kk + 2) * XP + ln]);
-    float r = val ? v.gam[kk * NX + l] - sx : 0.f;
+    float r = val ? gam0 - sx : 0.f;
     if (val) sr[(kk + 1) * XP + l] = r;
     __syncthreads();
     // Each lane keeps its own copies of r and p at its index in the neighbour knots (rm, rp, pm, pp) and updates
@@ -474,7 +479,10 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
         const int k = kk[s];
         const float* Uc = v.So + (size_t)(k + 1 < N ? k + 1 : k) * NB + l;
 #pragma unroll
-        for (int c = 0; c < NX; c++) uc[s][c] = k < N - 1 ? Uc[c * NX] : 0.f;
+        for (int c = 0; c < NX; c++) {
+            const float t = Uc[c * NX];  // in bounds for every k < N: loaded unconditionally, selected below
+            uc[s][c] = k < N - 1 ? t : 0.f;
+        }
     }
     // the off-diagonal blocks of block row kk[s] times a vector whose entries at this lane's index in knots k-1 and k+1
     // are ym and yp: row l of So_k (from LDS; zeros for k = 0), then column l of So_{k+1} (registers), from acc
@@ -869,9 +877,10 @@ __device__ __forceinline__ void phase_pcg_reg2(const Args& a1, const Args& a2, i
         const bool hasl = kk > 0, hasu = kk < N - 1;
 #pragma unroll
         for (int c = 0; c < NX; c++) {
+            const float l0 = L[c], u0 = U[c * NX];  // in bounds: loaded unconditionally, selected below
             sd[c] = D[c];
-            sl[c] = hasl ? L[c] : 0.f;
-            su[c] = hasu ? U[c * NX] : 0.f;
+            sl[c] = hasl ? l0 : 0.f;
+            su[c] = hasu ? u0 : 0.f;
         }
     }
     auto blk = [&](const float* M, const float4* xb, float s) {

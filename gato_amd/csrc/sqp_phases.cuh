@@ -1139,15 +1139,30 @@ __device__ __forceinline__ S dz_knot(int k, int N, int lane, const S* A, const S
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
     const bool last = (k == N - 1);
+    // every operand is loaded up front from clamped addresses inside knot k's blocks (which exist for every k < N;
+    // lam1 points at a valid knot) and used under the same conditions as before: loads placed under those conditions
+    // become branches that each wait for their own loads
+    const int lx = lane < NX ? lane : 0, lq = lane < NQ ? lane : 0, lu = lane < NU ? lane : 0;
+    const int lv = (lane >= NQ && lane < NX) ? lane - NQ : 0;
+    S acol[NX], bcol[NX], l1[NX], qrow[NQ];
+#pragma unroll
+    for (int i = 0; i < NX; i++) {
+        acol[i] = A[i * NX + lx];
+        bcol[i] = Bm[i * NU + lu];
+        l1[i] = lam1[i];
+    }
+#pragma unroll
+    for (int l = 0; l < NQ; l++) qrow[l] = Qip[lq * NQ + l];
+    const S qv = q[lx], l0 = lam0[lx], cv = c[lx], qiv = Qiv[lv], rrv = rr[lu], ri = Ri[lu];
     S res = S(0);
     if (lane < NX) {
-        S v = q[lane] - lam0[lane];
+        S v = qv - l0;
         if (!last) {
 #pragma unroll
-            for (int i = 0; i < NX; i++) v += A[i * NX + lane] * lam1[i];
+            for (int i = 0; i < NX; i++) v += acol[i] * l1[i];
         }
         shv[lane] = v;
-        res = smax(sabs(v), sabs(c[lane]));
+        res = smax(sabs(v), sabs(cv));
     }
     wave_sync();
     if (lane < NX) {
@@ -1155,17 +1170,17 @@ __device__ __forceinline__ S dz_knot(int k, int N, int lane, const S* A, const S
         if (lane < NQ) {
             s = S(0);
 #pragma unroll
-            for (int l = 0; l < NQ; l++) s += Qip[lane * NQ + l] * shv[l];
+            for (int l = 0; l < NQ; l++) s += qrow[l] * shv[l];
         } else {
-            s = Qiv[lane - NQ] * shv[lane];
+            s = qiv * shv[lane];
         }
         dzk[lane] = -s;
     }
     if (!last && lane < NU) {
-        S w = rr[lane];
+        S w = rrv;
 #pragma unroll
-        for (int i = 0; i < NX; i++) w += Bm[i * NU + lane] * lam1[i];
-        dzk[NX + lane] = -Ri[lane] * w;
+        for (int i = 0; i < NX; i++) w += bcol[i] * l1[i];
+        dzk[NX + lane] = -ri * w;
         res = smax(res, sabs(w));
     }
     wave_sync();
