@@ -673,8 +673,20 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const Cos
     for (int i = 0; i < NQ; i++) {
         q[i] = xsrc[i];
         qd[i] = xsrc[NQ + i];
-        u[i] = term ? S(0) : xk[NX + i];
+        const S ui = xk[NX + i];  // xk is a valid knot for the terminal block too: loaded, then discarded there
+        u[i] = term ? S(0) : ui;
     }
+    // the lane's operands of the integrator error and the cost model, loaded up front from clamped lane indices (a
+    // load under a lane or knot condition compiles to a branch that waits for its own load)
+    const int lq = lane < NQ ? lane : 0, lx = lane < NX ? lane : 0;
+    const S* xr = xrefk ? xrefk : Qd;
+    const S ql_ = xsrc[lq], qdl_ = xsrc[NQ + lq], xnq = xk[NX + NU + lq], xnv = xk[NX + NU + NQ + lq];
+    const S rf0 = refk[0], rf1 = refk[1], rf2 = refk[2];
+    const S qdw_q = Qd[lq], qdw_v = Qd[NQ + lq], rdw = Rd[lq], xrq = xr[lq], xrv = xr[NQ + lq], ul_ = xk[NX + lq];
+    const S x0l = x0[lx], xsl = xs[lx];
+    S xsm[NQ];
+#pragma unroll
+    for (int m = 0; m < NQ; m++) xsm[m] = xsrc[m];
     Kin<PL, S> K;
     K.set(q);
     // knot-group LDS: RNEA table, Cholesky factor of M (identical in every lane of the group,
@@ -733,10 +745,9 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const Cos
             }
         }
         if (lane < NQ) {  // c_{k+1} = x_{k+1} - x+ (integrator.cuh:49-62)
-            const S ql = xsrc[lane], qdl = xsrc[NQ + lane], al = pick<S[NQ], S>(qdd, NQ, lane);
-            const S* xn = xk + NX + NU;
-            cn[lane] = xn[lane] - (ql + dt * qdl + S(0.5) * al * dt * dt);
-            cn[NQ + lane] = xn[NQ + lane] - (qdl + dt * al);
+            const S ql = ql_, qdl = qdl_, al = pick<S[NQ], S>(qdd, NQ, lane);
+            cn[lane] = xnq - (ql + dt * qdl + S(0.5) * al * dt * dt);
+            cn[NQ + lane] = xnv - (qdl + dt * al);
         }
     }
     // cost model (iiwa14_plant.cuh:339-424); joint-space diagonal weights generalise qd_cost/u_cost.
@@ -753,7 +764,7 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const Cos
         }
         ee_position<PL>(K2, lane < NQ ? lane : -1, p, Jc);
     }
-    const S e0 = p[0] - refk[0], e1 = p[1] - refk[1], e2 = p[2] - refk[2];
+    const S e0 = p[0] - rf0, e1 = p[1] - rf1, e2 = p[2] - rf2;
     const S gl = Jc[0] * e0 + Jc[1] * e1 + Jc[2] * e2;
     if (lane < NQ) {
         shg[lane] = gl;
@@ -761,7 +772,7 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const Cos
     }
     wave_sync();
     if (lane < NQ) {
-        const S ql = xsrc[lane], qdl = xsrc[NQ + lane];
+        const S ql = ql_, qdl = qdl_;
         const S bgl = barrier_g<PL, S>(ql, plim<PL, S>(0, lane), plim<PL, S>(1, lane));
 #pragma unroll
         for (int m = 0; m < NQ; m++) {
@@ -770,29 +781,29 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const Cos
                 v = w * (Jc[0] * shg[NQ + 3 * m] + Jc[1] * shg[NQ + 3 * m + 1] + Jc[2] * shg[NQ + 3 * m + 2]);
             else
                 v = w * gl * shg[m];  // rank-1 EE term (quirk Q5)
-            if (PL == 1) v += cw.q_lim * bgl * barrier_g<PL, S>(xsrc[m], P::template qlo<S>(m), P::template qhi<S>(m));
+            if (PL == 1) v += cw.q_lim * bgl * barrier_g<PL, S>(xsm[m], P::template qlo<S>(m), P::template qhi<S>(m));
             if (m == lane) {
-                v += Qd[lane];
+                v += qdw_q;
                 if (PL != 1) v += cw.q_lim * barrier_h<S>(ql, plim<PL, S>(0, lane), plim<PL, S>(1, lane));
             }
             Qp[lane * NQ + m] = v;
         }
-        const S xr_q = xrefk ? xrefk[lane] : S(0), xr_v = xrefk ? xrefk[NQ + lane] : S(0);
-        qv[lane] = w * gl + cw.q_lim * bgl + Qd[lane] * (ql - xr_q);
+        const S xr_q = xrefk ? xrq : S(0), xr_v = xrefk ? xrv : S(0);
+        qv[lane] = w * gl + cw.q_lim * bgl + qdw_q * (ql - xr_q);
         const S bv = barrier_g<PL, S>(qdl, plim<PL, S>(2, lane), plim<PL, S>(3, lane));
-        qv[NQ + lane] = Qd[NQ + lane] * (qdl - xr_v) + cw.vel_lim * bv;
-        Qv[lane] = Qd[NQ + lane] + cw.vel_lim * (PL == 1 ? bv * bv : barrier_h<S>(qdl, plim<PL, S>(2, lane), plim<PL, S>(3, lane)));
+        qv[NQ + lane] = qdw_v * (qdl - xr_v) + cw.vel_lim * bv;
+        Qv[lane] = qdw_v + cw.vel_lim * (PL == 1 ? bv * bv : barrier_h<S>(qdl, plim<PL, S>(2, lane), plim<PL, S>(3, lane)));
         if (!term) {
-            const S ul = xk[NX + lane];
+            const S ul = ul_;
             const S bu = barrier_g<PL, S>(ul, plim<PL, S>(4, lane), plim<PL, S>(5, lane));
-            rv[lane] = Rd[lane] * ul + cw.ctrl_lim * bu;
-            Rg[lane] = Rd[lane] + cw.ctrl_lim * (PL == 1 ? bu * bu : barrier_h<S>(ul, plim<PL, S>(4, lane), plim<PL, S>(5, lane)));
+            rv[lane] = rdw * ul + cw.ctrl_lim * bu;
+            Rg[lane] = rdw + cw.ctrl_lim * (PL == 1 ? bu * bu : barrier_h<S>(ul, plim<PL, S>(4, lane), plim<PL, S>(5, lane)));
         } else {
             rv[lane] = S(0);
             Rg[lane] = S(1);
         }
     }
-    if (term && lane < NX) c0[lane] = x0[lane] - xs[lane];
+    if (term && lane < NX) c0[lane] = x0l - xsl;
     wave_sync();
 }
 
@@ -858,8 +869,12 @@ __device__ __forceinline__ void invert_q_knot(int lane, bool has_r, S rho, const
                                               const S* Rg, S* Qip, S* Qiv, S* Ri, S* piv) {
     constexpr int NQ = Plant<PL>::NQ;
     S row[NQ];
+    const int lr = lane < NQ ? lane : 0;  // clamped: the row is loaded unconditionally, then selected
 #pragma unroll
-    for (int c = 0; c < NQ; c++) row[c] = lane < NQ ? Qp[lane * NQ + c] + (c == lane ? rho : S(0)) : S(0);
+    for (int c = 0; c < NQ; c++) {
+        const S t = Qp[lr * NQ + c];
+        row[c] = lane < NQ ? t + (c == lane ? rho : S(0)) : S(0);
+    }
     gj_inplace<NQ>(lane, row, piv);
     if (lane < NQ) {
 #pragma unroll
@@ -1061,14 +1076,21 @@ __device__ __forceinline__ void schur_row0(int lane, S rho, const S* Qp0, const 
     S g = c0[r];
     S qir[NX];
     qinv_row<PL, S>(Qip0, Qiv0, r, qir);
+    // the blocks' entries loaded unconditionally from clamped indices, then selected (a load under the branches
+    // below compiles to a branch that waits for its own load)
+    const int rq = r < NQ ? r : 0, rv = (r >= NQ && r < NX) ? r - NQ : 0;
+    S qprow[NQ];
+#pragma unroll
+    for (int m = 0; m < NQ; m++) qprow[m] = Qp0[rq * NQ + m];
+    const S qvr = Qv0[rv];
 #pragma unroll
     for (int m = 0; m < NX; m++) {
         const S qi = qir[m];
         g -= qi * q0[m];
         Sd0[r * NX + m] = -qi;
         S qt = S(0);
-        if (r < NQ && m < NQ) qt = Qp0[r * NQ + m] + (r == m ? rho : S(0));
-        else if (r >= NQ && m == r) qt = Qv0[r - NQ];
+        if (r < NQ && m < NQ) qt = qprow[m < NQ ? m : 0] + (r == m ? rho : S(0));
+        else if (r >= NQ && m == r) qt = qvr;
         Pd0[r * NX + m] = -qt;
     }
     g0[r] = g;
@@ -1197,23 +1219,49 @@ __device__ __forceinline__ S merit_item(int k, int N, S alpha, S dt, const CostW
     using P = Plant<PL>;
     constexpr int NQ = P::NQ, NX = P::NX, NU = P::NU;
     const bool last = (k == N - 1);
-    S q[NQ], qd[NQ], u[NQ];
+    // Every operand is loaded up front from an address that is valid for every item (the last knot has no control
+    // and no successor in the trajectory, so its control and successor reads are pointed at its own state and the
+    // values discarded; an absent joint reference reads Qd instead and is discarded): loads under conditions
+    // compile to branches that each wait for their own loads.
+    const S* xu_u = xuk + (last ? 0 : NX);
+    const S* dz_u = dzk + (last ? 0 : NX);
+    const S* xn = xuk + (last ? 0 : NX + NU);
+    const S* dn = dzk + (last ? 0 : NX + NU);
+    const S* xr = xrefk ? xrefk : Qd;
+    S q[NQ], qd[NQ], u[NQ], xrv[NX], qdw[NX], rdw[NQ], xnv[NX], dnv[NX], x0v[NX], dz0v[NX], xsv[NX], fe[6], rf[3];
+#pragma unroll
+    for (int i = 0; i < NX; i++) {
+        xrv[i] = xr[i];
+        qdw[i] = Qd[i];
+        xnv[i] = xn[i];
+        dnv[i] = dn[i];
+        x0v[i] = x0[i];
+        dz0v[i] = dz0[i];
+        xsv[i] = xs[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NQ; i++) rdw[i] = Rd[i];
+#pragma unroll
+    for (int i = 0; i < 6; i++) fe[i] = fext[i];
+#pragma unroll
+    for (int i = 0; i < 3; i++) rf[i] = refk[i];
 #pragma unroll
     for (int i = 0; i < NQ; i++) {
         q[i] = xuk[i] + alpha * dzk[i];
         qd[i] = xuk[NQ + i] + alpha * dzk[NQ + i];
-        u[i] = last ? S(0) : xuk[NX + i] + alpha * dzk[NX + i];
+        const S uu = xu_u[i] + alpha * dz_u[i];
+        u[i] = last ? S(0) : uu;
     }
     S cost = S(0);
 #pragma unroll
     for (int i = 0; i < NQ; i++) {
-        const S e = qd[i] - (xrefk ? xrefk[NQ + i] : S(0));
-        const S eq = q[i] - (xrefk ? xrefk[i] : S(0));
-        cost += S(0.5) * Qd[NQ + i] * e * e + S(0.5) * Qd[i] * eq * eq;
+        const S e = qd[i] - (xrefk ? xrv[NQ + i] : S(0));
+        const S eq = q[i] - (xrefk ? xrv[i] : S(0));
+        cost += S(0.5) * qdw[NQ + i] * e * e + S(0.5) * qdw[i] * eq * eq;
         cost += S(cw.q_lim) * barrier(q[i], P::template qlo<S>(i), P::template qhi<S>(i));
         cost += S(cw.vel_lim) * barrier(qd[i], P::template vlo<S>(i), P::template vhi<S>(i));
         if (!last) {
-            cost += S(0.5) * Rd[i] * u[i] * u[i];
+            cost += S(0.5) * rdw[i] * u[i] * u[i];
             cost += S(cw.ctrl_lim) * barrier(u[i], P::template ulo<S>(i), P::template uhi<S>(i));
         }
     }
@@ -1224,24 +1272,22 @@ __device__ __forceinline__ S merit_item(int k, int N, S alpha, S dt, const CostW
     const S w = S(last ? cw.N_cost : cw.q_cost);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        const S e = p[i] - refk[i];
+        const S e = p[i] - rf[i];
         cost += S(0.5) * w * e * e;
     }
     S cons = S(0);
     if (!last) {
         S qdd[NQ], L[NQ * (NQ + 1) / 2];
-        forward_dynamics<PL>(K, qd, u, fext, qdd, L);
-        const S* xn = xuk + NX + NU;
-        const S* dn = dzk + NX + NU;
+        forward_dynamics<PL>(K, qd, u, fe, qdd, L);
 #pragma unroll
         for (int i = 0; i < NQ; i++) {
-            const S qn = xn[i] + alpha * dn[i], qdn = xn[NQ + i] + alpha * dn[NQ + i];
+            const S qn = xnv[i] + alpha * dnv[i], qdn = xnv[NQ + i] + alpha * dnv[NQ + i];
             cons += sabs(qn - (q[i] + dt * qd[i] + S(0.5) * qdd[i] * dt * dt));
             cons += sabs(qdn - (qd[i] + dt * qdd[i]));
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < NX; i++) cons += sabs(x0[i] + alpha * dz0[i] - xs[i]);
+        for (int i = 0; i < NX; i++) cons += sabs(x0v[i] + alpha * dz0v[i] - xsv[i]);
     }
     return cost + mu * cons;
 }
