@@ -179,20 +179,34 @@ __device__ __forceinline__ S btd_row(const S* D, const S* L, const S* x, int N, 
     constexpr int XP = 16;
     const S* Dr = D + (size_t)k * NX * NX + l * NX;
     const S* xk = x + (k + 1) * XP;
+    // the neighbour blocks' entries are loaded from addresses inside the batch's blocks for every k (knot 0's left
+    // block and the last knot's own block stand in) with all loads in flight at once; value barriers keep the
+    // compiler from sinking them into the conditional sums, where each would wait for its own load
+    const S* Lr = L + (size_t)k * NX * NX + l * NX;
+    const S* Lc = L + (size_t)(k + 1 < N ? k + 1 : k) * NX * NX + l;
+    S lr[NX], lc[NX];
+#pragma unroll
+    for (int c = 0; c < NX; c++) {
+        lr[c] = Lr[c];
+        lc[c] = Lc[c * NX];
+    }
     S s = S(0);
 #pragma unroll
     for (int c = 0; c < NX; c++) s += Dr[c] * xk[c];
+#pragma unroll
+    for (int c = 0; c < NX; c++) {
+        lr[c] = opaque(lr[c]);
+        lc[c] = opaque(lc[c]);
+    }
     if (k > 0) {
-        const S* Lr = L + (size_t)k * NX * NX + l * NX;
         const S* xm = x + k * XP;
 #pragma unroll
-        for (int c = 0; c < NX; c++) s += Lr[c] * xm[c];
+        for (int c = 0; c < NX; c++) s += lr[c] * xm[c];
     }
     if (k < N - 1) {
-        const S* Lc = L + (size_t)(k + 1) * NX * NX + l;
         const S* xp = x + (k + 2) * XP;
 #pragma unroll
-        for (int c = 0; c < NX; c++) s += Lc[c * NX] * xp[c];
+        for (int c = 0; c < NX; c++) s += lc[c] * xp[c];
     }
     return s;
 }

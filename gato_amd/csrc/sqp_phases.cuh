@@ -868,13 +868,16 @@ template <int PL, typename S>
 __device__ __forceinline__ void invert_q_knot(int lane, bool has_r, S rho, const S* Qp, const S* Qv,
                                               const S* Rg, S* Qip, S* Qiv, S* Ri, S* piv) {
     constexpr int NQ = Plant<PL>::NQ;
-    S row[NQ];
-    const int lr = lane < NQ ? lane : 0;  // clamped: the row is loaded unconditionally, then selected
+    S row[NQ], t[NQ];
+    // the row is loaded from a clamped index with every load in flight at once, then selected: the value barrier
+    // (opaque) keeps the compiler from sinking each load into a lane-conditional branch that waits for it
+    const int lr = lane < NQ ? lane : 0;
 #pragma unroll
-    for (int c = 0; c < NQ; c++) {
-        const S t = Qp[lr * NQ + c];
-        row[c] = lane < NQ ? t + (c == lane ? rho : S(0)) : S(0);
-    }
+    for (int c = 0; c < NQ; c++) t[c] = Qp[lr * NQ + c];
+#pragma unroll
+    for (int c = 0; c < NQ; c++) t[c] = opaque(t[c]);
+#pragma unroll
+    for (int c = 0; c < NQ; c++) row[c] = lane < NQ ? t[c] + (c == lane ? rho : S(0)) : S(0);
     gj_inplace<NQ>(lane, row, piv);
     if (lane < NQ) {
 #pragma unroll
@@ -1082,7 +1085,9 @@ __device__ __forceinline__ void schur_row0(int lane, S rho, const S* Qp0, const 
     S qprow[NQ];
 #pragma unroll
     for (int m = 0; m < NQ; m++) qprow[m] = Qp0[rq * NQ + m];
-    const S qvr = Qv0[rv];
+    const S qvr = opaque(Qv0[rv]);
+#pragma unroll
+    for (int m = 0; m < NQ; m++) qprow[m] = opaque(qprow[m]);  // not sunk into the branches below
 #pragma unroll
     for (int m = 0; m < NX; m++) {
         const S qi = qir[m];
@@ -1176,6 +1181,12 @@ __device__ __forceinline__ S dz_knot(int k, int N, int lane, const S* A, const S
 #pragma unroll
     for (int l = 0; l < NQ; l++) qrow[l] = Qip[lq * NQ + l];
     const S qv = q[lx], l0 = lam0[lx], cv = c[lx], qiv = Qiv[lv], rrv = rr[lu], ri = Ri[lu];
+    // value barriers: the loads stay here, all in flight at once, instead of being sunk into the conditional uses
+#pragma unroll
+    for (int i = 0; i < NX; i++) {
+        acol[i] = opaque(acol[i]);
+        bcol[i] = opaque(bcol[i]);
+    }
     S res = S(0);
     if (lane < NX) {
         S v = qv - l0;
