@@ -1090,25 +1090,31 @@ __device__ __forceinline__ void phase_ls(const ArgsT<S>& a, int b, int it, const
     constexpr int NX = Plant<PL>::NX, NU = Plant<PL>::NU;
     View<PL, S> v(a, b);
     if (threadIdx.x == 0) {
+        // the problem's scalars read once, together (interleaved with the stores below, each read would be a round
+        // trip of its own: the compiler cannot tell that the arrays do not overlap)
+        const S phi0 = a.phi[b], drho0 = a.drho[b], rho0 = a.rho[b];
         S best = S(1e38f);
         int jb = 0;
         for (int j = 0; j < NUM_ALPHAS; j++)
             if (merits[j] < best) { best = merits[j]; jb = j; }
-        const bool ok = best < a.phi[b];
+        const bool ok = best < phi0;
+        S rho = rho0, phi = phi0;
         if (a.adapt_rho) {
-            const S mult = ok ? smin(a.drho[b] / S(RHO_FACTOR), S(1) / S(RHO_FACTOR)) : smax(a.drho[b] * S(RHO_FACTOR), S(RHO_FACTOR));
+            const S mult = ok ? smin(drho0 / S(RHO_FACTOR), S(1) / S(RHO_FACTOR)) : smax(drho0 * S(RHO_FACTOR), S(RHO_FACTOR));
             a.drho[b] = mult;
-            a.rho[b] = smin(smax(a.rho[b] * mult, S(RHO_MIN)), S(RHO_MAX));
+            rho = smin(smax(rho0 * mult, S(RHO_MIN)), S(RHO_MAX));
         }
         S step = -S(1);
         if (!ok) {
-            if (a.rho[b] > S(RHO_MAX)) a.rho[b] = S(RHO_INIT);
+            if (rho > S(RHO_MAX)) rho = S(RHO_INIT);
         } else {
             step = S(1) / (S)(1 << jb);
-            a.phi[b] = best;
+            phi = best;
         }
+        a.rho[b] = rho;
+        a.phi[b] = phi;
         a.st_step[it * a.B + b] = step;
-        a.st_lsm[it * a.B + b] = a.phi[b];
+        a.st_lsm[it * a.B + b] = phi;
         shs[0] = step;
     }
     __syncthreads();
@@ -1211,13 +1217,16 @@ template <int PL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_ls_lin_schur_n(Args a, int it, int ts) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
-    if (stopped(a)) return;
-    if ((double)a.solved[it - 1] >= (double)a.B * (double)a.solve_ratio) {
+    // the batch's and the problem's flags read together, ahead of the tests that use them
+    const int stop = a.stop ? *a.stop : 0, solved = a.solved[it - 1], act = a.active[b];
+    const float mu = a.mu[b];
+    if (stop) return;
+    if ((double)solved >= (double)a.B * (double)a.solve_ratio) {
         if (b == 0 && threadIdx.x == 0) *a.stop_w = 1;
         return;
     }
-    if (a.exit_rule == GATO_EXIT_KKT && !a.active[b]) return;
-    phase_merit<PL>(a, b, a.mu[b], NUM_ALPHAS, sm, sm + 2 * NUM_ALPHAS);
+    if (a.exit_rule == GATO_EXIT_KKT && !act) return;
+    phase_merit<PL>(a, b, mu, NUM_ALPHAS, sm, sm + 2 * NUM_ALPHAS);
     phase_ls<PL>(a, b, it - 1, sm, sm + NUM_ALPHAS);
     __syncthreads();  // the updated trajectory and rho are settled, and the LDS is free for the linearization
     phase_lin<PL>(a, b, sm, ts);
@@ -1382,11 +1391,14 @@ template <int PL, int NPASS>
 __device__ __forceinline__ void pcg_dz_body(const Args& a, int it) {
     extern __shared__ __align__(16) float sm[];
     const int b = blockIdx.x;
-    if (stopped(a)) return;
-    const bool skip = (a.exit_rule == GATO_EXIT_KKT) ? !a.active[b] : a.conv[b];  // quirk Q9 (pcg.cuh:30-33)
-    const bool frozen = a.exit_rule == GATO_EXIT_KKT && !a.active[b];
+    // the problem's flags and tolerance read together, ahead of the tests that use them
+    const int stop = a.stop ? *a.stop : 0, act = a.active[b], cnv = a.conv[b];
+    const float tol = a.tol[b];
+    if (stop) return;
+    const bool skip = (a.exit_rule == GATO_EXIT_KKT) ? !act : cnv;  // quirk Q9 (pcg.cuh:30-33)
+    const bool frozen = a.exit_rule == GATO_EXIT_KKT && !act;
     int n = 0;
-    if (!skip) n = pcg_dispatch<PL, NPASS>(a, b, a.tol[b], sm);
+    if (!skip) n = pcg_dispatch<PL, NPASS>(a, b, tol, sm);
     float kr = 0.f;
     if (!frozen) {
         __syncthreads();  // lambda (global) and the LDS of the PCG are settled for the dz phase
