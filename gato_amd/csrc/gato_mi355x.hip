@@ -493,10 +493,7 @@ __device__ __forceinline__ int phase_pcg_res(const Args& a, int b, float eps, fl
         const int k = kk[s];
         const float* Uc = v.So + (size_t)(k + 1 < N ? k + 1 : k) * NB + l;
 #pragma unroll
-        for (int c = 0; c < NX; c++) {
-            const float t = Uc[c * NX];  // in bounds for every k < N: loaded unconditionally, selected below
-            uc[s][c] = k < N - 1 ? t : 0.f;
-        }
+        for (int c = 0; c < NX; c++) uc[s][c] = k < N - 1 ? Uc[c * NX] : 0.f;
     }
     // the off-diagonal blocks of block row kk[s] times a vector whose entries at this lane's index in knots k-1 and k+1
     // are ym and yp: row l of So_k (from LDS; zeros for k = 0), then column l of So_{k+1} (registers), from acc

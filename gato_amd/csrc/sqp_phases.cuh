@@ -1181,12 +1181,6 @@ __device__ __forceinline__ S dz_knot(int k, int N, int lane, const S* A, const S
 #pragma unroll
     for (int l = 0; l < NQ; l++) qrow[l] = Qip[lq * NQ + l];
     const S qv = q[lx], l0 = lam0[lx], cv = c[lx], qiv = Qiv[lv], rrv = rr[lu], ri = Ri[lu];
-    // value barriers: the loads stay here, all in flight at once, instead of being sunk into the conditional uses
-#pragma unroll
-    for (int i = 0; i < NX; i++) {
-        acol[i] = opaque(acol[i]);
-        bcol[i] = opaque(bcol[i]);
-    }
     S res = S(0);
     if (lane < NX) {
         S v = qv - l0;
