@@ -1561,7 +1561,7 @@ __device__ __forceinline__ void fused_problem(const Args& a, int b, float* sm, f
         __syncthreads();
         int it = 0;
         for (; it < a.max_sqp; it++) {
-            phase_lin<PL>(a, b, sm, blockDim.x / KG);
+            phase_lin<PL>(a, b, sm, fused_lin_slots(a.N, blockDim.x / KG));
             __syncthreads();
             phase_schur<PL>(a, b, a.rho[b], sm);
             __syncthreads();
@@ -2708,6 +2708,8 @@ size_t pcg_lds_bytes(const gato_solver* s) {
 // PCG's (pcg_dispatch_fused) where that is larger
 size_t fused_lds_bytes(const gato_solver* s, int G) {
     size_t f = lds_bytes_g(s, G);
+    const int lin = s->plant == GATO_PLANT_IIWA14 ? gato::lin_lds_floats<0>() : gato::lin_lds_floats<1>();
+    f = std::max(f, (size_t)gato::fused_lin_slots(s->N, G) * lin * sizeof(float));  // the linearization's chunk
     const int np = passes(s->N, G);
     if (np == 2 || np == 4) {
         const int r = s->plant == GATO_PLANT_IIWA14 ? gato::pcg_res_lds_floats<0>(s->N) : gato::pcg_res_lds_floats<1>(s->N);

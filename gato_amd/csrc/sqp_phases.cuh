@@ -624,6 +624,14 @@ __host__ __device__ constexpr int lin_lds_floats() {
     return 24 * NQ + 9 * (NQ * (NQ + 1) / 2) + 8;
 }
 
+// Knots per chunk of the linearization in the fused kernels (k_fused, k_fused_mixed): up to 32 whatever the knot-group
+// count G, so that a 256-lane work-group (G = 16) runs the shared dynamics of all knots of an N = 32 problem in one
+// chunk (one single-wave latency instead of two); at most 64 (the chunk's lanes of lin_dyn), at most N.
+__host__ __device__ constexpr int fused_lin_slots(int N, int G) {
+    const int t = G > 32 ? G : 32;
+    return N < t ? N : t;
+}
+
 // The dynamics of knot k <= N-2 that every column of its linearization shares, by ONE lane: qdd = M^-1 (u - c) with
 // the Cholesky factor of M, the RNEA table at (q, qd, qdd) and the transported axes, written to the knot's LDS table
 // shg (lin_lds_floats). xk points at [x_k u_k].
@@ -676,17 +684,9 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const Cos
         const S ui = xk[NX + i];  // xk is a valid knot for the terminal block too: loaded, then discarded there
         u[i] = term ? S(0) : ui;
     }
-    // the lane's operands of the integrator error and the cost model, loaded up front from clamped lane indices (a
-    // load under a lane or knot condition compiles to a branch that waits for its own load)
+    // the lane's operands of the integrator error and of the cost model are loaded from clamped lane indices right
+    // before each section (a load under a lane or knot condition compiles to a branch that waits for its own load)
     const int lq = lane < NQ ? lane : 0, lx = lane < NX ? lane : 0;
-    const S* xr = xrefk ? xrefk : Qd;
-    const S ql_ = xsrc[lq], qdl_ = xsrc[NQ + lq], xnq = xk[NX + NU + lq], xnv = xk[NX + NU + NQ + lq];
-    const S rf0 = refk[0], rf1 = refk[1], rf2 = refk[2];
-    const S qdw_q = Qd[lq], qdw_v = Qd[NQ + lq], rdw = Rd[lq], xrq = xr[lq], xrv = xr[NQ + lq], ul_ = xk[NX + lq];
-    const S x0l = x0[lx], xsl = xs[lx];
-    S xsm[NQ];
-#pragma unroll
-    for (int m = 0; m < NQ; m++) xsm[m] = xsrc[m];
     Kin<PL, S> K;
     K.set(q);
     // knot-group LDS: RNEA table, Cholesky factor of M (identical in every lane of the group,
@@ -744,6 +744,7 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const Cos
                 Bm[(NQ + r) * NU + lane] = dt * mc[r];
             }
         }
+        const S ql_ = xsrc[lq], qdl_ = xsrc[NQ + lq], xnq = xk[NX + NU + lq], xnv = xk[NX + NU + NQ + lq];
         if (lane < NQ) {  // c_{k+1} = x_{k+1} - x+ (integrator.cuh:49-62)
             const S ql = ql_, qdl = qdl_, al = pick<S[NQ], S>(qdd, NQ, lane);
             cn[lane] = xnq - (ql + dt * qdl + S(0.5) * al * dt * dt);
@@ -754,6 +755,14 @@ __device__ __forceinline__ void lin_knot(int k, int N, int lane, S dt, const Cos
     // Lane scalars are re-read from memory and the kinematics restart from opaque copies, so nothing of
     // the dynamics block above stays live into this section.
     const S w = S(term ? (cw.terminal_mode ? cw.N_cost : cw.q_cost) : cw.q_cost);
+    const S* xr = xrefk ? xrefk : Qd;
+    const S ql_ = xsrc[lq], qdl_ = xsrc[NQ + lq];
+    const S rf0 = refk[0], rf1 = refk[1], rf2 = refk[2];
+    const S qdw_q = Qd[lq], qdw_v = Qd[NQ + lq], rdw = Rd[lq], xrq = xr[lq], xrv = xr[NQ + lq], ul_ = xk[NX + lq];
+    const S x0l = x0[lx], xsl = xs[lx];
+    S xsm[NQ];
+#pragma unroll
+    for (int m = 0; m < NQ; m++) xsm[m] = xsrc[m];
     S p[3], Jc[3];
     {
         Kin<PL, S> K2;
@@ -1228,21 +1237,16 @@ __device__ __forceinline__ S merit_item(int k, int N, S alpha, S dt, const CostW
     // and no successor in the trajectory, so its control and successor reads are pointed at its own state and the
     // values discarded; an absent joint reference reads Qd instead and is discarded): loads under conditions
     // compile to branches that each wait for their own loads.
+    // (each group of operands is loaded right before the code that uses it, so that none stays live across the
+    // forward dynamics)
     const S* xu_u = xuk + (last ? 0 : NX);
     const S* dz_u = dzk + (last ? 0 : NX);
-    const S* xn = xuk + (last ? 0 : NX + NU);
-    const S* dn = dzk + (last ? 0 : NX + NU);
     const S* xr = xrefk ? xrefk : Qd;
-    S q[NQ], qd[NQ], u[NQ], xrv[NX], qdw[NX], rdw[NQ], xnv[NX], dnv[NX], x0v[NX], dz0v[NX], xsv[NX], fe[6], rf[3];
+    S q[NQ], qd[NQ], u[NQ], xrv[NX], qdw[NX], rdw[NQ], fe[6], rf[3];
 #pragma unroll
     for (int i = 0; i < NX; i++) {
         xrv[i] = xr[i];
         qdw[i] = Qd[i];
-        xnv[i] = xn[i];
-        dnv[i] = dn[i];
-        x0v[i] = x0[i];
-        dz0v[i] = dz0[i];
-        xsv[i] = xs[i];
     }
 #pragma unroll
     for (int i = 0; i < NQ; i++) rdw[i] = Rd[i];
@@ -1284,6 +1288,14 @@ __device__ __forceinline__ S merit_item(int k, int N, S alpha, S dt, const CostW
     if (!last) {
         S qdd[NQ], L[NQ * (NQ + 1) / 2];
         forward_dynamics<PL>(K, qd, u, fe, qdd, L);
+        const S* xn = xuk + NX + NU;
+        const S* dn = dzk + NX + NU;
+        S xnv[NX], dnv[NX];
+#pragma unroll
+        for (int i = 0; i < NX; i++) {
+            xnv[i] = xn[i];
+            dnv[i] = dn[i];
+        }
 #pragma unroll
         for (int i = 0; i < NQ; i++) {
             const S qn = xnv[i] + alpha * dnv[i], qdn = xnv[NQ + i] + alpha * dnv[NQ + i];
@@ -1292,7 +1304,7 @@ __device__ __forceinline__ S merit_item(int k, int N, S alpha, S dt, const CostW
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < NX; i++) cons += sabs(x0v[i] + alpha * dz0v[i] - xsv[i]);
+        for (int i = 0; i < NX; i++) cons += sabs(x0[i] + alpha * dz0[i] - xs[i]);
     }
     return cost + mu * cons;
 }
