@@ -1247,11 +1247,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         __syncthreads();  // the line search has read dz
     }
     for (int i = threadIdx.x; i < a.traj + NU; i += blockDim.x) v.dz[i] = 0.f;
-    __syncthreads();
-    phase_merit<PL>(a, b, a.mu[b], 1, sm, sm + 2 * NUM_ALPHAS);
+    // The final merit (bsqp.cuh:179-182) at the returned trajectory with dz = 0 is phi: the line search stored the merit
+    // of the point it accepted, whose every entry it formed as the trajectory update does (z + alpha dz, one fma), and
+    // left phi unchanged when it kept the trajectory; the solve start's phi is the merit at the initial trajectory.
+    // Reading it back skips a merit pass with the same result.
     if (threadIdx.x == 0) {
-        a.phi[b] = sm[0];
-        a.st_final[b] = sm[0];
+        a.st_final[b] = a.phi[b];
         a.drho[b] = a.drho_init[b];  // bsqp.cuh:189
     }
 }
