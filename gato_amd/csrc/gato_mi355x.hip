@@ -1594,10 +1594,9 @@ __device__ __forceinline__ void fused_problem(const Args& a, int b, float* sm, f
             }
         }
         for (int i = threadIdx.x; i < a.traj + NU; i += blockDim.x) v.dz[i] = 0.f;
-        __syncthreads();
-        phase_merit<PL>(a, b, mu, 1, shs, sm);
+        // the final merit at the returned trajectory is phi, as at the end of the graph solve (k_last_n)
         if (threadIdx.x == 0) {
-            a.st_final[b] = shs[0];
+            a.st_final[b] = a.phi[b];
             a.drho[b] = a.drho_init[b];
         }
         __syncthreads();
