@@ -1468,6 +1468,10 @@ template <typename S = float>
 __global__ void k_end(ArgsT<S> a) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= a.B) return;
+    // The final merit (bsqp.cuh:179-182) at the returned trajectory with dz = 0 is phi: the line search stored the merit
+    // of the point it accepted, whose every entry it formed as the trajectory update does (z + alpha dz, one fma), and
+    // left phi unchanged when it kept the trajectory; the solve start's phi is the merit at the initial trajectory.
+    a.st_final[b] = a.phi[b];
     a.drho[b] = a.drho_init[b];  // bsqp.cuh:189
 }
 
@@ -2795,8 +2799,7 @@ void enqueue_graph_solve(gato_solver* s, hipStream_t st) {
         launch_merit<PL>(B, blk, lds, st, a, gato::NUM_ALPHAS, 3, it);
     }
     hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, nz);
-    launch_merit<PL>(B, blk, lds, st, a, 1, 1, 0);
-    hipLaunchKernelGGL(gato::k_end, dim3(sb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(gato::k_end, dim3(sb), dim3(256), 0, st, a);  // with the final merit (phi)
 }
 
 void drop_graph(gato_solver* s) {
@@ -2893,8 +2896,7 @@ int launch_solve(gato_solver* s, hipStream_t st) {
         const size_t n = (size_t)B * (s->traj + s->nu);
         hipLaunchKernelGGL(gato::k_zero, dim3(256), dim3(256), 0, st, a.dz, n);
     }
-    launch_merit<PL>(B, blk, lds, st, a, 1, 1, 0);
-    hipLaunchKernelGGL(gato::k_end, dim3(sb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(gato::k_end, dim3(sb), dim3(256), 0, st, a);  // with the final merit (phi)
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
     s->last_kernel_us = pcg_launches ? pcg_ms * 1e3 / pcg_launches : 0.0;
@@ -3257,8 +3259,7 @@ int launch_solve64(gato_solver64* s, hipStream_t st) {
         HIP_TRY(hipGetLastError());
     }
     hipLaunchKernelGGL(gato::k_zero<double>, dim3(256), dim3(256), 0, st, a.dz, nz);
-    hipLaunchKernelGGL((gato::k_merit<PL, double>), dim3(B), blk, lds, st, a, 1, 1, 0);
-    hipLaunchKernelGGL(gato::k_end<double>, dim3(sb), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(gato::k_end<double>, dim3(sb), dim3(256), 0, st, a);  // with the final merit (phi)
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
     return 0;
